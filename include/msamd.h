@@ -1,0 +1,170 @@
+/*
+ * msamd.h -- C ABI of the MI355X (gfx950) MountainSort front end.
+ *
+ * This is the drop-in seam beneath the MountainLab processors
+ * ms3.bandpass_filter, ms3.mask_out_artifacts, ms3.whiten, ms3.extract_clips
+ * and the detect / clip / PCA phase of mountainsortalg.ms3alg.  Each entry
+ * point names the reference function it replaces (paths relative to the
+ * MountainSort repository root).
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer unless its name ends in _host.
+ *  - Timeseries use the .mda column-major layout of the reference: an M x N
+ *    array stores sample (m, t) at index m + M*t (channels interleaved per
+ *    time sample, packages/mountainsortalg/mda/mda32.h:91-96).
+ *  - Clips W x T x L store (w, t, i) at w + W*t + W*T*i.
+ *  - Buffers are caller-owned.  Scratch memory comes in through a workspace
+ *    argument whose size is reported by the matching *_workspace_size call.
+ *  - stream is a hipStream_t (NULL = default stream).  Calls are
+ *    asynchronous unless documented otherwise.
+ *  - Return value: MSAMD_OK (0) or an MSAMD_ERR_* code; msamd_last_error()
+ *    returns the message of the last failure on the calling thread.  No C++
+ *    exception crosses this boundary.
+ */
+#ifndef MSAMD_H
+#define MSAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSAMD_OK 0
+#define MSAMD_ERR_ARG 1
+#define MSAMD_ERR_HIP 2
+#define MSAMD_ERR_WORKSPACE 3
+#define MSAMD_ERR_CAPACITY 4
+#define MSAMD_ERR_UNSUPPORTED 5
+
+/* .mda element type codes (packages/ms3/mda/mdaio.h:55-62) accepted as input. */
+#define MSAMD_DTYPE_FLOAT32 (-3)
+#define MSAMD_DTYPE_INT16 (-4)
+
+/* Library identity. */
+int msamd_version(void);
+const char* msamd_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Event detection (a10) -- replaces detect_events()
+ * (packages/mountainsortalg/detect_events.cpp:69-135) as driven by
+ * NeighborhoodSorter::addTimeChunk (neighborhoodsorter.cpp:74-106) over the
+ * time-chunk plan of p_mountainsort3 (p_mountainsort3.cpp:170-216,507-533).
+ * ------------------------------------------------------------------------ */
+
+/* Chunk length of the detection plan for M channels, N timepoints and the
+ * reference thread count (p_mountainsort3.cpp:170-172,191,507-533). */
+int64_t msamd_detect_chunk_size(int64_t M, int64_t N, int64_t num_threads);
+
+/* Workspace bytes for msamd_detect_events on the time range of n_range
+ * samples and num_channels central channels. */
+int64_t msamd_detect_workspace_size(int64_t n_range, int64_t chunk_size, int32_t num_channels,
+                                    double detect_interval);
+
+/* Detects events on every central channel: per (channel, padded chunk) the
+ * mean and stdev are the reference's sequential double sums over the padded
+ * chunk (zero-filled outside the file), events are threshold crossings that
+ * win detect_interval suppression, and only core events are kept.
+ * X: M x N_file float32.  t1/t2: inclusive time range (t1 < 0 -> 0, t2 < 0 ->
+ * N_file-1), as the ms3alg t1/t2 parameters.  channels: num_channels 0-based
+ * channel indices (int32, device).  padding: 1000 in the reference.
+ * Synchronizes the stream once and returns the total event count in
+ * *num_events_host; the events stay in the workspace until
+ * msamd_detect_fetch. */
+int msamd_detect_events(const float* X, int64_t M, int64_t N_file, int64_t t1, int64_t t2,
+                        int64_t chunk_size, int64_t padding, const int32_t* channels,
+                        int32_t num_channels, double detect_threshold, double detect_interval,
+                        int detect_sign, void* workspace, int64_t workspace_bytes,
+                        int64_t* num_events_host, void* stream);
+
+/* Writes the events found by the last msamd_detect_events on this workspace:
+ * times (int64, absolute 0-based sample indices, ascending per channel,
+ * channels in list order) and offsets (num_channels+1 int64: events of
+ * channel c are times[offsets[c]..offsets[c+1])). */
+int msamd_detect_fetch(const void* workspace, int32_t num_channels, int64_t* times,
+                       int64_t* offsets, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Clip extraction (a11) -- replaces NeighborhoodSorterPrivate::get_clip
+ * (neighborhoodsorter.cpp:409-419) and the gather of p_extract_clips
+ * (packages/ms3/p_extract_clips.cpp:53-69).
+ * ------------------------------------------------------------------------ */
+
+/* clips[w + W*t + W*T*i] = X[channels[w] + M*(starts[i] + t)], 0 outside
+ * [0, N).  starts: L int64 first-sample indices (the processor form uses
+ * (int64)(event_time - Tmid), Tmid = (T+1)/2 - 1).  channels: W 0-based. */
+int msamd_extract_clips(const float* X, int64_t M, int64_t N, const int64_t* starts, int64_t L,
+                        const int32_t* channels, int32_t W, int32_t T, float* clips,
+                        void* stream);
+
+/* Clip gather for G neighborhoods in one launch.  Neighborhood g owns events
+ * times[event_offsets[g] .. event_offsets[g+1]) (centers; start = time -
+ * Tmid) and channels nbhd_channels[nbhd_offsets[g] .. nbhd_offsets[g+1]);
+ * its W_g x T x L_g clips are written at clips + clip_offsets[g].
+ * event_offsets, nbhd_offsets, clip_offsets: G+1 int64 (device). */
+int msamd_extract_clips_batched(const float* X, int64_t M, int64_t N, const int64_t* times,
+                                const int64_t* event_offsets, int64_t total_events,
+                                const int32_t* nbhd_channels, const int64_t* nbhd_offsets,
+                                int32_t G, int32_t T, const int64_t* clip_offsets, float* clips,
+                                void* stream);
+
+/* ------------------------------------------------------------------------
+ * Whitening (a4-a6) -- replaces p_whiten (packages/ms3/p_whiten.cpp:45-206)
+ * and whitening_matrix_from_XXt (packages/mountainsortalg/pca.cpp:634-650).
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for msamd_whiten_covariance. */
+int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N);
+
+/* XXt = X X^T / (N-1) (no mean subtraction; p_whiten.cpp:64-118).
+ * Split-BF16 3-term MFMA products (x = hi + lo; hi*hi + hi*lo + lo*hi)
+ * accumulated in FP32 per block of samples and in FP64 across blocks.
+ * XXt: M x M float64, symmetric.  M <= 1024. */
+int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt, void* workspace,
+                            int64_t workspace_bytes, void* stream);
+
+/* Workspace bytes for msamd_whitening_matrix. */
+int64_t msamd_whitening_matrix_workspace_size(int64_t M);
+
+/* W from XXt by the reference's method: per component 10 power steps from
+ * normalize(sin(1..M)), sigma = |G v|, deflation G <- (I-vv')G(I-vv'), then
+ * W = C diag(sigma^-1/2) C' (zero where sigma == 0).  FP64 on the GPU.
+ * XXt and W: M x M float64 (device). */
+int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* workspace,
+                           int64_t workspace_bytes, void* stream);
+
+/* Workspace bytes for msamd_whiten_apply. */
+int64_t msamd_whiten_apply_workspace_size(int64_t M);
+
+/* Y = quantize(W X, 1e-4) with quantize(x,u) = floor(x/u + 0.5)*u
+ * (p_whiten.cpp:31-34,153-206).  Split-BF16 3-term MFMA, FP32 accumulation.
+ * Y: M x N float32 (may be NULL when Yq is given).  When quantization_unit
+ * > 0, Yq receives the int16 values (int)(y/q + 0.5) (p_whiten.cpp:445-451). */
+int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,
+                       double quantization_unit, int16_t* Yq, void* workspace,
+                       int64_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Artifact masking (a7) -- replaces p_mask_out_artifacts
+ * (packages/ms3/p_mask_out_artifacts.cpp:24-104).
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for msamd_mask_out_artifacts. */
+int64_t msamd_mask_workspace_size(int64_t M, int64_t N, int64_t interval_size);
+
+/* Per (channel, interval) norms sqrt(sum x^2) (sequential double sums),
+ * per-channel mean/stdev of the norms, intervals whose norm exceeds
+ * mean + stdev*threshold on any channel are cleared together with both
+ * neighbours; kept intervals are copied and everything else (including the
+ * tail N mod interval_size) is zero.  X, Y: M x N float32.  use_it (may be
+ * NULL): N/interval_size + 1 int32 flags.  threshold == 0 or
+ * interval_size == 0 is an error, as in the reference. */
+int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, double threshold,
+                             int64_t interval_size, float* Y, int32_t* use_it, void* workspace,
+                             int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSAMD_H */

@@ -1,0 +1,116 @@
+// Clip extraction (a11): gather W x T windows of the timeseries at event times.
+//
+// Reference: NeighborhoodSorterPrivate::get_clip
+// (packages/mountainsortalg/neighborhoodsorter.cpp:409-419) and the clip loop
+// of p_extract_clips (packages/ms3/p_extract_clips.cpp:53-69), whose reads
+// are zero-filled outside the file (mda/diskreadmda32.cpp:631-650).
+//
+// One wavefront per clip: lanes walk the W*T outputs in storage order
+// (w fastest), so each clip is written as one contiguous run and each time
+// row of the source is read as a short run of nearby channels.  Pure copy:
+// results are bit-identical to the reference.
+#include "common.h"
+
+namespace msamd {
+namespace {
+
+__global__ void __launch_bounds__(256) clips_kernel(
+    const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ starts,
+    int64_t L, const int32_t* __restrict__ channels, int32_t W, int32_t T,
+    float* __restrict__ clips) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    const int32_t WT = W * T;
+    for (int64_t i = wave; i < L; i += nwaves) {
+        const int64_t t0 = starts[i];
+        float* out = clips + i * WT;
+        for (int32_t r = lane; r < WT; r += 64) {
+            const int32_t t = r / W;
+            const int32_t w = r - t * W;
+            const int64_t tt = t0 + t;
+            out[r] = (tt >= 0 && tt < N) ? X[tt * M + channels[w]] : 0.0f;
+        }
+    }
+}
+
+// Batched form: events of neighborhood g are times[eo[g]..eo[g+1]) (centers),
+// its channels nc[no[g]..no[g+1]), its clips start at clips + co[g].
+__global__ void __launch_bounds__(256) clips_batched_kernel(
+    const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ times,
+    const int64_t* __restrict__ eo, int64_t total, const int32_t* __restrict__ nc,
+    const int64_t* __restrict__ no, int32_t G, int32_t T, int32_t tmid,
+    const int64_t* __restrict__ co, float* __restrict__ clips) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    for (int64_t e = wave; e < total; e += nwaves) {
+        // neighborhood of global event e: last g with eo[g] <= e
+        int32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (eo[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int32_t g = lo;
+        const int64_t i = e - eo[g];
+        const int32_t* ch = nc + no[g];
+        const int32_t W = static_cast<int32_t>(no[g + 1] - no[g]);
+        const int32_t WT = W * T;
+        const int64_t t0 = times[e] - tmid;
+        float* out = clips + co[g] + i * WT;
+        for (int32_t r = lane; r < WT; r += 64) {
+            const int32_t t = r / W;
+            const int32_t w = r - t * W;
+            const int64_t tt = t0 + t;
+            out[r] = (tt >= 0 && tt < N) ? X[tt * M + ch[w]] : 0.0f;
+        }
+    }
+}
+
+inline unsigned grid_for_waves(int64_t nwaves) {
+    int64_t blocks = ceil_div(nwaves, 4);
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    return static_cast<unsigned>(blocks);
+}
+
+}  // namespace
+}  // namespace msamd
+
+using namespace msamd;
+
+extern "C" int msamd_extract_clips(const float* X, int64_t M, int64_t N, const int64_t* starts,
+                                   int64_t L, const int32_t* channels, int32_t W, int32_t T,
+                                   float* clips, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && N >= 0 && L >= 0, "extract_clips: bad shape");
+    MSAMD_REQUIRE(T >= 1, "extract_clips: clip_size is zero");
+    MSAMD_REQUIRE(W >= 1, "extract_clips: empty channel list");
+    if (L == 0) return MSAMD_OK;
+    clips_kernel<<<grid_for_waves(L), 256, 0, as_stream(stream)>>>(X, M, N, starts, L, channels,
+                                                                   W, T, clips);
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
+}
+
+extern "C" int msamd_extract_clips_batched(const float* X, int64_t M, int64_t N,
+                                           const int64_t* times, const int64_t* event_offsets,
+                                           int64_t total_events, const int32_t* nbhd_channels,
+                                           const int64_t* nbhd_offsets, int32_t G, int32_t T,
+                                           const int64_t* clip_offsets, float* clips,
+                                           void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && N >= 0 && G >= 1 && total_events >= 0,
+                  "extract_clips_batched: bad shape");
+    MSAMD_REQUIRE(T >= 1, "extract_clips_batched: clip_size is zero");
+    if (total_events == 0) return MSAMD_OK;
+    const int32_t tmid = (T + 1) / 2 - 1;
+    clips_batched_kernel<<<grid_for_waves(total_events), 256, 0, as_stream(stream)>>>(
+        X, M, N, times, event_offsets, total_events, nbhd_channels, nbhd_offsets, G, T, tmid,
+        clip_offsets, clips);
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
+}

@@ -1,0 +1,388 @@
+// Event detection (a10): per (central channel, padded time chunk) statistics
+// and the detect_interval suppression stencil.
+//
+// Reference: detect_events (packages/mountainsortalg/detect_events.cpp:69-135)
+// driven by NeighborhoodSorter::addTimeChunk (neighborhoodsorter.cpp:74-106)
+// over the chunk plan of p_mountainsort3.cpp:170-216,507-533.
+//
+// The reference's sequential candidate/last-best state machine keeps sample n
+// exactly when  v[n] > thr,  v[n] > v[u] for every u in [n-D, n-1]  and
+// v[n] >= v[u] for every u in [n+1, n+D]  (windows cut at the padded-chunk
+// ends), with v = x - mean (negated for sign < 0, |.| for sign == 0) and
+// thr = detect_threshold * stdev.  That closed form is a per-sample stencil,
+// evaluated here in parallel; only the two statistics need the reference's
+// sequential order, and they are computed that way (one thread per
+// (chunk, channel), double sums in time order) so that thresholds are
+// bit-identical.  Contraction is off in this file: every double operation
+// is rounded as on the reference's x86-64 build.
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace msamd {
+namespace {
+
+constexpr int kSeg = 1024;  // samples per scan segment (fits a uint16 offset)
+
+struct DetectPlan {
+    int64_t M, N_file, t_start, n_range, cs, pad, nchunks, spc, nseg;
+    int32_t Mc;
+    int D;         // floor(detect_interval), lags <= D are suppressed
+    int64_t cap;   // slot capacity per (segment, channel)
+};
+
+inline DetectPlan make_plan(int64_t M, int64_t N_file, int64_t t_start, int64_t n_range,
+                            int64_t cs, int64_t pad, int32_t Mc, double detect_interval) {
+    DetectPlan p{};
+    p.M = M;
+    p.N_file = N_file;
+    p.t_start = t_start;
+    p.n_range = n_range;
+    p.cs = cs;
+    p.pad = pad;
+    p.Mc = Mc;
+    p.nchunks = n_range > 0 ? ceil_div(n_range, cs) : 0;
+    p.spc = ceil_div(cs < n_range ? cs : n_range, kSeg);
+    if (p.spc < 1) p.spc = 1;
+    p.nseg = p.nchunks * p.spc;
+    double d = detect_interval;
+    p.D = d >= 0 ? static_cast<int>(d) : -1;  // reference compares lag > D in double
+    if (p.D > (1 << 20)) p.D = 1 << 20;
+    p.cap = p.D >= 1 ? (kSeg + p.D) / (p.D + 1) : kSeg;
+    return p;
+}
+
+struct DetectBuffers {
+    double* mean;      // [nchunks * Mc]
+    double* thr;       // [nchunks * Mc]
+    int32_t* counts;   // [Mc * nseg] (channel-major)
+    int64_t* seg_off;  // [Mc * nseg] exclusive scan per channel
+    int64_t* chan_off; // [Mc + 1]
+    uint16_t* slots;   // [nseg * Mc * cap]
+    DetectPlan* header;  // the plan, read back by msamd_detect_fetch
+};
+
+inline DetectBuffers carve(void* ws, const DetectPlan& p, int64_t* bytes) {
+    Carver c(ws);
+    DetectBuffers b{};
+    b.header = c.take<DetectPlan>(1);
+    b.mean = c.take<double>(p.nchunks * p.Mc);
+    b.thr = c.take<double>(p.nchunks * p.Mc);
+    b.counts = c.take<int32_t>(p.nseg * p.Mc);
+    b.seg_off = c.take<int64_t>(p.nseg * p.Mc);
+    b.chan_off = c.take<int64_t>(p.Mc + 1);
+    b.slots = c.take<uint16_t>(p.nseg * p.Mc * p.cap);
+    if (bytes) *bytes = c.bytes();
+    return b;
+}
+
+// One thread per (chunk, channel): the sequential double sums of
+// MLCompute::mean / MLCompute::stdev (common/mlcompute.cpp:62-85) over the
+// padded chunk.  Zero-filled samples outside the file add exact zeros and are
+// skipped.  Lanes map to consecutive channels, so each time step is one
+// coalesced row read.
+__global__ void __launch_bounds__(256) detect_stats_kernel(
+    const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels,
+    double k_thr, double* __restrict__ mean_out, double* __restrict__ thr_out) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= p.nchunks * p.Mc) return;
+    const int j = static_cast<int>(g % p.Mc);
+    const int64_t c = g / p.Mc;
+    int64_t size = p.n_range - c * p.cs;
+    if (size > p.cs) size = p.cs;
+    const int64_t len = size + 2 * p.pad;
+    const int64_t base = p.t_start + c * p.cs - p.pad;
+    int64_t a = base < 0 ? 0 : base;
+    int64_t b = base + len;
+    if (b > p.N_file) b = p.N_file;
+    const float* xp = X + channels[j];
+    double s = 0.0, ss = 0.0;
+    int64_t t = a;
+    constexpr int U = 16;
+    for (; t + U <= b; t += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(xp + (t + u) * p.M);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double d = static_cast<double>(v[u]);
+            s = s + d;
+            ss = ss + d * d;  // d*d is exact (24-bit mantissa squared)
+        }
+    }
+    for (; t < b; ++t) {
+        const double d = static_cast<double>(xp[t * p.M]);
+        s = s + d;
+        ss = ss + d * d;
+    }
+    const double mean = s / static_cast<double>(len);
+    const int ct = static_cast<int>(len);
+    const double sd = (ct >= 2) ? sqrt((ss - s * s / ct) / (ct - 1)) : 0.0;
+    mean_out[c * p.Mc + j] = mean;
+    thr_out[c * p.Mc + j] = k_thr * sd;
+}
+
+__device__ __forceinline__ double detect_value(float x, double mean, int sign) {
+    double v = static_cast<double>(x) - mean;
+    if (sign < 0)
+        v = -v;
+    else if (sign == 0)
+        v = fabs(v);
+    return v;
+}
+
+// One thread per (segment, channel): scan the segment's core samples, keep
+// the stencil winners, store their offsets in the slot list.
+__global__ void __launch_bounds__(256) detect_scan_kernel(
+    const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels, int sign,
+    const double* __restrict__ mean_in, const double* __restrict__ thr_in,
+    int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= p.nseg * p.Mc) return;
+    const int j = static_cast<int>(g % p.Mc);
+    const int64_t seg = g / p.Mc;
+    const int64_t c = seg / p.spc;
+    const int64_t s = seg % p.spc;
+    int64_t size = p.n_range - c * p.cs;
+    if (size > p.cs) size = p.cs;
+    const int64_t r0 = s * kSeg;
+    int64_t r1 = r0 + kSeg;
+    if (r1 > size) r1 = size;
+    int32_t cnt = 0;
+    if (r0 < r1) {
+        const double mean = mean_in[c * p.Mc + j];
+        const double thr = thr_in[c * p.Mc + j];
+        const int64_t core0 = p.t_start + c * p.cs;   // absolute index of core start
+        const int64_t wa = core0 - p.pad;             // padded window [wa, wb)
+        const int64_t wb = core0 + size + p.pad;
+        const float* xp = X + channels[j];
+        uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
+        auto load = [&](int64_t t) -> float {
+            return (t >= 0 && t < p.N_file) ? xp[t * p.M] : 0.0f;
+        };
+        constexpr int U = 8;
+        for (int64_t r = r0; r < r1; r += U) {
+            float xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t t = core0 + r + u;
+                xv[u] = (r + u < r1) ? load(t) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (r + u >= r1) break;
+                const int64_t t = core0 + r + u;
+                const double v = detect_value(xv[u], mean, sign);
+                if (!(v > thr)) continue;
+                bool keep = true;
+                for (int d = 1; d <= p.D && keep; ++d) {
+                    const int64_t tu = t - d;
+                    if (tu < wa) break;
+                    if (detect_value(load(tu), mean, sign) >= v) keep = false;
+                }
+                for (int d = 1; d <= p.D && keep; ++d) {
+                    const int64_t tu = t + d;
+                    if (tu >= wb) break;
+                    if (detect_value(load(tu), mean, sign) > v) keep = false;
+                }
+                // kept events are > D apart, so cnt < cap always holds
+                if (keep && cnt < p.cap) {
+                    my_slots[cnt] = static_cast<uint16_t>(r + u - r0);
+                    ++cnt;
+                }
+            }
+        }
+    }
+    counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
+}
+
+// One block per channel: exclusive scan of the per-segment counts.
+__global__ void __launch_bounds__(1024) detect_seg_scan_kernel(
+    const int32_t* __restrict__ counts, int64_t nseg, int64_t* __restrict__ seg_off,
+    int64_t* __restrict__ chan_total) {
+    __shared__ int64_t warp_sums[16];
+    __shared__ int64_t carry_s;
+    const int j = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = tid >> 6;
+    const int32_t* cj = counts + static_cast<int64_t>(j) * nseg;
+    int64_t* oj = seg_off + static_cast<int64_t>(j) * nseg;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < nseg; base += blockDim.x) {
+        const int64_t i = base + tid;
+        const int64_t v = (i < nseg) ? cj[i] : 0;
+        int64_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) warp_sums[wid] = x;
+        __syncthreads();
+        if (tid == 0) {
+            int64_t acc = 0;
+            for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+                const int64_t t = warp_sums[w];
+                warp_sums[w] = acc;
+                acc += t;
+            }
+        }
+        __syncthreads();
+        const int64_t carry = carry_s;
+        if (i < nseg) oj[i] = carry + warp_sums[wid] + x - v;
+        __syncthreads();
+        if (tid == blockDim.x - 1) carry_s = carry + warp_sums[wid] + x;
+        __syncthreads();
+    }
+    if (tid == 0) chan_total[j] = carry_s;
+}
+
+// Single block: exclusive scan over channels -> chan_off[0..Mc].
+__global__ void detect_chan_scan_kernel(int64_t* __restrict__ chan_off, int32_t Mc) {
+    if (threadIdx.x != 0) return;
+    int64_t acc = 0;
+    for (int32_t j = 0; j < Mc; ++j) {
+        const int64_t t = chan_off[j];
+        chan_off[j] = acc;
+        acc += t;
+    }
+    chan_off[Mc] = acc;
+}
+
+__global__ void __launch_bounds__(256) detect_compact_kernel(
+    DetectPlan p, const int32_t* __restrict__ counts, const int64_t* __restrict__ seg_off,
+    const int64_t* __restrict__ chan_off, const uint16_t* __restrict__ slots,
+    int64_t* __restrict__ times) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= p.nseg * p.Mc) return;
+    const int j = static_cast<int>(g % p.Mc);
+    const int64_t seg = g / p.Mc;
+    const int64_t idx = static_cast<int64_t>(j) * p.nseg + seg;
+    const int32_t n = counts[idx];
+    if (n == 0) return;
+    const int64_t c = seg / p.spc;
+    const int64_t s = seg % p.spc;
+    const int64_t t0 = p.t_start + c * p.cs + s * kSeg;
+    const int64_t o = chan_off[j] + seg_off[idx];
+    const uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
+    for (int32_t k = 0; k < n; ++k) times[o + k] = t0 + my_slots[k];
+}
+
+__global__ void detect_offsets_kernel(const int64_t* __restrict__ chan_off, int32_t Mc,
+                                      int64_t* __restrict__ offsets) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= Mc) offsets[i] = chan_off[i];
+}
+
+}  // namespace
+}  // namespace msamd
+
+using namespace msamd;
+
+extern "C" int64_t msamd_detect_chunk_size(int64_t M, int64_t N, int64_t num_threads) {
+    // p_mountainsort3.cpp:170-172 (simultaneous neighborhoods, batch count)
+    // and get_time_chunk_infos :507-533 with min_num_chunks = 1.
+    if (M < 1) M = 1;
+    int64_t nsim = num_threads < M ? num_threads : M;
+    if (nsim < 1) nsim = 1;
+    const int64_t num_batches = ceil_div(M, nsim);
+    const int64_t ram = 1000000000LL;
+    int64_t chunk = ram / (M * num_batches * 4);
+    if (chunk > N) chunk = N;
+    if (chunk < 20000) chunk = 20000;
+    return chunk;
+}
+
+extern "C" int64_t msamd_detect_workspace_size(int64_t n_range, int64_t chunk_size,
+                                               int32_t num_channels, double detect_interval) {
+    if (n_range < 0 || chunk_size < 1 || num_channels < 0) return -1;
+    const DetectPlan p = make_plan(1, n_range, 0, n_range, chunk_size, 0, num_channels,
+                                   detect_interval);
+    int64_t bytes = 0;
+    carve(nullptr, p, &bytes);
+    return bytes;
+}
+
+extern "C" int msamd_detect_events(const float* X, int64_t M, int64_t N_file, int64_t t1,
+                                   int64_t t2, int64_t chunk_size, int64_t padding,
+                                   const int32_t* channels, int32_t num_channels,
+                                   double detect_threshold, double detect_interval,
+                                   int detect_sign, void* workspace, int64_t workspace_bytes,
+                                   int64_t* num_events_host, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && N_file >= 0, "detect: bad shape M=%lld N=%lld", (long long)M,
+                  (long long)N_file);
+    MSAMD_REQUIRE(chunk_size >= 1 && padding >= 0, "detect: bad chunk plan");
+    MSAMD_REQUIRE(num_channels >= 0 && num_events_host, "detect: bad channel list");
+    const int64_t t_start = t1 < 0 ? 0 : t1;
+    const int64_t t_end = t2 < 0 ? N_file - 1 : t2;
+    const int64_t n_range = t_end - t_start + 1;
+    MSAMD_REQUIRE(n_range >= 0, "detect: empty or inverted time range");
+    const DetectPlan p = make_plan(M, N_file, t_start, n_range, chunk_size, padding,
+                                   num_channels, detect_interval);
+    int64_t need = 0;
+    DetectBuffers b = carve(workspace, p, &need);
+    if (workspace_bytes < need || !workspace) {
+        set_error("detect: workspace too small (%lld < %lld bytes)", (long long)workspace_bytes,
+                  (long long)need);
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    MSAMD_HIP(hipMemcpyAsync(b.header, &p, sizeof(DetectPlan), hipMemcpyHostToDevice, st));
+    if (num_channels == 0 || n_range == 0) {
+        MSAMD_HIP(hipMemsetAsync(b.chan_off, 0, sizeof(int64_t) * (num_channels + 1), st));
+        MSAMD_HIP(hipStreamSynchronize(st));
+        *num_events_host = 0;
+        return MSAMD_OK;
+    }
+    {
+        const int64_t n = p.nchunks * p.Mc;
+        detect_stats_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            X, p, channels, detect_threshold, b.mean, b.thr);
+        MSAMD_LAUNCH_CHECK();
+    }
+    {
+        const int64_t n = p.nseg * p.Mc;
+        detect_scan_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            X, p, channels, detect_sign, b.mean, b.thr, b.counts, b.slots);
+        MSAMD_LAUNCH_CHECK();
+    }
+    detect_seg_scan_kernel<<<dim3(p.Mc), 1024, 0, st>>>(b.counts, p.nseg, b.seg_off,
+                                                         b.chan_off);
+    MSAMD_LAUNCH_CHECK();
+    detect_chan_scan_kernel<<<1, 64, 0, st>>>(b.chan_off, p.Mc);
+    MSAMD_LAUNCH_CHECK();
+    int64_t total = 0;
+    MSAMD_HIP(hipMemcpyAsync(&total, b.chan_off + p.Mc, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             st));
+    MSAMD_HIP(hipStreamSynchronize(st));
+    *num_events_host = total;
+    return MSAMD_OK;
+}
+
+extern "C" int msamd_detect_fetch(const void* workspace, int32_t num_channels, int64_t* times,
+                                  int64_t* offsets, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(workspace != nullptr, "detect_fetch: null workspace");
+    hipStream_t st = as_stream(stream);
+    // The plan was stored at the head of the workspace by msamd_detect_events.
+    DetectPlan p{};
+    MSAMD_HIP(hipMemcpyAsync(&p, workspace, sizeof(DetectPlan), hipMemcpyDeviceToHost, st));
+    MSAMD_HIP(hipStreamSynchronize(st));
+    MSAMD_REQUIRE(p.Mc == num_channels, "detect_fetch: channel count %d does not match %d",
+                  num_channels, p.Mc);
+    DetectBuffers b = carve(const_cast<void*>(workspace), p, nullptr);
+    if (offsets) {
+        detect_offsets_kernel<<<dim3(static_cast<unsigned>(ceil_div(p.Mc + 1, 256))), 256, 0,
+                                st>>>(b.chan_off, p.Mc, offsets);
+        MSAMD_LAUNCH_CHECK();
+    }
+    const int64_t n = p.nseg * p.Mc;
+    if (times && n > 0) {
+        detect_compact_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            p, b.counts, b.seg_off, b.chan_off, b.slots, times);
+        MSAMD_LAUNCH_CHECK();
+    }
+    return MSAMD_OK;
+}

@@ -1,0 +1,580 @@
+// Whitening (a4-a6): covariance X X^T / (N-1), the whitening matrix W by the
+// reference's deflated power method, and Y = quantize(W X, 1e-4).
+//
+// Reference: p_whiten (packages/ms3/p_whiten.cpp:45-206) and
+// whitening_matrix_from_XXt / pca_from_XXt / iterate_XXt_to_get_top_component
+// / subtract_out_rank_1_from_XXt (packages/mountainsortalg/pca.cpp:171-193,
+// 408-430, 541-557, 634-650).
+//
+// Contractions run on MFMA as split-BF16 products: x = hi + lo with
+// hi = bf16(x), lo = bf16(x - hi); x*y ~ hi*hi' + hi*lo' + lo*hi' (the lo*lo'
+// term is dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The
+// covariance sums FP32 block partials in FP64; the whitening matrix is FP64.
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace msamd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split8(const float* v, bf16x8& h, bf16x8& l) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 hb = static_cast<__bf16>(v[e]);
+        h[e] = hb;
+        l[e] = static_cast<__bf16>(v[e] - static_cast<float>(hb));
+    }
+}
+
+__device__ __forceinline__ f32x4 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh,
+                                       const bf16x8& bl, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+// ---------------------------------------------------------------- covariance
+// Operand fragment of v_mfma_f32_16x16x32_bf16 for a block of 16 channels
+// over 32 samples: lane (i + 16*b) holds samples 8b..8b+7 of channel i.  The
+// A and B operands use the same sample order, so any consistent assignment
+// of samples to (b, e) yields the full sum; the D tile holds element
+// (4*(lane/16) + r, lane % 16) in register r.
+
+constexpr int kCovTPW = 16;  // output tiles per wavefront
+constexpr int kCovWaves = 4;
+
+struct CovPlan {
+    int64_t M, N, span, nsplit;
+    int32_t Mp, nB, nTiles, G;
+};
+
+inline CovPlan cov_plan(int64_t M, int64_t N) {
+    CovPlan p{};
+    p.M = M;
+    p.N = N;
+    p.Mp = static_cast<int32_t>(align_up(M, 16));
+    p.nB = p.Mp / 16;
+    p.nTiles = p.nB * (p.nB + 1) / 2;
+    p.G = static_cast<int32_t>(ceil_div(p.nTiles, kCovTPW * kCovWaves));
+    int64_t nsplit = ceil_div(2048, p.G);
+    const int64_t ksteps = ceil_div(N, 32);
+    if (nsplit > ksteps) nsplit = ksteps;
+    if (nsplit < 1) nsplit = 1;
+    p.span = align_up(ceil_div(N, nsplit), 32);
+    p.nsplit = ceil_div(N, p.span);
+    if (p.nsplit < 1) p.nsplit = 1;
+    return p;
+}
+
+__device__ __forceinline__ void tile_ij(int t, int nB, int& I, int& J) {
+    // row-major enumeration of the upper triangle (I <= J)
+    int i = 0;
+    int rowlen = nB;
+    while (t >= rowlen) {
+        t -= rowlen;
+        ++i;
+        --rowlen;
+    }
+    I = i;
+    J = i + t;
+}
+
+__global__ void __launch_bounds__(256) cov_kernel(const float* __restrict__ X, CovPlan p,
+                                                  float* __restrict__ part) {
+    extern __shared__ bf16x8 lds[];
+    bf16x8* fh = lds;
+    bf16x8* fl = lds + p.nB * 64;
+    const int g = static_cast<int>(blockIdx.x % p.G);
+    const int64_t split = blockIdx.x / p.G;
+    const int64_t tb = split * p.span;
+    int64_t te = tb + p.span;
+    if (te > p.N) te = p.N;
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int tile0 = (g * kCovWaves + wid) * kCovTPW;
+    int tI[kCovTPW], tJ[kCovTPW];
+    {
+        int I = 0, J = 0;
+        if (tile0 < p.nTiles) tile_ij(tile0, p.nB, I, J);
+#pragma unroll
+        for (int k = 0; k < kCovTPW; ++k) {
+            tI[k] = I;
+            tJ[k] = J;
+            if (++J == p.nB) {
+                ++I;
+                J = I;
+            }
+        }
+    }
+    f32x4 acc[kCovTPW];
+#pragma unroll
+    for (int k = 0; k < kCovTPW; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int nitems = 4 * p.Mp;
+    for (int64_t t0 = tb; t0 < te; t0 += 32) {
+        for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
+            const int m = q % p.Mp;
+            const int b = q / p.Mp;
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t t = t0 + 8 * b + e;
+                v[e] = (m < p.M && t < te) ? __builtin_nontemporal_load(X + t * p.M + m) : 0.0f;
+            }
+            bf16x8 h, l;
+            split8(v, h, l);
+            const int idx = (m >> 4) * 64 + (m & 15) + 16 * b;
+            fh[idx] = h;
+            fl[idx] = l;
+        }
+        __syncthreads();
+        int lastI = -1;
+        bf16x8 ah = {}, al = {};
+#pragma unroll
+        for (int k = 0; k < kCovTPW; ++k) {
+            if (tile0 + k < p.nTiles) {
+                if (tI[k] != lastI) {
+                    ah = fh[tI[k] * 64 + lane];
+                    al = fl[tI[k] * 64 + lane];
+                    lastI = tI[k];
+                }
+                const bf16x8 bh = fh[tJ[k] * 64 + lane];
+                const bf16x8 bl = fl[tJ[k] * 64 + lane];
+                acc[k] = mfma3(ah, al, bh, bl, acc[k]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kCovTPW; ++k) {
+        if (tile0 + k < p.nTiles) {
+            float* dst = part + (split * p.nTiles + tile0 + k) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[k][r];
+        }
+    }
+}
+
+// FP64 sum of the block partials in split order, / (N-1), mirrored.
+__global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict__ part,
+                                                         CovPlan p, double* __restrict__ C) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= static_cast<int64_t>(p.nTiles) * 256) return;
+    const int tile = static_cast<int>(q >> 8);
+    const int e = static_cast<int>(q & 255);
+    int I, J;
+    tile_ij(tile, p.nB, I, J);
+    const int64_t ii = I * 16 + (e >> 4);
+    const int64_t jj = J * 16 + (e & 15);
+    if (ii >= p.M || jj >= p.M) return;
+    if (I == J && ii > jj) return;  // diagonal tiles: upper half only, mirrored below
+    double s = 0.0;
+    for (int64_t sp = 0; sp < p.nsplit; ++sp)
+        s = s + static_cast<double>(part[(sp * p.nTiles + tile) * 256 + e]);
+    if (p.N > 1) s = s / static_cast<double>(p.N - 1);
+    C[ii * p.M + jj] = s;
+    C[jj * p.M + ii] = s;
+}
+
+// ------------------------------------------------------- whitening matrix
+// One workgroup runs the whole sequential power method.  G (M x M, symmetric)
+// lives in the workspace; deflation G <- (I-vv')G(I-vv') is applied in its
+// rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal to the
+// reference's two M^3 products in exact arithmetic.
+constexpr int kWThreads = 1024;
+
+__device__ double block_sum(double x, double* red) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wid] = x;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s = s + red[w];
+        red[32] = s;
+    }
+    __syncthreads();
+    s = red[32];
+    __syncthreads();
+    return s;
+}
+
+// w = G v; G row-major symmetric in global memory, v and w in LDS.
+__device__ void wave_matvec(const double* __restrict__ G, int M, const double* v, double* w) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    for (int r = wid; r < M; r += nw) {
+        const double* row = G + static_cast<int64_t>(r) * M;
+        double acc = 0.0;
+        for (int n = lane; n < M; n += 64) acc += row[n] * v[n];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) w[r] = acc;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
+    double* __restrict__ G, int M, const double* __restrict__ v0, double* __restrict__ comp,
+    double* __restrict__ sig) {
+    extern __shared__ double sh[];
+    double* v = sh;          // [M]
+    double* w = sh + M;      // [M]
+    double* red = sh + 2 * M;  // [33]
+    for (int k = 0; k < M; ++k) {
+        for (int i = threadIdx.x; i < M; i += blockDim.x) v[i] = v0[i];
+        __syncthreads();
+        double sigma = 0.0;
+        for (int it = 0; it < 10; ++it) {
+            wave_matvec(G, M, v, w);
+            double part = 0.0;
+            for (int i = threadIdx.x; i < M; i += blockDim.x) part += w[i] * w[i];
+            sigma = sqrt(block_sum(part, red));
+            for (int i = threadIdx.x; i < M; i += blockDim.x) v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < M; i += blockDim.x) comp[static_cast<int64_t>(i) * M + k] = v[i];
+        if (threadIdx.x == 0) sig[k] = sigma;
+        // deflation
+        wave_matvec(G, M, v, w);  // u = G v
+        double part = 0.0;
+        for (int i = threadIdx.x; i < M; i += blockDim.x) part += v[i] * w[i];
+        const double alpha = block_sum(part, red);
+        const int64_t MM = static_cast<int64_t>(M) * M;
+        for (int64_t q = threadIdx.x; q < MM; q += blockDim.x) {
+            const int i = static_cast<int>(q / M);
+            const int j = static_cast<int>(q - static_cast<int64_t>(i) * M);
+            const double cross = v[i] * w[j] + w[i] * v[j];
+            const double vv = v[i] * v[j];
+            G[q] = (G[q] - cross) + alpha * vv;
+        }
+        __syncthreads();
+    }
+}
+
+// W[m, n] = sum_l (C[m,l] d_l) C[n,l], l ascending (mult_AB then
+// mult_ABtrans, pca.cpp:648-649), d_l = 1/sqrt(sigma_l) or 0.
+// comp is row-major C[i*M + k]; W is written column-major like the Mda.
+__global__ void __launch_bounds__(256) whitening_assemble_kernel(
+    const double* __restrict__ comp, const double* __restrict__ sig, int M,
+    double* __restrict__ W) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= static_cast<int64_t>(M) * M) return;
+    const int n = static_cast<int>(q / M);
+    const int m = static_cast<int>(q - static_cast<int64_t>(n) * M);
+    const double* cm = comp + static_cast<int64_t>(m) * M;
+    const double* cn = comp + static_cast<int64_t>(n) * M;
+    double acc = 0.0;
+    for (int l = 0; l < M; ++l) {
+        const double s = sig[l];
+        const double d = s != 0.0 ? 1.0 / sqrt(s) : 0.0;
+        acc = acc + (cm[l] * d) * cn[l];
+    }
+    W[m + static_cast<int64_t>(M) * n] = acc;
+}
+
+// ------------------------------------------------------------------- apply
+// Y[i, t] = sum_j X[j, t] W(j, i) with W(j, i) = Wcm[j + M*i] (the reference
+// walks W column-major, p_whiten.cpp:174-182).  MFMA view: D(16 samples x
+// 16 outputs) += A(16 samples x 32 channels) B(32 channels x 16 outputs).
+// Each workgroup keeps the split-BF16 fragments of a block of CT column tiles
+// (16*CT outputs) of W in LDS and streams 8 waves x 32 samples per pass over
+// its time block.  CT is 4, 2 or 1 so that the fragments fit in LDS.
+constexpr int kApplyWaves = 8;
+constexpr int kApplyRowTiles = 2;    // 16-sample row tiles per wave per pass
+constexpr int kApplyPasses = 4;      // passes per workgroup
+constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 1024 samples
+
+struct ApplyPlan {
+    int64_t M, N;
+    int32_t Mp, nK, nColTiles, nCB, CT;
+    double qunit;
+};
+
+inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
+    ApplyPlan p{};
+    p.M = M;
+    p.N = N;
+    p.Mp = static_cast<int32_t>(align_up(M, 64));
+    p.nK = p.Mp / 32;
+    p.nColTiles = p.Mp / 16;
+    p.CT = 4;
+    while (p.CT > 1 && 2 * p.CT * p.nK * 1024 > 100 * 1024) p.CT >>= 1;
+    p.nCB = p.nColTiles / p.CT;
+    p.qunit = qunit;
+    return p;
+}
+
+// Fragments of W: index ((ct * nK + kk) * 64 + lane), lane (i + 16b) of
+// column tile ct holds W(32kk + 8b + e, 16ct + i), e = 0..7.
+__global__ void __launch_bounds__(256) apply_prep_kernel(const double* __restrict__ Wcm,
+                                                         ApplyPlan p, bf16x8* __restrict__ fh,
+                                                         bf16x8* __restrict__ fl) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
+    if (q >= total) return;
+    const int lane = static_cast<int>(q & 63);
+    const int64_t tk = q >> 6;
+    const int kk = static_cast<int>(tk % p.nK);
+    const int ct = static_cast<int>(tk / p.nK);
+    const int64_t i = ct * 16 + (lane & 15);
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int64_t j = kk * 32 + 8 * (lane >> 4) + e;
+        v[e] = (i < p.M && j < p.M) ? static_cast<float>(Wcm[j + p.M * i]) : 0.0f;
+    }
+    bf16x8 h, l;
+    split8(v, h, l);
+    fh[q] = h;
+    fl[q] = l;
+}
+
+template <bool kVec, int CT>
+__global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
+    const float* __restrict__ X, ApplyPlan p, const bf16x8* __restrict__ gfh,
+    const bf16x8* __restrict__ gfl, float* __restrict__ Y, int16_t* __restrict__ Yq) {
+    extern __shared__ bf16x8 lds[];
+    const int nfr = CT * p.nK * 64;  // fragments of this column block
+    bf16x8* fh = lds;
+    bf16x8* fl = lds + nfr;
+    const int cb = blockIdx.y;
+    {
+        const bf16x8* sh = gfh + static_cast<int64_t>(cb) * nfr;
+        const bf16x8* sl = gfl + static_cast<int64_t>(cb) * nfr;
+        for (int q = threadIdx.x; q < nfr; q += blockDim.x) {
+            fh[q] = sh[q];
+            fl[q] = sl[q];
+        }
+    }
+    __syncthreads();
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int li = lane & 15;
+    const int lb = lane >> 4;
+    const int64_t tblock = static_cast<int64_t>(blockIdx.x) * kApplyTB;
+    for (int pass = 0; pass < kApplyPasses; ++pass) {
+        const int64_t row0 = tblock + (static_cast<int64_t>(pass) * kApplyWaves + wid) * (kApplyRowTiles * 16);
+        if (row0 >= p.N) break;
+        f32x4 acc[kApplyRowTiles][CT];
+#pragma unroll
+        for (int rt = 0; rt < kApplyRowTiles; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int kk = 0; kk < p.nK; ++kk) {
+            bf16x8 ah[kApplyRowTiles], al[kApplyRowTiles];
+#pragma unroll
+            for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+                const int64_t t = row0 + rt * 16 + li;
+                const int64_t j0 = static_cast<int64_t>(kk) * 32 + 8 * lb;
+                float v[8];
+                if (kVec) {
+                    if (t < p.N && j0 < p.M) {
+                        const vf4* src = reinterpret_cast<const vf4*>(X + t * p.M + j0);
+                        const vf4 a = __builtin_nontemporal_load(src);
+                        const vf4 b = __builtin_nontemporal_load(src + 1);
+                        v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
+                        v[4] = b[0]; v[5] = b[1]; v[6] = b[2]; v[7] = b[3];
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        v[e] = (t < p.N && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
+                }
+                split8(v, ah[rt], al[rt]);
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int fi = (ct * p.nK + kk) * 64 + lane;
+                const bf16x8 bh = fh[fi];
+                const bf16x8 bl = fl[fi];
+#pragma unroll
+                for (int rt = 0; rt < kApplyRowTiles; ++rt)
+                    acc[rt][ct] = mfma3(ah[rt], al[rt], bh, bl, acc[rt][ct]);
+            }
+        }
+        // epilogue: quantize(x, 1e-4) = floor(x/1e-4 + 0.5)*1e-4 in double,
+        // stored as float; optional int16 (int)(y/q + 0.5).
+#pragma unroll
+        for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int64_t i = static_cast<int64_t>(cb) * (CT * 16) + ct * 16 + li;
+                if (i >= p.M) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t t = row0 + rt * 16 + 4 * lb + r;
+                    if (t >= p.N) continue;
+                    const double xq = floor(static_cast<double>(acc[rt][ct][r]) / 0.0001 + 0.5) * 0.0001;
+                    const float yf = static_cast<float>(xq);
+                    if (Y) Y[t * p.M + i] = yf;
+                    if (Yq) {
+                        const double s = static_cast<double>(yf) / p.qunit + 0.5;
+                        Yq[t * p.M + i] = static_cast<int16_t>(static_cast<int>(s));
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace msamd
+
+using namespace msamd;
+
+extern "C" int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N) {
+    if (M < 1 || N < 0) return -1;
+    const CovPlan p = cov_plan(M, N > 0 ? N : 1);
+    Carver c(nullptr);
+    c.take<float>(p.nsplit * p.nTiles * 256);
+    return c.bytes();
+}
+
+extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt,
+                                       void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_covariance: M=%lld outside 1..1024", (long long)M);
+    MSAMD_REQUIRE(N >= 0, "whiten_covariance: negative N");
+    hipStream_t st = as_stream(stream);
+    if (N == 0) {
+        MSAMD_HIP(hipMemsetAsync(XXt, 0, sizeof(double) * M * M, st));
+        return MSAMD_OK;
+    }
+    const CovPlan p = cov_plan(M, N);
+    Carver c(workspace);
+    float* part = c.take<float>(p.nsplit * p.nTiles * 256);
+    if (!workspace || workspace_bytes < c.bytes()) {
+        set_error("whiten_covariance: workspace too small (%lld < %lld)",
+                  (long long)workspace_bytes, (long long)c.bytes());
+        return MSAMD_ERR_WORKSPACE;
+    }
+    const size_t lds = sizeof(bf16x8) * 2 * p.nB * 64;
+    const unsigned grid = static_cast<unsigned>(p.nsplit * p.G);
+    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&cov_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
+    cov_kernel<<<grid, 256, lds, st>>>(X, p, part);
+    MSAMD_LAUNCH_CHECK();
+    const int64_t nred = static_cast<int64_t>(p.nTiles) * 256;
+    cov_reduce_kernel<<<static_cast<unsigned>(ceil_div(nred, 256)), 256, 0, st>>>(part, p, XXt);
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
+}
+
+extern "C" int64_t msamd_whitening_matrix_workspace_size(int64_t M) {
+    if (M < 1) return -1;
+    Carver c(nullptr);
+    c.take<double>(M * M);  // G
+    c.take<double>(M * M);  // components
+    c.take<double>(M);      // sigma
+    c.take<double>(M);      // v0
+    return c.bytes();
+}
+
+extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && M <= 2048, "whitening_matrix: M=%lld outside 1..2048", (long long)M);
+    Carver c(workspace);
+    double* G = c.take<double>(M * M);
+    double* comp = c.take<double>(M * M);
+    double* sig = c.take<double>(M);
+    double* v0 = c.take<double>(M);
+    if (!workspace || workspace_bytes < c.bytes()) {
+        set_error("whitening_matrix: workspace too small (%lld < %lld)",
+                  (long long)workspace_bytes, (long long)c.bytes());
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    // Start vector normalize(sin(1..M)) (pca.cpp:545-548), computed on the host
+    // with the C library's sin and a sequential double norm.
+    double* h = static_cast<double*>(malloc(sizeof(double) * M));
+    if (!h) {
+        set_error("whitening_matrix: host allocation failed");
+        return MSAMD_ERR_ARG;
+    }
+    double s2 = 0.0;
+    for (int64_t i = 0; i < M; ++i) {
+        h[i] = sin(static_cast<double>(i + 1));
+        s2 = s2 + h[i] * h[i];
+    }
+    const double nrm = sqrt(s2);
+    if (nrm != 0.0)
+        for (int64_t i = 0; i < M; ++i) h[i] = h[i] / nrm;
+    hipError_t e1 = hipMemcpyAsync(v0, h, sizeof(double) * M, hipMemcpyHostToDevice, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    free(h);
+    MSAMD_HIP(e1);
+    MSAMD_HIP(e2);
+    // XXt is symmetric: row-major and column-major views coincide.
+    MSAMD_HIP(hipMemcpyAsync(G, XXt, sizeof(double) * M * M, hipMemcpyDeviceToDevice, st));
+    const size_t lds = sizeof(double) * (2 * M + 33);
+    whitening_power_kernel<<<1, kWThreads, lds, st>>>(G, static_cast<int>(M), v0, comp, sig);
+    MSAMD_LAUNCH_CHECK();
+    whitening_assemble_kernel<<<static_cast<unsigned>(ceil_div(M * M, 256)), 256, 0, st>>>(
+        comp, sig, static_cast<int>(M), W);
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
+}
+
+extern "C" int64_t msamd_whiten_apply_workspace_size(int64_t M) {
+    if (M < 1) return -1;
+    const ApplyPlan p = apply_plan(M, 1, 0.0);
+    Carver c(nullptr);
+    c.take<bf16x8>(static_cast<int64_t>(p.nColTiles) * p.nK * 64);
+    c.take<bf16x8>(static_cast<int64_t>(p.nColTiles) * p.nK * 64);
+    return c.bytes();
+}
+
+extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,
+                                  double quantization_unit, int16_t* Yq, void* workspace,
+                                  int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_apply: M=%lld outside 1..1024", (long long)M);
+    MSAMD_REQUIRE(N >= 0, "whiten_apply: negative N");
+    MSAMD_REQUIRE(Y || Yq, "whiten_apply: no output");
+    MSAMD_REQUIRE(!Yq || quantization_unit > 0, "whiten_apply: int16 output needs quantization_unit > 0");
+    const ApplyPlan p = apply_plan(M, N, quantization_unit);
+    Carver c(workspace);
+    const int64_t nfr = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
+    bf16x8* fh = c.take<bf16x8>(nfr);
+    bf16x8* fl = c.take<bf16x8>(nfr);
+    if (!workspace || workspace_bytes < c.bytes()) {
+        set_error("whiten_apply: workspace too small (%lld < %lld)", (long long)workspace_bytes,
+                  (long long)c.bytes());
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    apply_prep_kernel<<<static_cast<unsigned>(ceil_div(nfr, 256)), 256, 0, st>>>(W, p, fh, fl);
+    MSAMD_LAUNCH_CHECK();
+    if (N == 0) return MSAMD_OK;
+    const size_t lds = sizeof(bf16x8) * 2 * p.CT * p.nK * 64;
+    dim3 grid(static_cast<unsigned>(ceil_div(N, kApplyTB)), static_cast<unsigned>(p.nCB));
+    const bool vec = (M % 8 == 0);
+#define MSAMD_APPLY_LAUNCH(V, C)                                                              \
+    do {                                                                                      \
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<V, C>),      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,             \
+                                      static_cast<int>(lds)));                                \
+        apply_kernel<V, C><<<grid, kApplyWaves * 64, lds, st>>>(X, p, fh, fl, Y, Yq);         \
+    } while (0)
+    if (p.CT == 4) {
+        if (vec) MSAMD_APPLY_LAUNCH(true, 4); else MSAMD_APPLY_LAUNCH(false, 4);
+    } else if (p.CT == 2) {
+        if (vec) MSAMD_APPLY_LAUNCH(true, 2); else MSAMD_APPLY_LAUNCH(false, 2);
+    } else {
+        if (vec) MSAMD_APPLY_LAUNCH(true, 1); else MSAMD_APPLY_LAUNCH(false, 1);
+    }
+#undef MSAMD_APPLY_LAUNCH
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
+}

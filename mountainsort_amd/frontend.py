@@ -1,0 +1,215 @@
+"""Device-resident operators of the MountainSort front end over the C ABI.
+
+These mirror the reference's processors and in-process stages with the same
+parameter names (packages/ms3/mountainsort3_main.cpp:145-198,343-350;
+packages/mountainsortalg/mountainsortalg_main.cpp:31-46) but take and return
+torch tensors that already live on the GPU.  torch only provides device memory
+and the current stream; all arithmetic runs in the HIP kernels of libmsamd.so.
+
+Layouts (identical to the .mda memory order of the reference):
+  timeseries M x N  -> tensor (N, M), element (m, t) at m + M*t
+  clips W x T x L   -> tensor (L, T, W)
+  M x M matrices    -> tensor (M, M) indexed [row, col]; the C ABI receives
+                       the column-major (Mda) buffer.
+"""
+import ctypes
+
+import torch
+
+from . import _capi
+
+DETECT_PADDING = 1000      # p_mountainsort3.cpp:515
+BANDPASS_OVERLAP = 60000   # p_bandpass_filter.cpp:150
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _check_ts(X, name="timeseries", dtype=torch.float32):
+    if not isinstance(X, torch.Tensor) or not X.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if X.dim() != 2 or X.dtype != dtype or not X.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous ({dtype}) tensor of shape (N, M)")
+
+
+def _channel_list(channels, M, dev):
+    ch = torch.as_tensor(channels, dtype=torch.int32).flatten()
+    if ch.numel() and (int(ch.min()) < 0 or int(ch.max()) >= M):
+        raise ValueError(f"channel index outside [0, {M})")
+    return ch.to(dev).contiguous()
+
+
+def _workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+# --------------------------------------------------------------------------
+# detection (a8-a10)
+# --------------------------------------------------------------------------
+def detect_chunk_size(M, N, num_threads):
+    """Chunk length of the ms3alg time-chunk plan (p_mountainsort3.cpp:170-172,507-533)."""
+    return int(_capi.lib().msamd_detect_chunk_size(int(M), int(N), int(num_threads)))
+
+
+def detect_events(X, detect_threshold=3.0, detect_interval=10, detect_sign=0, num_threads=8,
+                  chunk_size=None, t1=-1, t2=-1, channels=None, padding=DETECT_PADDING):
+    """Detection phase of mountainsortalg.ms3alg on every central channel.
+
+    Returns (offsets, times): int64 device tensors; the events of
+    channels[c] are times[offsets[c]:offsets[c+1]] (absolute 0-based sample
+    indices, ascending).  num_threads is the reference thread count that
+    fixes the chunk plan (p_mountainsort3.cpp:170-172).
+    """
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    if channels is None:
+        channels = torch.arange(M, dtype=torch.int32, device=dev)
+    else:
+        channels = _channel_list(channels, M, dev)
+    Mc = int(channels.numel())
+    t_start = 0 if t1 < 0 else int(t1)
+    t_end = N - 1 if t2 < 0 else int(t2)
+    n_range = t_end - t_start + 1
+    cs = int(chunk_size) if chunk_size else detect_chunk_size(M, n_range, num_threads)
+    L = _capi.lib()
+    nb = L.msamd_detect_workspace_size(n_range, cs, Mc, float(detect_interval))
+    ws = _workspace(nb, dev)
+    total = ctypes.c_int64(0)
+    st = _stream(dev)
+    _capi.call("msamd_detect_events", _ptr(X), M, N, int(t1), int(t2), cs, int(padding),
+               _ptr(channels), Mc, float(detect_threshold), float(detect_interval),
+               int(detect_sign), _ptr(ws), ws.numel(), ctypes.byref(total), st)
+    times = torch.empty(max(total.value, 1), dtype=torch.int64, device=dev)
+    offsets = torch.empty(Mc + 1, dtype=torch.int64, device=dev)
+    _capi.call("msamd_detect_fetch", _ptr(ws), Mc, _ptr(times), _ptr(offsets), st)
+    return offsets, times[:total.value]
+
+
+# --------------------------------------------------------------------------
+# clips (a11)
+# --------------------------------------------------------------------------
+def clip_tmid(T):
+    return (int(T) + 1) // 2 - 1
+
+
+def extract_clips(X, starts, channels, clip_size):
+    """Gather clips (L, T, W) with clip[i, t, w] = X[starts[i] + t, channels[w]]
+    (zero outside the recording).  starts: int64 first-sample indices."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    starts = torch.as_tensor(starts, dtype=torch.int64).to(dev).contiguous()
+    channels = _channel_list(channels, M, dev)
+    L, W, T = int(starts.numel()), int(channels.numel()), int(clip_size)
+    clips = torch.empty((L, T, W), dtype=torch.float32, device=dev)
+    _capi.call("msamd_extract_clips", _ptr(X), M, N, _ptr(starts), L, _ptr(channels), W, T,
+               _ptr(clips), _stream(dev))
+    return clips
+
+
+def extract_clips_batched(X, times, event_offsets, nbhd_channels, nbhd_offsets, clip_size):
+    """Clips of G neighborhoods in one launch.  Returns (clips_flat, clip_offsets):
+    neighborhood g's (L_g, T, W_g) clips start at clips_flat[clip_offsets[g]]."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    T = int(clip_size)
+    eo = torch.as_tensor(event_offsets, dtype=torch.int64).to(dev).contiguous()
+    no = torch.as_tensor(nbhd_offsets, dtype=torch.int64).to(dev).contiguous()
+    nc = _channel_list(nbhd_channels, M, dev)
+    times = times.to(dev).contiguous()
+    G = int(no.numel()) - 1
+    sizes = (eo[1:] - eo[:-1]) * (no[1:] - no[:-1]) * T
+    co = torch.zeros(G + 1, dtype=torch.int64, device=dev)
+    co[1:] = torch.cumsum(sizes, 0)
+    total_elems = int(co[-1].item())
+    clips = torch.empty(max(total_elems, 1), dtype=torch.float32, device=dev)
+    total_events = int(eo[-1].item())
+    _capi.call("msamd_extract_clips_batched", _ptr(X), M, N, _ptr(times), _ptr(eo), total_events,
+               _ptr(nc), _ptr(no), G, T, _ptr(co), _ptr(clips), _stream(dev))
+    return clips[:total_elems], co
+
+
+# --------------------------------------------------------------------------
+# whitening (a4-a6)
+# --------------------------------------------------------------------------
+def whiten_covariance(X):
+    """X X^T / (N-1) as an (M, M) float64 tensor (p_whiten.cpp:64-118)."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    L = _capi.lib()
+    ws = _workspace(L.msamd_whiten_covariance_workspace_size(M, N), dev)
+    C = torch.empty((M, M), dtype=torch.float64, device=dev)
+    _capi.call("msamd_whiten_covariance", _ptr(X), M, N, _ptr(C), _ptr(ws), ws.numel(),
+               _stream(dev))
+    return C
+
+
+def whitening_matrix(C):
+    """W from the covariance by the reference's deflated power method (pca.cpp:634-650)."""
+    M = C.shape[0]
+    dev = C.device
+    Ccm = C.to(torch.float64).t().contiguous()
+    L = _capi.lib()
+    ws = _workspace(L.msamd_whitening_matrix_workspace_size(M), dev)
+    Wcm = torch.empty((M, M), dtype=torch.float64, device=dev)
+    _capi.call("msamd_whitening_matrix", _ptr(Ccm), M, _ptr(Wcm), _ptr(ws), ws.numel(),
+               _stream(dev))
+    return Wcm.t()  # [row, col] view of the column-major result
+
+
+def whiten_apply(X, W, quantization_unit=0.0, out=None):
+    """quantize(W applied to every sample, 1e-4) (p_whiten.cpp:153-206).
+
+    Returns float32 (N, M); with quantization_unit > 0 returns int16 values
+    (int)(y/q + 0.5) (p_whiten.cpp:445-451)."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    Wcm = W.to(device=dev, dtype=torch.float64).t().contiguous()
+    L = _capi.lib()
+    ws = _workspace(L.msamd_whiten_apply_workspace_size(M), dev)
+    q = float(quantization_unit)
+    if q > 0:
+        Yq = torch.empty((N, M), dtype=torch.int16, device=dev)
+        _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), None, q, _ptr(Yq), _ptr(ws),
+                   ws.numel(), _stream(dev))
+        return Yq
+    Y = out if out is not None else torch.empty((N, M), dtype=torch.float32, device=dev)
+    _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), _ptr(Y), 0.0, None, _ptr(ws),
+               ws.numel(), _stream(dev))
+    return Y
+
+
+def whiten(X, quantization_unit=0.0):
+    """ms3.whiten: covariance, whitening matrix, apply (p_whiten.cpp:45-206)."""
+    C = whiten_covariance(X)
+    W = whitening_matrix(C)
+    return whiten_apply(X, W, quantization_unit)
+
+
+# --------------------------------------------------------------------------
+# artifact masking (a7)
+# --------------------------------------------------------------------------
+def mask_out_artifacts(X, threshold=6.0, interval_size=2000, return_use_it=False, out=None):
+    """ms3.mask_out_artifacts (p_mask_out_artifacts.cpp:24-104)."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    isz = int(interval_size)
+    if threshold == 0 or isz == 0:
+        raise ValueError("Problem with input parameters. Either threshold or interval_size is zero.")
+    L = _capi.lib()
+    ws = _workspace(L.msamd_mask_workspace_size(M, N, isz), dev)
+    Y = out if out is not None else torch.empty((N, M), dtype=torch.float32, device=dev)
+    use_it = torch.empty(N // isz + 1, dtype=torch.int32, device=dev)
+    _capi.call("msamd_mask_out_artifacts", _ptr(X), M, N, float(threshold), isz, _ptr(Y),
+               _ptr(use_it), _ptr(ws), ws.numel(), _stream(dev))
+    return (Y, use_it) if return_use_it else Y

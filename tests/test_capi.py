@@ -1,0 +1,53 @@
+"""The C ABI library loads and exports every symbol declared in include/msamd.h;
+host-only entry points agree with the oracle.  CPU only (no kernel launches)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from mountainsort_amd import _capi
+from oracle import core as oc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "msamd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(msamd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libmsamd.so not built")
+    L = ctypes.CDLL(_capi.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # every declared symbol has a ctypes signature in the binding
+    assert set(names) <= set(_capi.SIGNATURES), set(names) - set(_capi.SIGNATURES)
+
+
+def test_host_entry_points():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libmsamd.so not built")
+    L = _capi.lib()
+    assert L.msamd_version() >= 10000
+    for M, N, T in [(384, 9_000_000, 8), (64, 18_000_000, 8), (64, 18_000_000, 1), (4, 1_800_000, 8),
+                    (4, 5000, 8), (1, 100, 3)]:
+        assert L.msamd_detect_chunk_size(M, N, T) == oc.detect_chunk_size(M, N, T)
+    assert L.msamd_detect_workspace_size(9_000_000, 20000, 384, 10.0) > 0
+    assert L.msamd_whiten_covariance_workspace_size(384, 9_000_000) > 0
+    assert L.msamd_whitening_matrix_workspace_size(384) >= 2 * 384 * 384 * 8
+    assert L.msamd_whiten_apply_workspace_size(384) > 0
+    assert L.msamd_mask_workspace_size(384, 9_000_000, 2000) > 0
+    assert L.msamd_mask_workspace_size(0, 10, 10) < 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_capi, "_lib", None)
+    with pytest.raises(_capi.MsamdLibraryMissing):
+        _capi.lib()

@@ -1,0 +1,177 @@
+"""GPU parity: detection, clips, whitening and masking through the C ABI
+against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import core as oc
+from tests.conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _recording(M, N, seed, units=6, amp=(60, 200)):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(N, M)) * 10
+    t = np.arange(50)
+    for _ in range(units):
+        w = -rng.uniform(*amp) * np.exp(-0.5 * ((t - 20) / 2.5) ** 2) \
+            + 0.3 * rng.uniform(*amp) * np.exp(-0.5 * ((t - 30) / 5) ** 2)
+        g = rng.uniform(0.1, 1.0, M)
+        for st in rng.integers(0, N - 50, N // 3000):
+            X[st:st + 50] += np.outer(w, g)
+    return X.astype(np.float32)
+
+
+def _whitened(M, N, seed):
+    """A realistic whitened (1e-4-quantized) recording produced by the oracle."""
+    X = _recording(M, N, seed)
+    C = oc.whiten_covariance(X)
+    W = oc.whitening_matrix(C)
+    return oc.whiten_apply(X, W)
+
+
+# ------------------------------------------------------------------ detect
+@pytest.mark.parametrize("M,N,T,sign,D,t1,t2,cs", [
+    (4, 200_000, 8, -1, 10, -1, -1, None),
+    (4, 200_000, 8, 0, 10, -1, -1, 30_000),
+    (7, 150_000, 8, 1, 3, 1234, 120_000, 20_000),
+    (64, 100_000, 8, -1, 10, -1, -1, 20_000),
+    (64, 100_000, 1, -1, 0, -1, -1, None),
+    (3, 45_000, 8, -1, 25, -1, -1, 20_000),
+])
+def test_detect_bit_exact(gpu, M, N, T, sign, D, t1, t2, cs):
+    from mountainsort_amd import frontend as fe
+    say(f"detect M={M} N={N} T={T} sign={sign} D={D} t1={t1} t2={t2} cs={cs}")
+    Y = _whitened(M, N, seed=M + N)
+    n_range = (N - 1 if t2 < 0 else t2) - (0 if t1 < 0 else t1) + 1
+    chunk = cs or oc.detect_chunk_size(M, n_range, T)
+    assert fe.detect_chunk_size(M, n_range, T) == oc.detect_chunk_size(M, n_range, T)
+    counts, times = oc.ms3alg_detect(Y, 3.0, D, sign, chunk_size=chunk, t1=t1, t2=t2)
+    off, gt = fe.detect_events(torch.from_numpy(Y).to(gpu), 3.0, D, sign, num_threads=T,
+                               chunk_size=chunk, t1=t1, t2=t2)
+    np.testing.assert_array_equal(off.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
+    np.testing.assert_array_equal(gt.cpu().numpy(), times)
+    assert len(times) > 0
+
+
+def test_detect_channel_subset_and_empty(gpu):
+    from mountainsort_amd import frontend as fe
+    say("detect channel subset / empty range")
+    Y = _whitened(6, 60_000, seed=3)
+    ch = np.array([4, 1, 5], np.int32)
+    counts, times = oc.ms3alg_detect(Y, 3.0, 10, -1, chunk_size=20000, channels=ch)
+    off, gt = fe.detect_events(torch.from_numpy(Y).to(gpu), 3.0, 10, -1, chunk_size=20000,
+                               channels=ch)
+    np.testing.assert_array_equal(gt.cpu().numpy(), times)
+    off2, gt2 = fe.detect_events(torch.from_numpy(Y).to(gpu), 3.0, 10, -1, chunk_size=20000,
+                                 channels=[])
+    assert gt2.numel() == 0 and off2.cpu().tolist() == [0]
+
+
+# ------------------------------------------------------------------- clips
+def test_clips_bit_exact(gpu):
+    from mountainsort_amd import frontend as fe
+    say("clips single + batched")
+    rng = np.random.default_rng(11)
+    N, M, T = 30_000, 10, 50
+    X = rng.normal(size=(N, M)).astype(np.float32)
+    starts = np.concatenate([[-30, -1, 0], rng.integers(0, N - T, 500), [N - 10, N + 5]])
+    ch = np.array([3, 0, 9, 4], np.int32)
+    Xg = torch.from_numpy(X).to(gpu)
+    got = fe.extract_clips(Xg, starts, ch, T).cpu().numpy()
+    np.testing.assert_array_equal(got, oc.extract_clips(X, starts, ch, T))
+    # batched: three neighborhoods with uneven widths and counts (one empty)
+    nb = [np.array([2, 1, 3], np.int32), np.array([5], np.int32), np.array([7, 8, 6, 9, 0], np.int32)]
+    ev = [np.sort(rng.integers(0, N, 40)), np.array([], np.int64), np.sort(rng.integers(0, N, 75))]
+    times = torch.from_numpy(np.concatenate(ev).astype(np.int64)).to(gpu)
+    eo = np.concatenate([[0], np.cumsum([len(e) for e in ev])])
+    no = np.concatenate([[0], np.cumsum([len(c) for c in nb])])
+    flat, co = fe.extract_clips_batched(Xg, times, eo, np.concatenate(nb), no, T)
+    flat = flat.cpu().numpy()
+    co = co.cpu().numpy()
+    tmid = oc.clip_tmid(T)
+    for g in range(3):
+        ref = oc.extract_clips(X, ev[g] - tmid, nb[g], T)
+        np.testing.assert_array_equal(flat[co[g]:co[g + 1]].reshape(ref.shape), ref)
+
+
+# --------------------------------------------------------------- whitening
+@pytest.mark.parametrize("M,N", [(4, 100_000), (16, 50_000), (64, 40_000), (384, 20_000), (5, 1000)])
+def test_covariance_split_bf16_error(gpu, M, N):
+    from mountainsort_amd import frontend as fe
+    say(f"covariance M={M} N={N}")
+    X = _recording(M, N, seed=M)
+    C = fe.whiten_covariance(torch.from_numpy(X).to(gpu)).cpu().numpy()
+    R = oc.covariance_fp64(X)
+    np.testing.assert_array_equal(C, C.T)
+    # split-BF16 3-term products, FP32 block sums: relative to the diagonal scale
+    err = np.max(np.abs(C - R)) / np.max(np.abs(np.diag(R)))
+    say(f"  max |C_gpu - C_fp64| / max diag = {err:.3e}")
+    assert err < 2e-5
+
+
+@pytest.mark.parametrize("M", [1, 3, 4, 8, 16])
+def test_whitening_matrix_matches_oracle_small_m(gpu, M):
+    from mountainsort_amd import frontend as fe
+    say(f"whitening matrix M={M}")
+    X = _recording(M, 30_000, seed=100 + M)
+    C = oc.whiten_covariance(X)
+    Wo = oc.whitening_matrix(C)
+    Wg = fe.whitening_matrix(torch.from_numpy(C).to(gpu)).cpu().numpy()
+    rel = np.max(np.abs(Wg - Wo)) / np.max(np.abs(Wo))
+    say(f"  max rel |W_gpu - W_oracle| = {rel:.3e}")
+    assert rel < 1e-10
+
+
+def test_whitening_matrix_quality_m64(gpu):
+    """At M = 64 the 10-step power method is chaotic in rounding; compare the
+    whitening quality with the oracle's on the same covariance."""
+    from mountainsort_amd import frontend as fe
+    say("whitening matrix M=64 quality")
+    X = _recording(64, 40_000, seed=164)
+    C = oc.whiten_covariance(X)
+    Wo = oc.whitening_matrix(C)
+    Wg = fe.whitening_matrix(torch.from_numpy(C).to(gpu)).cpu().numpy()
+    qo = np.max(np.abs(Wo @ C @ Wo.T - np.eye(64)))
+    qg = np.max(np.abs(Wg @ C @ Wg.T - np.eye(64)))
+    say(f"  max|WCW'-I| gpu={qg:.3e} oracle={qo:.3e} |dW|/|W|={np.linalg.norm(Wg - Wo) / np.linalg.norm(Wo):.3e}")
+    assert qg < max(2 * qo, 1e-6) + 0.05
+
+
+@pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000)])
+def test_whiten_apply_within_one_quantum(gpu, M, N):
+    from mountainsort_amd import frontend as fe
+    say(f"whiten apply M={M} N={N}")
+    X = _recording(M, N, seed=200 + M)
+    C = oc.covariance_fp64(X)
+    w, V = np.linalg.eigh(C)
+    W = (V / np.sqrt(w)) @ V.T  # any fixed W: the apply is what is tested
+    Yg = fe.whiten_apply(torch.from_numpy(X).to(gpu), torch.from_numpy(W).to(gpu)).cpu().numpy()
+    k_gpu = np.round(Yg.astype(np.float64) / 1e-4)
+    np.testing.assert_array_equal(Yg, (k_gpu * 1e-4).astype(np.float32))
+    k64 = np.floor(oc.whiten_apply_fp64(X, W) / 1e-4 + 0.5)
+    d = np.abs(k_gpu - k64)
+    say(f"  off-by-one-quantum fraction vs FP64: {np.mean(d > 0):.4f}; max {d.max():.0f}")
+    assert d.max() <= 1
+    Yo = oc.whiten_apply(X, W)
+    assert np.max(np.abs(np.round(Yo.astype(np.float64) / 1e-4) - k_gpu)) <= 1
+    Yq = fe.whiten_apply(torch.from_numpy(X).to(gpu), torch.from_numpy(W).to(gpu),
+                         quantization_unit=0.01).cpu().numpy()
+    np.testing.assert_array_equal(Yq, (Yg.astype(np.float64) / 0.01 + 0.5).astype(np.int32).astype(np.int16))
+
+
+# -------------------------------------------------------------------- mask
+@pytest.mark.parametrize("M,N,isz", [(4, 2000 * 50 + 123, 2000), (64, 500 * 80, 500), (5, 999, 2000)])
+def test_mask_bit_exact(gpu, M, N, isz):
+    from mountainsort_amd import frontend as fe
+    say(f"mask M={M} N={N} isz={isz}")
+    rng = np.random.default_rng(M)
+    X = (rng.normal(size=(N, M)) * 10).astype(np.float32)
+    for _ in range(3):
+        t0 = int(rng.integers(0, max(N - 300, 1)))
+        X[t0:t0 + 300, int(rng.integers(0, M))] += 500
+    Yo, _, uo = oc.mask_out_artifacts(X, 6.0, isz)
+    Yg, ug = fe.mask_out_artifacts(torch.from_numpy(X).to(gpu), 6.0, isz, return_use_it=True)
+    np.testing.assert_array_equal(ug.cpu().numpy(), uo)
+    np.testing.assert_array_equal(Yg.cpu().numpy(), Yo)
