@@ -137,7 +137,10 @@ int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* worksp
 int64_t msamd_whiten_apply_workspace_size(int64_t M);
 
 /* Y = quantize(W X, 1e-4) with quantize(x,u) = floor(x/u + 0.5)*u
- * (p_whiten.cpp:31-34,153-206).  Split-BF16 3-term MFMA, FP32 accumulation.
+ * (p_whiten.cpp:31-34,153-206).  Split-BF16 MFMA with FP32 accumulation:
+ * each operand is split into three bf16 pieces (exact for fp32) and the six
+ * products of order >= 2^-16 are accumulated, so products carry FP32-level
+ * error (the 1e-4 output quantum needs it; two pieces would not).
  * Y: M x N float32 (may be NULL when Yq is given).  When quantization_unit
  * > 0, Yq receives the int16 values (int)(y/q + 0.5) (p_whiten.cpp:445-451). */
 int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,

@@ -282,16 +282,28 @@ __global__ void __launch_bounds__(256) whitening_assemble_kernel(
 // Y[i, t] = sum_j X[j, t] W(j, i) with W(j, i) = Wcm[j + M*i] (the reference
 // walks W column-major, p_whiten.cpp:174-182).  MFMA view: D(16 samples x
 // 16 outputs) += A(16 samples x 32 channels) B(32 channels x 16 outputs).
-// Each workgroup keeps the split-BF16 fragments of a block of CT column tiles
-// (16*CT outputs) of W in LDS and streams 8 waves x 32 samples per pass over
-// its time block.  CT is 4, 2 or 1 so that the fragments fit in LDS.
+//
+// Precision: the 1e-4 output quantum of the reference needs FP32-level
+// products, and bf16 hi+lo keeps only 16 significant bits (|x - hi - lo| <=
+// 2^-16 |x|), which with cancellation inside W x reaches two quanta.  Both
+// operands are therefore split into three bf16 pieces x = h + m + l (exact
+// for an fp32 value) and the six products of order >= 2^-16 are accumulated:
+// hh + hm + mh + hl + lh + mm; the dropped terms are O(2^-24) relative.
+//
+// Each workgroup keeps the three-piece fragments of CT column tiles (16*CT
+// outputs) of W in LDS and streams 8 waves x 32 samples per pass over its
+// 1024-sample time block.  Workgroups are numbered so that all column
+// blocks of one time block run back to back on the same XCD (blockIdx % 8
+// selects the XCD), and the time block is read once from HBM into that
+// XCD's L2.
 constexpr int kApplyWaves = 8;
 constexpr int kApplyRowTiles = 2;    // 16-sample row tiles per wave per pass
 constexpr int kApplyPasses = 4;      // passes per workgroup
 constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 1024 samples
+constexpr int kXcd = 8;
 
 struct ApplyPlan {
-    int64_t M, N;
+    int64_t M, N, nTB, nWG;
     int32_t Mp, nK, nColTiles, nCB, CT;
     double qunit;
 };
@@ -304,17 +316,44 @@ inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
     p.nK = p.Mp / 32;
     p.nColTiles = p.Mp / 16;
     p.CT = 4;
-    while (p.CT > 1 && 2 * p.CT * p.nK * 1024 > 100 * 1024) p.CT >>= 1;
+    while (p.CT > 1 && 3 * p.CT * p.nK * 1024 > 150 * 1024) p.CT >>= 1;
     p.nCB = p.nColTiles / p.CT;
     p.qunit = qunit;
+    p.nTB = ceil_div(N, kApplyTB);
+    p.nWG = align_up(p.nTB, kXcd) * p.nCB;
     return p;
 }
 
-// Fragments of W: index ((ct * nK + kk) * 64 + lane), lane (i + 16b) of
-// column tile ct holds W(32kk + 8b + e, 16ct + i), e = 0..7.
+__device__ __forceinline__ void split3_8(const float* v, bf16x8& h, bf16x8& m, bf16x8& l) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 hb = static_cast<__bf16>(v[e]);
+        const float r1 = v[e] - static_cast<float>(hb);   // exact
+        const __bf16 mb = static_cast<__bf16>(r1);
+        const float r2 = r1 - static_cast<float>(mb);     // exact
+        h[e] = hb;
+        m[e] = mb;
+        l[e] = static_cast<__bf16>(r2);
+    }
+}
+
+__device__ __forceinline__ f32x4 mfma6(const bf16x8& ah, const bf16x8& am, const bf16x8& al,
+                                       const bf16x8& bh, const bf16x8& bm, const bf16x8& bl,
+                                       f32x4 acc) {
+    // smallest terms first
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+// Fragments of W: index ((ct * nK + kk) * 64 + lane) within each piece array;
+// lane (i + 16b) of column tile ct holds W(32kk + 8b + e, 16ct + i), e = 0..7.
 __global__ void __launch_bounds__(256) apply_prep_kernel(const double* __restrict__ Wcm,
-                                                         ApplyPlan p, bf16x8* __restrict__ fh,
-                                                         bf16x8* __restrict__ fl) {
+                                                         ApplyPlan p, bf16x8* __restrict__ fr) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
     if (q >= total) return;
@@ -329,27 +368,36 @@ __global__ void __launch_bounds__(256) apply_prep_kernel(const double* __restric
         const int64_t j = kk * 32 + 8 * (lane >> 4) + e;
         v[e] = (i < p.M && j < p.M) ? static_cast<float>(Wcm[j + p.M * i]) : 0.0f;
     }
-    bf16x8 h, l;
-    split8(v, h, l);
-    fh[q] = h;
-    fl[q] = l;
+    bf16x8 h, m, l;
+    split3_8(v, h, m, l);
+    fr[q] = h;
+    fr[total + q] = m;
+    fr[2 * total + q] = l;
 }
 
 template <bool kVec, int CT>
 __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
-    const float* __restrict__ X, ApplyPlan p, const bf16x8* __restrict__ gfh,
-    const bf16x8* __restrict__ gfl, float* __restrict__ Y, int16_t* __restrict__ Yq) {
+    const float* __restrict__ X, ApplyPlan p, const bf16x8* __restrict__ gfr,
+    float* __restrict__ Y, int16_t* __restrict__ Yq) {
     extern __shared__ bf16x8 lds[];
-    const int nfr = CT * p.nK * 64;  // fragments of this column block
+    const int nfr = CT * p.nK * 64;  // fragments of this column block, per piece
+    // XCD-aware numbering: all column blocks of a time block share blockIdx % 8
+    const int64_t w = blockIdx.x;
+    const int64_t xcd = w % kXcd;
+    const int64_t jx = w / kXcd;
+    const int cb = static_cast<int>(jx % p.nCB);
+    const int64_t tb = (jx / p.nCB) * kXcd + xcd;
+    if (tb >= p.nTB) return;
     bf16x8* fh = lds;
-    bf16x8* fl = lds + nfr;
-    const int cb = blockIdx.y;
+    bf16x8* fm = lds + nfr;
+    bf16x8* fl = lds + 2 * nfr;
     {
-        const bf16x8* sh = gfh + static_cast<int64_t>(cb) * nfr;
-        const bf16x8* sl = gfl + static_cast<int64_t>(cb) * nfr;
+        const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
+        const int64_t base = static_cast<int64_t>(cb) * nfr;
         for (int q = threadIdx.x; q < nfr; q += blockDim.x) {
-            fh[q] = sh[q];
-            fl[q] = sl[q];
+            fh[q] = gfr[base + q];
+            fm[q] = gfr[total + base + q];
+            fl[q] = gfr[2 * total + base + q];
         }
     }
     __syncthreads();
@@ -357,7 +405,7 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     const int lane = threadIdx.x & 63;
     const int li = lane & 15;
     const int lb = lane >> 4;
-    const int64_t tblock = static_cast<int64_t>(blockIdx.x) * kApplyTB;
+    const int64_t tblock = tb * kApplyTB;
     for (int pass = 0; pass < kApplyPasses; ++pass) {
         const int64_t row0 = tblock + (static_cast<int64_t>(pass) * kApplyWaves + wid) * (kApplyRowTiles * 16);
         if (row0 >= p.N) break;
@@ -367,7 +415,7 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int kk = 0; kk < p.nK; ++kk) {
-            bf16x8 ah[kApplyRowTiles], al[kApplyRowTiles];
+            bf16x8 ah[kApplyRowTiles], am[kApplyRowTiles], al[kApplyRowTiles];
 #pragma unroll
             for (int rt = 0; rt < kApplyRowTiles; ++rt) {
                 const int64_t t = row0 + rt * 16 + li;
@@ -376,8 +424,8 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                 if (kVec) {
                     if (t < p.N && j0 < p.M) {
                         const vf4* src = reinterpret_cast<const vf4*>(X + t * p.M + j0);
-                        const vf4 a = __builtin_nontemporal_load(src);
-                        const vf4 b = __builtin_nontemporal_load(src + 1);
+                        const vf4 a = src[0];
+                        const vf4 b = src[1];
                         v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
                         v[4] = b[0]; v[5] = b[1]; v[6] = b[2]; v[7] = b[3];
                     } else {
@@ -389,16 +437,17 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                     for (int e = 0; e < 8; ++e)
                         v[e] = (t < p.N && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
                 }
-                split8(v, ah[rt], al[rt]);
+                split3_8(v, ah[rt], am[rt], al[rt]);
             }
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
                 const int fi = (ct * p.nK + kk) * 64 + lane;
                 const bf16x8 bh = fh[fi];
+                const bf16x8 bm = fm[fi];
                 const bf16x8 bl = fl[fi];
 #pragma unroll
                 for (int rt = 0; rt < kApplyRowTiles; ++rt)
-                    acc[rt][ct] = mfma3(ah[rt], al[rt], bh, bl, acc[rt][ct]);
+                    acc[rt][ct] = mfma6(ah[rt], am[rt], al[rt], bh, bm, bl, acc[rt][ct]);
             }
         }
         // epilogue: quantize(x, 1e-4) = floor(x/1e-4 + 0.5)*1e-4 in double,
@@ -530,8 +579,7 @@ extern "C" int64_t msamd_whiten_apply_workspace_size(int64_t M) {
     if (M < 1) return -1;
     const ApplyPlan p = apply_plan(M, 1, 0.0);
     Carver c(nullptr);
-    c.take<bf16x8>(static_cast<int64_t>(p.nColTiles) * p.nK * 64);
-    c.take<bf16x8>(static_cast<int64_t>(p.nColTiles) * p.nK * 64);
+    c.take<bf16x8>(3 * static_cast<int64_t>(p.nColTiles) * p.nK * 64);
     return c.bytes();
 }
 
@@ -546,26 +594,25 @@ extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const do
     const ApplyPlan p = apply_plan(M, N, quantization_unit);
     Carver c(workspace);
     const int64_t nfr = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
-    bf16x8* fh = c.take<bf16x8>(nfr);
-    bf16x8* fl = c.take<bf16x8>(nfr);
+    bf16x8* fr = c.take<bf16x8>(3 * nfr);
     if (!workspace || workspace_bytes < c.bytes()) {
         set_error("whiten_apply: workspace too small (%lld < %lld)", (long long)workspace_bytes,
                   (long long)c.bytes());
         return MSAMD_ERR_WORKSPACE;
     }
     hipStream_t st = as_stream(stream);
-    apply_prep_kernel<<<static_cast<unsigned>(ceil_div(nfr, 256)), 256, 0, st>>>(W, p, fh, fl);
+    apply_prep_kernel<<<static_cast<unsigned>(ceil_div(nfr, 256)), 256, 0, st>>>(W, p, fr);
     MSAMD_LAUNCH_CHECK();
     if (N == 0) return MSAMD_OK;
-    const size_t lds = sizeof(bf16x8) * 2 * p.CT * p.nK * 64;
-    dim3 grid(static_cast<unsigned>(ceil_div(N, kApplyTB)), static_cast<unsigned>(p.nCB));
+    const size_t lds = sizeof(bf16x8) * 3 * p.CT * p.nK * 64;
+    const dim3 grid(static_cast<unsigned>(p.nWG));
     const bool vec = (M % 8 == 0);
 #define MSAMD_APPLY_LAUNCH(V, C)                                                              \
     do {                                                                                      \
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<V, C>),      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize,             \
                                       static_cast<int>(lds)));                                \
-        apply_kernel<V, C><<<grid, kApplyWaves * 64, lds, st>>>(X, p, fh, fl, Y, Yq);         \
+        apply_kernel<V, C><<<grid, kApplyWaves * 64, lds, st>>>(X, p, fr, Y, Yq);             \
     } while (0)
     if (p.CT == 4) {
         if (vec) MSAMD_APPLY_LAUNCH(true, 4); else MSAMD_APPLY_LAUNCH(false, 4);
