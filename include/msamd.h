@@ -46,6 +46,40 @@ int msamd_version(void);
 const char* msamd_last_error(void);
 
 /* ------------------------------------------------------------------------
+ * Bandpass filter (a1-a3) -- replaces p_bandpass_filter with its
+ * Kernel_runner::init/apply and define_kernel
+ * (packages/ms3/p_bandpass_filter.cpp:47-101,115-229,421-447).
+ * ------------------------------------------------------------------------ */
+
+/* Core chunk length of the filter windows for M channels and the reference
+ * thread count (p_bandpass_filter.cpp:147-155); the FFT length is
+ * chunk + 2*overlap with overlap = 60000. */
+int64_t msamd_bandpass_chunk_size(int64_t M, int64_t num_threads);
+
+/* Gain H[i], i < n_fft, of define_kernel (FP64, host memory). */
+int msamd_bandpass_kernel(int64_t n_fft, double samplerate, double freq_min, double freq_max,
+                          double freq_wid, double* kernel_host);
+
+/* 1 when the FFT length chunk_size + 2*overlap has a compiled factorization. */
+int msamd_bandpass_supported(int64_t chunk_size, int64_t overlap);
+
+/* Workspace bytes for msamd_bandpass_filter (-1 when unsupported). */
+int64_t msamd_bandpass_workspace_size(int64_t M, int64_t N, int64_t chunk_size, int64_t overlap);
+
+/* Filters X (M x N, dtype MSAMD_DTYPE_FLOAT32 or MSAMD_DTYPE_INT16) window by
+ * window exactly as the reference: window [t-overlap, t+chunk+overlap),
+ * zero outside [0, N), DFT, times H/n, inverse DFT, core kept.  FP32
+ * arithmetic.  subsample_factor k > 1 keeps every k-th core sample
+ * (p_bandpass_filter.cpp:480-491); Y has N/k time points.  Y is float32, or
+ * int16 when quantization_unit > 0 (y * (1/q) rounded to float, then
+ * truncated, :207-209).  freq_max == 0 (a plain copy in the reference) is an
+ * error here. */
+int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_t N, double samplerate,
+                          double freq_min, double freq_max, double freq_wid, int64_t chunk_size,
+                          int64_t overlap, int subsample_factor, double quantization_unit,
+                          void* Y, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Event detection (a10) -- replaces detect_events()
  * (packages/mountainsortalg/detect_events.cpp:69-135) as driven by
  * NeighborhoodSorter::addTimeChunk (neighborhoodsorter.cpp:74-106) over the

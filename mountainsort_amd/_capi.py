@@ -18,6 +18,12 @@ i64, i32, f64, vp, cint = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctype
 SIGNATURES = {
     "msamd_version": (cint, []),
     "msamd_last_error": (ctypes.c_char_p, []),
+    "msamd_bandpass_chunk_size": (i64, [i64, i64]),
+    "msamd_bandpass_kernel": (cint, [i64, f64, f64, f64, f64, vp]),
+    "msamd_bandpass_supported": (cint, [i64, i64]),
+    "msamd_bandpass_workspace_size": (i64, [i64, i64, i64, i64]),
+    "msamd_bandpass_filter": (cint, [vp, cint, i64, i64, f64, f64, f64, f64, i64, i64, cint, f64,
+                                     vp, vp, i64, vp]),
     "msamd_detect_chunk_size": (i64, [i64, i64, i64]),
     "msamd_detect_workspace_size": (i64, [i64, i64, i32, f64]),
     "msamd_detect_events": (cint, [vp, i64, i64, i64, i64, i64, i64, vp, i32, f64, f64, cint, vp,

@@ -1,0 +1,491 @@
+// FFT bandpass filter (a1-a3) with the reference's exact windows.
+//
+// Reference: p_bandpass_filter (packages/ms3/p_bandpass_filter.cpp:115-229),
+// Kernel_runner::apply (:77-101), define_kernel (:421-447).  For every core
+// chunk [t, t+cs) the window [t-ov, t+cs+ov) (zero outside the recording) is
+// transformed with an n = cs + 2*ov point DFT, scaled by H[i]/n, inverse
+// transformed, and the core kept.  The gain is even (H[i] == H[n-i] for even
+// n), so the filter maps real to real: two channels are packed into one
+// complex sequence z = x_a + i*x_b and come back as Re / Im.
+//
+// The n-point transform is a four-step FFT, n = N1*N2 with n = n1*N2 + n2 and
+// k = k1 + N1*k2:
+//   stage A  for each (n2, pair): N1-point DFT over n1, times W_n^{n2*k1}
+//   stage B  for each (k1, pair): N2-point DFT over n2, times H[k1+N1*k2]/n,
+//            inverse N2-point DFT
+//   stage C  for each (n2, pair): times W_n^{-n2*k1}, inverse N1-point DFT,
+//            core samples written out.
+// Each sub-transform N = R1*R2 runs as two Stockham radix passes held in
+// registers (fft_radix.h) with one LDS exchange between them.  Windows are
+// processed in units of (channel group, window) whose scratch spectra stay
+// in the MI355X last-level cache between the three stages.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "fft_radix.h"
+
+namespace msamd {
+namespace {
+
+using fft::cf;
+using fft::cmul;
+using fft::cconjmul;
+
+constexpr int kPB = 16;          // channel pairs per workgroup
+constexpr int kTwLo = 512;       // four-step twiddle: W = hi[e >> 9] * lo[e & 511]
+
+struct BpPlan {
+    int64_t M, N, cs, ov, n, N_out;
+    int32_t k_sub;
+    int32_t out_i16;
+    int32_t in_i16;
+    double qscale;   // 1/quantization_unit (0 -> float output)
+    int32_t N1, N2;
+    int32_t P;       // channel pairs
+    int32_t CG;      // pairs per unit (multiple of kPB)
+    int32_t nGroups;
+    int64_t nWin, nUnits;
+    int32_t U;       // units per batch
+    int32_t nTwHi;
+};
+
+// Factorizations with kernels compiled in: n = (R1*R2) * (Q1*Q2).
+struct Shape {
+    int R1, R2, Q1, Q2;
+};
+constexpr Shape kShapes[] = {
+    {20, 20, 24, 25},  // 400 x 600 = 240000 (chunk 120000, overlap 60000)
+};
+
+// ------------------------------------------------------------------ stage A
+// Workgroup: one n2, kPB pairs of one unit; thread (j, pl) with pl fastest.
+// pass 1: radix R1 over x[j + r*R2] (R2 tasks), pass 2: radix R2 (R1 tasks).
+template <int R1, int R2, bool kI16>
+__global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
+    const void* __restrict__ Xv, BpPlan p, int64_t unit0, const cf* __restrict__ twN1,
+    const cf* __restrict__ twHi, const cf* __restrict__ twLo, cf* __restrict__ S) {
+    constexpr int N1 = R1 * R2;
+    constexpr int TPS = R1 > R2 ? R1 : R2;
+    __shared__ cf buf[N1 * kPB];
+    __shared__ cf tw1[N1];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
+    const int pl = tid % kPB;
+    const int j = tid / kPB;
+    // block -> (unit, pair block, n2)
+    const int npb = p.CG / kPB;
+    int64_t b = blockIdx.x;
+    const int n2 = static_cast<int>(b % p.N2);
+    b /= p.N2;
+    const int pb = static_cast<int>(b % npb);
+    const int64_t u = unit0 + b / npb;
+    const int g = static_cast<int>(u / p.nWin);
+    const int64_t w = u % p.nWin;
+    const int pair = g * p.CG + pb * kPB + pl;   // global pair index
+    const int ch0 = 2 * pair;
+    const int64_t tbase = w * p.cs - p.ov;       // time of window sample 0
+    cf v[R1];
+    if (j < R2) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int64_t tl = static_cast<int64_t>(j + r * R2) * p.N2 + n2;
+            const int64_t t = tbase + tl;
+            float a = 0.f, c = 0.f;
+            if (pair < p.P && t >= 0 && t < p.N) {
+                if (kI16) {
+                    const int16_t* X = static_cast<const int16_t*>(Xv) + t * p.M + ch0;
+                    a = static_cast<float>(X[0]);
+                    if (ch0 + 1 < p.M) c = static_cast<float>(X[1]);
+                } else {
+                    const float* X = static_cast<const float*>(Xv) + t * p.M + ch0;
+                    a = X[0];
+                    if (ch0 + 1 < p.M) c = X[1];
+                }
+            }
+            v[r] = {a, c};
+        }
+        fft::Dft<R1, -1>::run(v);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) buf[(j * R1 + r) * kPB + pl] = v[r];
+    }
+    __syncthreads();
+    if (j < R1) {
+        cf y[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const cf x = buf[(j + r * R1) * kPB + pl];
+            y[r] = (r == 0 || j == 0) ? x : cmul(x, tw1[r * j]);
+        }
+        fft::Dft<R2, -1>::run(y);
+        cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const int k1 = j + r * R1;
+            const int e = n2 * k1;  // < n, no reduction needed
+            const cf w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kPB + pl] = cmul(y[r], w4);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stage B
+// Workgroup: one k1, kPB pairs of one unit.  Forward N2 = Q1*Q2 transform
+// (pass radix Q1 then Q2), gain, inverse (radix Q2 on the same registers,
+// then radix Q1), written back in place.
+template <int Q1, int Q2>
+__global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
+    BpPlan p, int64_t unit0, const cf* __restrict__ twN2, const float* __restrict__ Ht,
+    cf* __restrict__ S) {
+    constexpr int N2 = Q1 * Q2;
+    __shared__ cf buf[N2 * kPB];
+    __shared__ cf tw2[N2];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N2; q += blockDim.x) tw2[q] = twN2[q];
+    const int pl = tid % kPB;
+    const int j = tid / kPB;
+    const int npb = p.CG / kPB;
+    int64_t b = blockIdx.x;
+    const int k1 = static_cast<int>(b % p.N1);
+    b /= p.N1;
+    const int pb = static_cast<int>(b % npb);
+    const int64_t u = unit0 + b / npb;
+    cf* Srow = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG +
+               static_cast<int64_t>(k1) * N2 * p.CG + pb * kPB + pl;
+    // forward pass 1: radix Q1 over n2 = j + r*Q2 (Q2 tasks)
+    if (j < Q2) {
+        cf v[Q1];
+#pragma unroll
+        for (int r = 0; r < Q1; ++r) v[r] = Srow[static_cast<int64_t>(j + r * Q2) * p.CG];
+        fft::Dft<Q1, -1>::run(v);
+#pragma unroll
+        for (int r = 0; r < Q1; ++r) buf[(j * Q1 + r) * kPB + pl] = v[r];
+    }
+    __syncthreads();
+    // forward pass 2: radix Q2 (Q1 tasks) -> k2 = j + r*Q1; gain; inverse pass 1
+    // (radix Q2 over the same elements, Ns = 1) -> positions j*Q2 + r
+    cf y[Q2];
+    if (j < Q1) {
+#pragma unroll
+        for (int r = 0; r < Q2; ++r) {
+            const cf x = buf[(j + r * Q1) * kPB + pl];
+            y[r] = (r == 0 || j == 0) ? x : cmul(x, tw2[r * j]);
+        }
+        fft::Dft<Q2, -1>::run(y);
+        const float* Hrow = Ht + static_cast<int64_t>(k1) * N2;
+#pragma unroll
+        for (int r = 0; r < Q2; ++r) {
+            const float h = Hrow[j + r * Q1];
+            y[r] = {y[r].x * h, y[r].y * h};
+        }
+        fft::Dft<Q2, 1>::run(y);
+    }
+    __syncthreads();
+    if (j < Q1) {
+#pragma unroll
+        for (int r = 0; r < Q2; ++r) buf[(j * Q2 + r) * kPB + pl] = y[r];
+    }
+    __syncthreads();
+    // inverse pass 2: radix Q1, Ns = Q2 (Q2 tasks): inputs j + r*Q2, twiddle
+    // exp(+2 pi i r j / N2), outputs n2 = j + r*Q2
+    if (j < Q2) {
+        cf v[Q1];
+#pragma unroll
+        for (int r = 0; r < Q1; ++r) {
+            const cf x = buf[(j + r * Q2) * kPB + pl];
+            v[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw2[r * j]);
+        }
+        fft::Dft<Q1, 1>::run(v);
+#pragma unroll
+        for (int r = 0; r < Q1; ++r) Srow[static_cast<int64_t>(j + r * Q2) * p.CG] = v[r];
+    }
+}
+
+// ------------------------------------------------------------------ stage C
+// Workgroup: one n2, kPB pairs.  Inverse over k1: pass 1 radix R2 over
+// k1 = j + r*R1 (R1 tasks, after the conjugate four-step twiddle), pass 2
+// radix R1 (Ns = R2, R2 tasks) -> n1 = j + r*R2.
+template <int R1, int R2, bool kOutI16>
+__global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
+    BpPlan p, int64_t unit0, const cf* __restrict__ twN1, const cf* __restrict__ twHi,
+    const cf* __restrict__ twLo, const cf* __restrict__ S, void* __restrict__ Yv) {
+    constexpr int N1 = R1 * R2;
+    __shared__ cf buf[N1 * kPB];
+    __shared__ cf tw1[N1];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
+    const int pl = tid % kPB;
+    const int j = tid / kPB;
+    const int npb = p.CG / kPB;
+    int64_t b = blockIdx.x;
+    const int n2 = static_cast<int>(b % p.N2);
+    b /= p.N2;
+    const int pb = static_cast<int>(b % npb);
+    const int64_t u = unit0 + b / npb;
+    const int g = static_cast<int>(u / p.nWin);
+    const int64_t w = u % p.nWin;
+    const int pair = g * p.CG + pb * kPB + pl;
+    const int ch0 = 2 * pair;
+    const cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+    if (j < R1) {
+        cf v[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const int k1 = j + r * R1;
+            const int e = n2 * k1;
+            const cf w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            v[r] = cconjmul(Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kPB + pl], w4);
+        }
+        fft::Dft<R2, 1>::run(v);
+#pragma unroll
+        for (int r = 0; r < R2; ++r) buf[(j * R2 + r) * kPB + pl] = v[r];
+    }
+    __syncthreads();
+    if (j < R2) {
+        cf y[R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const cf x = buf[(j + r * R2) * kPB + pl];
+            y[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw1[r * j]);
+        }
+        fft::Dft<R1, 1>::run(y);
+        if (pair < p.P) {
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const int64_t n1 = j + r * R2;
+                const int64_t tl = n1 * p.N2 + n2 - p.ov;   // core-relative
+                if (tl < 0 || tl >= p.cs) continue;
+                if (tl % p.k_sub) continue;
+                if (tl / p.k_sub >= p.cs / p.k_sub) continue;
+                const int64_t to = (w * p.cs) / p.k_sub + tl / p.k_sub;
+                if (to >= p.N_out) continue;
+                const float vals[2] = {y[r].x, y[r].y};
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int64_t ch = ch0 + c;
+                    if (ch >= p.M) break;
+                    if (kOutI16) {
+                        // multiply_by_factor (:207-209, float *= double), then the
+                        // float -> int16 conversion of the writer (truncation)
+                        const float s = static_cast<float>(static_cast<double>(vals[c]) * p.qscale);
+                        static_cast<int16_t*>(Yv)[to * p.M + ch] = static_cast<int16_t>(static_cast<int>(s));
+                    } else {
+                        static_cast<float*>(Yv)[to * p.M + ch] = vals[c];
+                    }
+                }
+            }
+        }
+    }
+}
+
+bool pick_shape(int64_t n, Shape* s) {
+    for (const Shape& sh : kShapes) {
+        if (static_cast<int64_t>(sh.R1) * sh.R2 * sh.Q1 * sh.Q2 == n) {
+            *s = sh;
+            return true;
+        }
+    }
+    return false;
+}
+
+BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double qunit,
+                 bool in_i16, const Shape& sh, int64_t scratch_budget) {
+    BpPlan p{};
+    p.M = M;
+    p.N = N;
+    p.cs = cs;
+    p.ov = ov;
+    p.n = cs + 2 * ov;
+    p.k_sub = k_sub;
+    p.N_out = N / k_sub;
+    p.qscale = qunit != 0 ? 1.0 / qunit : 0.0;
+    p.out_i16 = qunit != 0;
+    p.in_i16 = in_i16;
+    p.N1 = sh.R1 * sh.R2;
+    p.N2 = sh.Q1 * sh.Q2;
+    p.P = static_cast<int32_t>((M + 1) / 2);
+    p.CG = static_cast<int32_t>(align_up(p.P < 32 ? p.P : 32, kPB));
+    p.nGroups = static_cast<int32_t>(ceil_div(p.P, p.CG));
+    p.nWin = ceil_div(N, cs);
+    p.nUnits = p.nWin * p.nGroups;
+    const int64_t per_unit = p.n * p.CG * static_cast<int64_t>(sizeof(cf));
+    int64_t U = scratch_budget / per_unit;
+    if (U < 1) U = 1;
+    if (U > p.nUnits) U = p.nUnits;
+    p.U = static_cast<int32_t>(U);
+    p.nTwHi = static_cast<int32_t>(ceil_div(p.n, kTwLo));
+    return p;
+}
+
+struct BpBuffers {
+    cf* S;
+    float* Ht;
+    cf* twN1;
+    cf* twN2;
+    cf* twHi;
+    cf* twLo;
+};
+
+BpBuffers bp_carve(void* ws, const BpPlan& p, int64_t* bytes) {
+    Carver c(ws);
+    BpBuffers b{};
+    b.S = c.take<cf>(static_cast<int64_t>(p.U) * p.n * p.CG);
+    b.Ht = c.take<float>(p.n);
+    b.twN1 = c.take<cf>(p.N1);
+    b.twN2 = c.take<cf>(p.N2);
+    b.twHi = c.take<cf>(p.nTwHi);
+    b.twLo = c.take<cf>(kTwLo);
+    if (bytes) *bytes = c.bytes();
+    return b;
+}
+
+constexpr int64_t kScratchBudget = 128LL << 20;  // spectra kept in the last-level cache
+
+}  // namespace
+}  // namespace msamd
+
+using namespace msamd;
+
+extern "C" int64_t msamd_bandpass_chunk_size(int64_t M, int64_t num_threads) {
+    // p_bandpass_filter.cpp:147-155
+    if (M < 1) M = 1;
+    if (num_threads < 1) num_threads = 1;
+    const int overhead = 6;
+    const int64_t overlap_size = 60000;
+    const int64_t min_chunk_size = overlap_size * 2;
+    const double target_ram_bytes = 1.0 * (1024 * 1024 * 1024);
+    const double target = target_ram_bytes / (overhead * sizeof(float) * M * num_threads) - 2 * overlap_size;
+    int64_t chunk = static_cast<int64_t>(target);
+    if (chunk < min_chunk_size) chunk = min_chunk_size;
+    return chunk;
+}
+
+extern "C" int msamd_bandpass_kernel(int64_t n_fft, double samplerate, double freq_min,
+                                     double freq_max, double freq_wid, double* kernel_host) {
+    // define_kernel, p_bandpass_filter.cpp:421-447 (host, FP64, C library erf)
+    clear_error();
+    MSAMD_REQUIRE(n_fft >= 1 && kernel_host, "bandpass_kernel: bad arguments");
+    const double T = n_fft / samplerate;
+    const double df = 1 / T;
+    const double relwid = 3.0;
+    for (int64_t i = 0; i < n_fft; ++i) {
+        const double fgrid = (i <= (n_fft + 1) / 2) ? df * i : df * (i - n_fft);
+        const double absf = std::fabs(fgrid);
+        double val = 1.0;
+        if (freq_min != 0) {
+            if (i == 0)
+                val = 0.0;
+            else
+                val *= (1 + std::erf(relwid * (absf - freq_min) / freq_min)) / 2;
+        }
+        if (freq_max != 0) val *= (1 - std::erf((absf - freq_max) / freq_wid)) / 2;
+        kernel_host[i] = std::sqrt(val);
+    }
+    return MSAMD_OK;
+}
+
+extern "C" int msamd_bandpass_supported(int64_t chunk_size, int64_t overlap) {
+    Shape s;
+    return pick_shape(chunk_size + 2 * overlap, &s) ? 1 : 0;
+}
+
+extern "C" int64_t msamd_bandpass_workspace_size(int64_t M, int64_t N, int64_t chunk_size,
+                                                 int64_t overlap) {
+    Shape sh;
+    if (M < 1 || N < 0 || chunk_size < 1 || !pick_shape(chunk_size + 2 * overlap, &sh)) return -1;
+    const BpPlan p = make_plan(M, N > 0 ? N : 1, chunk_size, overlap, 1, 0.0, false, sh,
+                               kScratchBudget);
+    int64_t bytes = 0;
+    bp_carve(nullptr, p, &bytes);
+    return bytes;
+}
+
+extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_t N,
+                                     double samplerate, double freq_min, double freq_max,
+                                     double freq_wid, int64_t chunk_size, int64_t overlap,
+                                     int subsample_factor, double quantization_unit, void* Y,
+                                     void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && N >= 0, "bandpass: bad shape");
+    MSAMD_REQUIRE(dtype == MSAMD_DTYPE_FLOAT32 || dtype == MSAMD_DTYPE_INT16,
+                  "bandpass: input dtype %d not supported (float32 or int16)", dtype);
+    MSAMD_REQUIRE(freq_max != 0, "bandpass: freq_max == 0 is the copy path of the processor");
+    MSAMD_REQUIRE(chunk_size >= 1 && overlap >= 0, "bandpass: bad chunk plan");
+    MSAMD_REQUIRE(samplerate > 0, "bandpass: samplerate must be positive");
+    const int k_sub = subsample_factor > 0 ? subsample_factor : 1;
+    Shape sh;
+    if (!pick_shape(chunk_size + 2 * overlap, &sh)) {
+        set_error("bandpass: FFT length %lld has no compiled factorization",
+                  (long long)(chunk_size + 2 * overlap));
+        return MSAMD_ERR_UNSUPPORTED;
+    }
+    const BpPlan p = make_plan(M, N, chunk_size, overlap, k_sub, quantization_unit,
+                               dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget);
+    int64_t need = 0;
+    BpBuffers b = bp_carve(workspace, p, &need);
+    if (!workspace || workspace_bytes < need) {
+        set_error("bandpass: workspace too small (%lld < %lld)", (long long)workspace_bytes,
+                  (long long)need);
+        return MSAMD_ERR_WORKSPACE;
+    }
+    if (N == 0) return MSAMD_OK;
+    hipStream_t st = as_stream(stream);
+    // host tables: gain H/n transposed to [k1][k2], Stockham twiddles,
+    // four-step twiddle factors (all computed in double, rounded once)
+    {
+        std::vector<double> H(p.n);
+        msamd_bandpass_kernel(p.n, samplerate, freq_min, freq_max, freq_wid, H.data());
+        const double factor = 1.0 / p.n;
+        std::vector<float> Ht(p.n);
+        for (int64_t k1 = 0; k1 < p.N1; ++k1)
+            for (int64_t k2 = 0; k2 < p.N2; ++k2)
+                Ht[k1 * p.N2 + k2] = static_cast<float>(H[k1 + p.N1 * k2] * factor);
+        auto table = [](int64_t len, int64_t period, int64_t step) {
+            std::vector<cf> t(len);
+            for (int64_t m = 0; m < len; ++m) {
+                const long double ang = -2.0L * 3.14159265358979323846264338327950288L *
+                                        static_cast<long double>((m * step) % period) / period;
+                t[m] = {static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang))};
+            }
+            return t;
+        };
+        const std::vector<cf> t1 = table(p.N1, p.N1, 1);
+        const std::vector<cf> t2 = table(p.N2, p.N2, 1);
+        const std::vector<cf> th = table(p.nTwHi, p.n, kTwLo);
+        const std::vector<cf> tl = table(kTwLo, p.n, 1);
+        MSAMD_HIP(hipMemcpyAsync(b.Ht, Ht.data(), sizeof(float) * p.n, hipMemcpyHostToDevice, st));
+        MSAMD_HIP(hipMemcpyAsync(b.twN1, t1.data(), sizeof(cf) * p.N1, hipMemcpyHostToDevice, st));
+        MSAMD_HIP(hipMemcpyAsync(b.twN2, t2.data(), sizeof(cf) * p.N2, hipMemcpyHostToDevice, st));
+        MSAMD_HIP(hipMemcpyAsync(b.twHi, th.data(), sizeof(cf) * p.nTwHi, hipMemcpyHostToDevice, st));
+        MSAMD_HIP(hipMemcpyAsync(b.twLo, tl.data(), sizeof(cf) * kTwLo, hipMemcpyHostToDevice, st));
+        MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
+    }
+    const int npb = p.CG / kPB;
+    for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
+        const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
+        const unsigned gridAC = static_cast<unsigned>(nu * npb * p.N2);
+        const unsigned gridB = static_cast<unsigned>(nu * npb * p.N1);
+        if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
+            if (p.in_i16)
+                bp_stage_a<20, 20, true><<<gridAC, kPB * 20, 0, st>>>(X, p, u0, b.twN1, b.twHi,
+                                                                      b.twLo, b.S);
+            else
+                bp_stage_a<20, 20, false><<<gridAC, kPB * 20, 0, st>>>(X, p, u0, b.twN1, b.twHi,
+                                                                       b.twLo, b.S);
+            MSAMD_LAUNCH_CHECK();
+            bp_stage_b<24, 25><<<gridB, kPB * 25, 0, st>>>(p, u0, b.twN2, b.Ht, b.S);
+            MSAMD_LAUNCH_CHECK();
+            if (p.out_i16)
+                bp_stage_c<20, 20, true><<<gridAC, kPB * 20, 0, st>>>(p, u0, b.twN1, b.twHi,
+                                                                      b.twLo, b.S, Y);
+            else
+                bp_stage_c<20, 20, false><<<gridAC, kPB * 20, 0, st>>>(p, u0, b.twN1, b.twHi,
+                                                                       b.twLo, b.S, Y);
+            MSAMD_LAUNCH_CHECK();
+        } else {
+            set_error("bandpass: shape not instantiated");
+            return MSAMD_ERR_UNSUPPORTED;
+        }
+    }
+    return MSAMD_OK;
+}

@@ -49,6 +49,50 @@ def _workspace(nbytes, device):
 
 
 # --------------------------------------------------------------------------
+# bandpass filter (a1-a3)
+# --------------------------------------------------------------------------
+def bandpass_chunk_size(M, num_threads):
+    """Core chunk of the filter windows (p_bandpass_filter.cpp:147-155)."""
+    return int(_capi.lib().msamd_bandpass_chunk_size(int(M), int(num_threads)))
+
+
+def bandpass_filter(X, samplerate, freq_min, freq_max, freq_wid=1000.0, quantization_unit=0.0,
+                    subsample_factor=1, num_threads=8, chunk_size=None, overlap=BANDPASS_OVERLAP,
+                    out=None):
+    """ms3.bandpass_filter on a device-resident (N, M) float32 or int16 tensor.
+
+    Windows follow the reference's chunk plan for `num_threads`
+    (p_bandpass_filter.cpp:147-155) unless chunk_size is given.  Returns
+    float32 (N/k, M), or int16 when quantization_unit > 0.  freq_max == 0 is
+    the reference's file copy (:117-119) and returns a copy here.
+    """
+    if not isinstance(X, torch.Tensor) or not X.is_cuda or X.dim() != 2 or not X.is_contiguous():
+        raise ValueError("timeseries must be a contiguous CUDA tensor of shape (N, M)")
+    if freq_max == 0:
+        return X.clone()
+    dtypes = {torch.float32: -3, torch.int16: -4}
+    if X.dtype not in dtypes:
+        raise ValueError("timeseries dtype must be float32 or int16")
+    N, M = X.shape
+    dev = X.device
+    k = int(subsample_factor) if subsample_factor else 1
+    cs = int(chunk_size) if chunk_size else bandpass_chunk_size(M, num_threads)
+    L = _capi.lib()
+    nb = L.msamd_bandpass_workspace_size(M, N, cs, int(overlap))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_bandpass_workspace_size", 5,
+                               f"FFT length {cs + 2 * int(overlap)} not supported")
+    ws = _workspace(nb, dev)
+    q = float(quantization_unit)
+    if out is None:
+        out = torch.empty((N // k, M), dtype=torch.int16 if q else torch.float32, device=dev)
+    _capi.call("msamd_bandpass_filter", _ptr(X), dtypes[X.dtype], M, N, float(samplerate),
+               float(freq_min), float(freq_max), float(freq_wid), cs, int(overlap), k, q,
+               _ptr(out), _ptr(ws), ws.numel(), _stream(dev))
+    return out
+
+
+# --------------------------------------------------------------------------
 # detection (a8-a10)
 # --------------------------------------------------------------------------
 def detect_chunk_size(M, N, num_threads):

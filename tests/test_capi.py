@@ -51,3 +51,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_capi, "_lib", None)
     with pytest.raises(_capi.MsamdLibraryMissing):
         _capi.lib()
+
+
+def test_bandpass_host_entry_points():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libmsamd.so not built")
+    import numpy as np
+    L = _capi.lib()
+    for M, T in [(64, 8), (64, 2), (4, 8), (384, 1)]:
+        assert L.msamd_bandpass_chunk_size(M, T) == oc.bandpass_chunk_size(M, T)
+    H = np.empty(240000)
+    assert L.msamd_bandpass_kernel(240000, 30000.0, 300.0, 6000.0, 1000.0,
+                                   H.ctypes.data_as(ctypes.c_void_p)) == 0
+    np.testing.assert_array_equal(H, oc.define_kernel(240000, 30000, 300, 6000, 1000))
+    assert L.msamd_bandpass_supported(120000, 60000) == 1
+    assert L.msamd_bandpass_supported(1278101, 60000) == 0
+    assert L.msamd_bandpass_workspace_size(384, 9_000_000, 120000, 60000) > 0
