@@ -1,0 +1,197 @@
+"""Benchmark: channel-samples/s through filter -> mask -> whiten -> detect -> clips -> PCA.
+
+Workload (single GPU): the 384-channel 4-column dense probe, 30 kHz, 300 s
+(9,000,000 samples), int16 raw input at 0.195 uV/bit with 300 units and
+artifact bursts on 1% of samples (mountainsort_amd.synth.config2, seed 2),
+generated on the GPU and resident in HBM before timing.  One step is one
+pass of the whole hot path over the recording:
+  bandpass 300-6000 Hz (reference windows, 8-thread chunk plan)
+  -> mask_out_artifacts (threshold 6, interval 2000)
+  -> whiten (covariance, whitening matrix, apply)
+  -> detect (threshold 3, sign -1, interval 10, 8-thread chunk plan)
+  -> clips (T = 50, neighborhoods of radius 40 um)
+  -> PCA features (10 per channel, then 10 per neighborhood).
+
+Prints one JSON line (see the contract in the repository docs).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def neighborhoods(M, columns, pitch, radius):
+    """get_channels_from_geom for every channel (p_mountainsort3.cpp:473-491), 0-based."""
+    import numpy as np
+    from mountainsort_amd.synth import probe_geometry
+    geom = probe_geometry(M, columns, pitch)
+    d = np.sqrt(((geom[:, None, :] - geom[None, :, :]) ** 2).sum(-1))
+    nb = []
+    for m in range(M):
+        others = [m2 for m2 in range(M) if m2 != m and (radius < 0 or d[m, m2] <= radius)]
+        nb.append([m] + others)
+    return nb
+
+
+class Pipeline:
+    def __init__(self, dev, M, N, cfg, radius=40.0, num_threads=8, with_pca=True):
+        import numpy as np
+        import torch
+        from mountainsort_amd import frontend as fe
+        self.fe = fe
+        self.torch = torch
+        self.dev = dev
+        self.M, self.N = M, N
+        self.num_threads = num_threads
+        self.with_pca = with_pca and hasattr(fe, "pca_features")
+        nb = neighborhoods(M, cfg.columns, cfg.pitch, radius)
+        self.nb_channels = torch.tensor(np.concatenate(nb), dtype=torch.int32, device=dev)
+        self.nb_offsets = torch.tensor(np.concatenate([[0], np.cumsum([len(c) for c in nb])]),
+                                       dtype=torch.int64, device=dev)
+        self.widths = [len(c) for c in nb]
+        # preallocated stage outputs
+        self.filt = torch.empty((N, M), dtype=torch.float32, device=dev)
+        self.masked = torch.empty((N, M), dtype=torch.float32, device=dev)
+        self.white = torch.empty((N, M), dtype=torch.float32, device=dev)
+        self.stats = {}
+
+    def run(self, raw, timers=None):
+        fe, torch = self.fe, self.torch
+        ev = {}
+
+        def mark(name):
+            if timers is not None:
+                e = torch.cuda.Event(enable_timing=True)
+                e.record()
+                ev[name] = e
+
+        mark("start")
+        fe.bandpass_filter(raw, 30000, 300, 6000, 1000, num_threads=self.num_threads, out=self.filt)
+        mark("bandpass")
+        fe.mask_out_artifacts(self.filt, 6.0, 2000, out=self.masked)
+        mark("mask")
+        C = fe.whiten_covariance(self.masked)
+        mark("covariance")
+        W = fe.whitening_matrix(C)
+        mark("whitening_matrix")
+        fe.whiten_apply(self.masked, W, out=self.white)
+        mark("apply")
+        offsets, times = fe.detect_events(self.white, 3.0, 10, -1, num_threads=self.num_threads)
+        mark("detect")
+        clips, clip_offsets = fe.extract_clips_batched(self.white, times, offsets, self.nb_channels,
+                                                       self.nb_offsets, 50)
+        mark("clips")
+        feats = None
+        if self.with_pca:
+            feats = fe.pca_features(clips, clip_offsets, offsets, self.nb_offsets, 50)
+            mark("pca")
+        if timers is not None:
+            torch.cuda.synchronize()
+            names = list(ev.keys())
+            for a, b in zip(names[:-1], names[1:]):
+                timers.setdefault(b, []).append(ev[a].elapsed_time(ev[b]))
+        self.stats["events"] = int(times.numel())
+        self.stats["clip_floats"] = int(clips.numel())
+        return feats
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seconds", type=float, default=300.0, help="recording length")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pca", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is "
+                                   "described in DESIGN.md and not run"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from mountainsort_amd import synth
+
+    dev = torch.device("cuda:0")
+    N = int(round(args.seconds * 30000))
+    cfg = synth.config2(N=N, seed=2)
+    t0 = time.time()
+    raw = synth.generate(cfg, device=dev)
+    torch.cuda.synchronize()
+    log(f"[bench] generated {cfg.M} x {N} int16 in {time.time() - t0:.1f}s")
+    pipe = Pipeline(dev, cfg.M, N, cfg, with_pca=not args.no_pca)
+    for i in range(args.warmup):
+        pipe.run(raw)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    timers = {}
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    start_ev = torch.cuda.Event(enable_timing=True)
+    end_ev = torch.cuda.Event(enable_timing=True)
+    start_ev.record()
+    for i in range(args.steps):
+        pipe.run(raw, timers=timers)
+        log(f"[bench] step {i + 1}/{args.steps} done")
+    end_ev.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1000.0 / args.steps
+    chs = cfg.M * N
+    value = chs * args.steps / wall
+    stage_ms = {k: float(np.mean(v)) for k, v in timers.items()}
+    log("[bench] stage ms: " + json.dumps({k: round(v, 3) for k, v in stage_ms.items()}))
+
+    # algorithmic bytes per channel-sample of each stage (input once, output once)
+    alg_bytes = {"bandpass": 2 + 4, "mask": 4 + 4 + 4, "covariance": 4, "apply": 4 + 4,
+                 "detect": 4 + 4}
+    rl = {}
+    for k, b in alg_bytes.items():
+        if k in stage_ms:
+            gbs = chs * b / (stage_ms[k] * 1e-3) / 1e9
+            rl[k] = {"achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max((k for k in alg_bytes if k in stage_ms), key=lambda k: stage_ms[k])
+    total_alg = sum(alg_bytes.values()) * chs
+    line = {
+        "metric": "channel-samples/sec through filter->whiten->detect->clips->PCA; % HBM peak",
+        "value": value,
+        "unit": "channel-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"384-ch 4-column probe, 30 kHz, {args.seconds:g} s, int16 "
+                               "0.195 uV/bit, 300 units, 1% artifacts (synth.config2 seed 2)",
+                   "channels": cfg.M, "samples": N, "reference_threads": 8,
+                   "adjacency_radius_um": 40, "clip_size": 50, "events": pipe.stats.get("events"),
+                   "pca": pipe.with_pca},
+        "hbm_frac_pipeline": round(total_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+        "stage_roofline": rl,
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": rl[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rl[dom]["frac_hbm"], "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     const int pair = g * p.CG + pb * kPB + pl;   // global pair index
     const int ch0 = 2 * pair;
     const int64_t tbase = w * p.cs - p.ov;       // time of window sample 0
+    const bool even_m = (p.M % 2) == 0;
     cf v[R1];
     if (j < R2) {
 #pragma unroll
@@ -95,12 +96,24 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
             if (pair < p.P && t >= 0 && t < p.N) {
                 if (kI16) {
                     const int16_t* X = static_cast<const int16_t*>(Xv) + t * p.M + ch0;
-                    a = static_cast<float>(X[0]);
-                    if (ch0 + 1 < p.M) c = static_cast<float>(X[1]);
+                    if (even_m) {  // both channels in one aligned 32-bit word
+                        const uint32_t w2 = *reinterpret_cast<const uint32_t*>(X);
+                        a = static_cast<float>(static_cast<int16_t>(w2 & 0xffffu));
+                        c = static_cast<float>(static_cast<int16_t>(w2 >> 16));
+                    } else {
+                        a = static_cast<float>(X[0]);
+                        if (ch0 + 1 < p.M) c = static_cast<float>(X[1]);
+                    }
                 } else {
                     const float* X = static_cast<const float*>(Xv) + t * p.M + ch0;
-                    a = X[0];
-                    if (ch0 + 1 < p.M) c = X[1];
+                    if (even_m) {
+                        const float2 v2 = *reinterpret_cast<const float2*>(X);
+                        a = v2.x;
+                        c = v2.y;
+                    } else {
+                        a = X[0];
+                        if (ch0 + 1 < p.M) c = X[1];
+                    }
                 }
             }
             v[r] = {a, c};
@@ -260,6 +273,19 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
                 const int64_t to = (w * p.cs) / p.k_sub + tl / p.k_sub;
                 if (to >= p.N_out) continue;
                 const float vals[2] = {y[r].x, y[r].y};
+                if ((p.M % 2) == 0) {  // both channels of the pair in one store
+                    if (kOutI16) {
+                        const float s0 = static_cast<float>(static_cast<double>(vals[0]) * p.qscale);
+                        const float s1 = static_cast<float>(static_cast<double>(vals[1]) * p.qscale);
+                        const uint32_t lo = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s0)));
+                        const uint32_t hi = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s1)));
+                        *reinterpret_cast<uint32_t*>(static_cast<int16_t*>(Yv) + to * p.M + ch0) = lo | (hi << 16);
+                    } else {
+                        *reinterpret_cast<float2*>(static_cast<float*>(Yv) + to * p.M + ch0) =
+                            make_float2(vals[0], vals[1]);
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
                     const int64_t ch = ch0 + c;

@@ -36,31 +36,26 @@ __global__ void __launch_bounds__(256) clips_kernel(
 
 // Batched form: events of neighborhood g are times[eo[g]..eo[g+1]) (centers),
 // its channels nc[no[g]..no[g+1]), its clips start at clips + co[g].
+// blockIdx.y selects the neighborhood; the blocks along x stride over its
+// events, one wavefront per clip.
 __global__ void __launch_bounds__(256) clips_batched_kernel(
     const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ times,
-    const int64_t* __restrict__ eo, int64_t total, const int32_t* __restrict__ nc,
-    const int64_t* __restrict__ no, int32_t G, int32_t T, int32_t tmid,
-    const int64_t* __restrict__ co, float* __restrict__ clips) {
+    const int64_t* __restrict__ eo, const int32_t* __restrict__ nc,
+    const int64_t* __restrict__ no, int32_t T, int32_t tmid, const int64_t* __restrict__ co,
+    float* __restrict__ clips) {
+    const int g = blockIdx.y;
     const int lane = threadIdx.x & 63;
+    const int64_t e0 = eo[g];
+    const int64_t L = eo[g + 1] - e0;
+    const int32_t* ch = nc + no[g];
+    const int32_t W = static_cast<int32_t>(no[g + 1] - no[g]);
+    const int32_t WT = W * T;
+    float* base = clips + co[g];
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
-    for (int64_t e = wave; e < total; e += nwaves) {
-        // neighborhood of global event e: last g with eo[g] <= e
-        int32_t lo = 0, hi = G - 1;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi + 1) >> 1;
-            if (eo[mid] <= e)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int32_t g = lo;
-        const int64_t i = e - eo[g];
-        const int32_t* ch = nc + no[g];
-        const int32_t W = static_cast<int32_t>(no[g + 1] - no[g]);
-        const int32_t WT = W * T;
-        const int64_t t0 = times[e] - tmid;
-        float* out = clips + co[g] + i * WT;
+    for (int64_t i = wave; i < L; i += nwaves) {
+        const int64_t t0 = times[e0 + i] - tmid;
+        float* out = base + i * WT;
         for (int32_t r = lane; r < WT; r += 64) {
             const int32_t t = r / W;
             const int32_t w = r - t * W;
@@ -108,9 +103,14 @@ extern "C" int msamd_extract_clips_batched(const float* X, int64_t M, int64_t N,
     MSAMD_REQUIRE(T >= 1, "extract_clips_batched: clip_size is zero");
     if (total_events == 0) return MSAMD_OK;
     const int32_t tmid = (T + 1) / 2 - 1;
-    clips_batched_kernel<<<grid_for_waves(total_events), 256, 0, as_stream(stream)>>>(
-        X, M, N, times, event_offsets, total_events, nbhd_channels, nbhd_offsets, G, T, tmid,
-        clip_offsets, clips);
+    // blocks per neighborhood sized for the mean event count (4 clips per block pass)
+    int64_t per = ceil_div(ceil_div(total_events, G), 16);
+    if (per < 1) per = 1;
+    if (per > 4096) per = 4096;
+    const dim3 grid(static_cast<unsigned>(per), static_cast<unsigned>(G));
+    clips_batched_kernel<<<grid, 256, 0, as_stream(stream)>>>(X, M, N, times, event_offsets,
+                                                              nbhd_channels, nbhd_offsets, T,
+                                                              tmid, clip_offsets, clips);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }

@@ -22,7 +22,8 @@
 namespace msamd {
 namespace {
 
-constexpr int kSeg = 1024;  // samples per scan segment (fits a uint16 offset)
+constexpr int kSeg = 1024;   // samples per (segment, channel) scan (fits a uint16 offset)
+constexpr int kB = 16;        // register batch; neighbours within +-kB come from registers
 
 struct DetectPlan {
     int64_t M, N_file, t_start, n_range, cs, pad, nchunks, spc, nseg;
@@ -131,66 +132,108 @@ __device__ __forceinline__ double detect_value(float x, double mean, int sign) {
     return v;
 }
 
-// One thread per (segment, channel): scan the segment's core samples, keep
-// the stencil winners, store their offsets in the slot list.
+// One thread per (segment, central channel), lanes over consecutive channels
+// so every load is one coalesced row.  The thread streams its segment in
+// batches of kB samples, loading two batches ahead.  The previous, current
+// and next batches also sit in a per-thread LDS ring (stride 49 floats, no
+// bank conflicts), from which the rare threshold crossings read their +-D
+// neighbours; lags beyond kB (detect_interval > 16) fall back to global
+// loads.
+__device__ __forceinline__ float load_x(const float* xp, int64_t t, int64_t M, int64_t N) {
+    return (t >= 0 && t < N) ? xp[t * M] : 0.0f;
+}
+
+constexpr int kRing = 3 * kB + 1;
+
 __global__ void __launch_bounds__(256) detect_scan_kernel(
     const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels, int sign,
     const double* __restrict__ mean_in, const double* __restrict__ thr_in,
     int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
+    __shared__ float ring_all[256 * kRing];
+    float* ring = ring_all + threadIdx.x * kRing;
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= p.nseg * p.Mc) return;
     const int j = static_cast<int>(g % p.Mc);
     const int64_t seg = g / p.Mc;
-    const int64_t c = seg / p.spc;
+    const int64_t ck = seg / p.spc;
     const int64_t s = seg % p.spc;
-    int64_t size = p.n_range - c * p.cs;
+    int64_t size = p.n_range - ck * p.cs;
     if (size > p.cs) size = p.cs;
     const int64_t r0 = s * kSeg;
     int64_t r1 = r0 + kSeg;
     if (r1 > size) r1 = size;
     int32_t cnt = 0;
     if (r0 < r1) {
-        const double mean = mean_in[c * p.Mc + j];
-        const double thr = thr_in[c * p.Mc + j];
-        const int64_t core0 = p.t_start + c * p.cs;   // absolute index of core start
-        const int64_t wa = core0 - p.pad;             // padded window [wa, wb)
+        const double mean = mean_in[ck * p.Mc + j];
+        const double thr = thr_in[ck * p.Mc + j];
+        const int64_t core0 = p.t_start + ck * p.cs;
+        const int64_t wa = core0 - p.pad;  // padded window [wa, wb)
         const int64_t wb = core0 + size + p.pad;
         const float* xp = X + channels[j];
         uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
-        auto load = [&](int64_t t) -> float {
-            return (t >= 0 && t < p.N_file) ? xp[t * p.M] : 0.0f;
-        };
-        constexpr int U = 8;
-        for (int64_t r = r0; r < r1; r += U) {
-            float xv[U];
+        const int64_t T0 = core0 + r0;
+        // ring slot of batch q (q = -1, 0, 1, ...) is ((q + 3) % 3) * kB
+        {
+            float a[kB], b[kB], c[kB];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t t = core0 + r + u;
-                xv[u] = (r + u < r1) ? load(t) : 0.0f;
+            for (int i = 0; i < kB; ++i) {
+                a[i] = load_x(xp, T0 - kB + i, p.M, p.N_file);
+                b[i] = load_x(xp, T0 + i, p.M, p.N_file);
+                c[i] = load_x(xp, T0 + kB + i, p.M, p.N_file);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (r + u >= r1) break;
-                const int64_t t = core0 + r + u;
-                const double v = detect_value(xv[u], mean, sign);
+            for (int i = 0; i < kB; ++i) {
+                ring[2 * kB + i] = a[i];
+                ring[i] = b[i];
+                ring[kB + i] = c[i];
+            }
+        }
+        const int D = p.D;
+        int q = 0;
+        for (int64_t rb = r0; rb < r1; rb += kB, ++q) {
+            const int64_t tb = core0 + rb;  // absolute time of cur[0]
+            float nn[kB];
+#pragma unroll
+            for (int i = 0; i < kB; ++i) nn[i] = load_x(xp, tb + 2 * kB + i, p.M, p.N_file);
+            const int base_slot = q % 3;  // slot of the current batch
+            const float* curp = ring + base_slot * kB;
+            for (int i = 0; i < kB; ++i) {
+                if (rb + i >= r1) break;
+                const double v = detect_value(curp[i], mean, sign);
                 if (!(v > thr)) continue;
+                const int64_t t = tb + i;
                 bool keep = true;
-                for (int d = 1; d <= p.D && keep; ++d) {
-                    const int64_t tu = t - d;
-                    if (tu < wa) break;
-                    if (detect_value(load(tu), mean, sign) >= v) keep = false;
+                for (int d = 1; keep && d <= D; ++d) {  // left: v must beat every earlier value
+                    if (t - d < wa) break;
+                    const int k = i - d;
+                    float xu;
+                    if (k >= -kB) {
+                        const int bq = k < 0 ? 2 : 0;  // batch offset -1 -> +2 (mod 3)
+                        xu = ring[((base_slot + bq) % 3) * kB + (k < 0 ? k + kB : k)];
+                    } else {
+                        xu = load_x(xp, t - d, p.M, p.N_file);
+                    }
+                    if (detect_value(xu, mean, sign) >= v) keep = false;
                 }
-                for (int d = 1; d <= p.D && keep; ++d) {
-                    const int64_t tu = t + d;
-                    if (tu >= wb) break;
-                    if (detect_value(load(tu), mean, sign) > v) keep = false;
+                for (int d = 1; keep && d <= D; ++d) {  // right: v must be >= every later value
+                    if (t + d >= wb) break;
+                    const int k = i + d;
+                    float xu;
+                    if (k < 2 * kB) {
+                        const int bq = k >= kB ? 1 : 0;
+                        xu = ring[((base_slot + bq) % 3) * kB + (k >= kB ? k - kB : k)];
+                    } else {
+                        xu = load_x(xp, t + d, p.M, p.N_file);
+                    }
+                    if (detect_value(xu, mean, sign) > v) keep = false;
                 }
                 // kept events are > D apart, so cnt < cap always holds
-                if (keep && cnt < p.cap) {
-                    my_slots[cnt] = static_cast<uint16_t>(r + u - r0);
-                    ++cnt;
-                }
+                if (keep && cnt < p.cap) my_slots[cnt++] = static_cast<uint16_t>(rb + i - r0);
             }
+            // batch q+2 replaces batch q-1 in the ring
+            const int ns = ((q + 2) % 3) * kB;
+#pragma unroll
+            for (int i = 0; i < kB; ++i) ring[ns + i] = nn[i];
         }
     }
     counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;

@@ -68,7 +68,19 @@ __global__ void mask_stats_kernel(const double* __restrict__ norms, int64_t M, i
     const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (m >= M) return;
     double s = 0.0, ss = 0.0;
-    for (int64_t i = 0; i < nint; ++i) {
+    int64_t i = 0;
+    constexpr int U = 16;
+    for (; i + U <= nint; i += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = norms[(i + u) * M + m];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s = s + v[u];
+            ss = ss + v[u] * v[u];
+        }
+    }
+    for (; i < nint; ++i) {
         const double v = norms[i * M + m];
         s = s + v;
         ss = ss + v * v;

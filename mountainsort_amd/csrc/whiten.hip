@@ -181,79 +181,163 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
 }
 
 // ------------------------------------------------------- whitening matrix
-// One workgroup runs the whole sequential power method.  G (M x M, symmetric)
-// lives in the workspace; deflation G <- (I-vv')G(I-vv') is applied in its
-// rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal to the
-// reference's two M^3 products in exact arithmetic.
-constexpr int kWThreads = 1024;
+// The power method is a chain of M * 11 dependent matrix-vector products, so
+// its speed is set by how fast one product and one exchange complete.  G is
+// split by rows over NP workgroups and held in their LDS (24 rows of 384
+// doubles = 74 KB each at M = 384); every power step each workgroup forms
+// its rows of w = G v plus its partial |w|^2, publishes them, and all
+// workgroups meet at a device-scope barrier before normalizing the full
+// vector (partials summed in workgroup order, so every workgroup holds the
+// same v).  Deflation G <- (I-vv')G(I-vv') is applied to each workgroup's
+// rows in the rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal
+// to the reference's two M^3 products in exact arithmetic.
+//
+// The kernel is launched cooperatively (all workgroups co-resident).  Each
+// barrier wait is bounded; on timeout the error flag is raised and every
+// workgroup leaves, so the kernel always terminates.
+constexpr int kWThreads = 256;
+constexpr unsigned kBarrierSpinLimit = 1u << 22;
 
-__device__ double block_sum(double x, double* red) {
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) red[wid] = x;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s = s + red[w];
-        red[32] = s;
-    }
-    __syncthreads();
-    s = red[32];
-    __syncthreads();
-    return s;
+struct WPlan {
+    int M, NP, RW;
+};
+
+inline WPlan w_plan(int64_t M) {
+    WPlan p{};
+    p.M = static_cast<int>(M);
+    int rw = static_cast<int>(ceil_div(M, 16));
+    // rows per workgroup so that rows + two vectors fit in ~150 KB of LDS
+    const int cap = static_cast<int>((19200 - 2 * M) / M);
+    if (rw > cap) rw = cap;
+    if (rw < 1) rw = 1;
+    p.RW = rw;
+    p.NP = static_cast<int>(ceil_div(M, rw));
+    return p;
 }
 
-// w = G v; G row-major symmetric in global memory, v and w in LDS.
-__device__ void wave_matvec(const double* __restrict__ G, int M, const double* v, double* w) {
+__device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __shared__ int failed;
+    if (threadIdx.x == 0) {
+        failed = 0;
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                ++spins > kBarrierSpinLimit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                failed = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return failed == 0;
+}
+
+struct WBufs {
+    double* wbuf;     // [2][M] published rows of G v
+    double* psum;     // [2][NP] published partial sums
+    unsigned* counter;
+    int* err;
+};
+
+// sum over this workgroup's rows of (G v)_r * y_r into w_out (rows), returns
+// the workgroup-local sum of w_r * z_r where z = w (norm) or v (v'u).
+__device__ double rows_matvec(const double* Gl, int M, int nrows, const double* v, double* w_out,
+                              const double* dotv, double* red) {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
-    for (int r = wid; r < M; r += nw) {
-        const double* row = G + static_cast<int64_t>(r) * M;
+    for (int r = wid; r < nrows; r += nw) {
+        const double* row = Gl + static_cast<int64_t>(r) * M;
         double acc = 0.0;
         for (int n = lane; n < M; n += 64) acc += row[n] * v[n];
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (lane == 0) w[r] = acc;
+        if (lane == 0) w_out[r] = acc;
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int r = 0; r < nrows; ++r) s = s + w_out[r] * (dotv ? dotv[r] : w_out[r]);
+        red[0] = s;
+    }
+    __syncthreads();
+    return red[0];
 }
 
 __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
-    double* __restrict__ G, int M, const double* __restrict__ v0, double* __restrict__ comp,
-    double* __restrict__ sig) {
+    const double* __restrict__ XXt, WPlan p, const double* __restrict__ v0,
+    double* __restrict__ comp, double* __restrict__ sig, WBufs bufs) {
     extern __shared__ double sh[];
-    double* v = sh;          // [M]
-    double* w = sh + M;      // [M]
-    double* red = sh + 2 * M;  // [33]
+    const int M = p.M;
+    const int wg = blockIdx.x;
+    const int r0 = wg * p.RW;
+    const int nrows = (M - r0) < p.RW ? (M - r0) : p.RW;
+    double* Gl = sh;                                    // [RW][M]
+    double* v = Gl + static_cast<int64_t>(p.RW) * M;     // [M]
+    double* u = v + M;                                  // [M]
+    double* wl = u + M;                                 // [RW]
+    double* red = wl + p.RW;                            // [2]
+    for (int64_t q = threadIdx.x; q < static_cast<int64_t>(nrows) * M; q += blockDim.x) {
+        const int r = static_cast<int>(q / M);
+        const int n = static_cast<int>(q - static_cast<int64_t>(r) * M);
+        Gl[q] = XXt[static_cast<int64_t>(r0 + r) * M + n];
+    }
+    unsigned epoch = 0;
     for (int k = 0; k < M; ++k) {
         for (int i = threadIdx.x; i < M; i += blockDim.x) v[i] = v0[i];
         __syncthreads();
         double sigma = 0.0;
-        for (int it = 0; it < 10; ++it) {
-            wave_matvec(G, M, v, w);
-            double part = 0.0;
-            for (int i = threadIdx.x; i < M; i += blockDim.x) part += w[i] * w[i];
-            sigma = sqrt(block_sum(part, red));
-            for (int i = threadIdx.x; i < M; i += blockDim.x) v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
+        for (int it = 0; it < 11; ++it) {
+            // it < 10: power step w = G v; it == 10: u = G v for the deflation
+            const bool defl = (it == 10);
+            const int par = epoch & 1;
+            const double part = rows_matvec(Gl, M, nrows, v, wl, defl ? v + r0 : nullptr, red);
+            for (int r = threadIdx.x; r < nrows; r += blockDim.x) bufs.wbuf[par * M + r0 + r] = wl[r];
+            if (threadIdx.x == 0) bufs.psum[par * p.NP + wg] = part;
+            ++epoch;
+            if (!grid_barrier(bufs.counter, epoch * static_cast<unsigned>(p.NP), bufs.err)) return;
+            // partials: loaded in parallel (after the acquire), summed in order
+            for (int q = threadIdx.x; q < p.NP; q += blockDim.x) u[q] = bufs.psum[par * p.NP + q];
             __syncthreads();
+            if (threadIdx.x == 0) {
+                double s = 0.0;
+                for (int q = 0; q < p.NP; ++q) s = s + u[q];
+                red[1] = s;
+            }
+            __syncthreads();
+            const double tot = red[1];
+            if (!defl) {
+                sigma = sqrt(tot);
+                for (int i = threadIdx.x; i < M; i += blockDim.x) {
+                    const double w = bufs.wbuf[par * M + i];
+                    v[i] = sigma != 0.0 ? w / sigma : w;
+                }
+                __syncthreads();
+                if (it == 9) {
+                    for (int r = threadIdx.x; r < nrows; r += blockDim.x)
+                        comp[static_cast<int64_t>(r0 + r) * M + k] = v[r0 + r];
+                    if (wg == 0 && threadIdx.x == 0) sig[k] = sigma;
+                }
+            } else {
+                const double alpha = tot;
+                for (int i = threadIdx.x; i < M; i += blockDim.x) u[i] = bufs.wbuf[par * M + i];
+                __syncthreads();
+                for (int64_t q = threadIdx.x; q < static_cast<int64_t>(nrows) * M; q += blockDim.x) {
+                    const int r = static_cast<int>(q / M);
+                    const int n = static_cast<int>(q - static_cast<int64_t>(r) * M);
+                    const int i = r0 + r;
+                    const double cross = v[i] * u[n] + u[i] * v[n];
+                    const double vv = v[i] * v[n];
+                    Gl[q] = (Gl[q] - cross) + alpha * vv;
+                }
+                __syncthreads();
+            }
         }
-        for (int i = threadIdx.x; i < M; i += blockDim.x) comp[static_cast<int64_t>(i) * M + k] = v[i];
-        if (threadIdx.x == 0) sig[k] = sigma;
-        // deflation
-        wave_matvec(G, M, v, w);  // u = G v
-        double part = 0.0;
-        for (int i = threadIdx.x; i < M; i += blockDim.x) part += v[i] * w[i];
-        const double alpha = block_sum(part, red);
-        const int64_t MM = static_cast<int64_t>(M) * M;
-        for (int64_t q = threadIdx.x; q < MM; q += blockDim.x) {
-            const int i = static_cast<int>(q / M);
-            const int j = static_cast<int>(q - static_cast<int64_t>(i) * M);
-            const double cross = v[i] * w[j] + w[i] * v[j];
-            const double vv = v[i] * v[j];
-            G[q] = (G[q] - cross) + alpha * vv;
-        }
-        __syncthreads();
     }
 }
 
@@ -521,23 +605,29 @@ extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, dou
 
 extern "C" int64_t msamd_whitening_matrix_workspace_size(int64_t M) {
     if (M < 1) return -1;
+    const WPlan wp = w_plan(M);
     Carver c(nullptr);
-    c.take<double>(M * M);  // G
     c.take<double>(M * M);  // components
     c.take<double>(M);      // sigma
     c.take<double>(M);      // v0
+    c.take<double>(2 * M);  // published G v rows
+    c.take<double>(2 * wp.NP);
+    c.take<int>(2);         // barrier counter, error flag
     return c.bytes();
 }
 
 extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* workspace,
                                       int64_t workspace_bytes, void* stream) {
     clear_error();
-    MSAMD_REQUIRE(M >= 1 && M <= 2048, "whitening_matrix: M=%lld outside 1..2048", (long long)M);
+    MSAMD_REQUIRE(M >= 1 && M <= 1024, "whitening_matrix: M=%lld outside 1..1024", (long long)M);
+    const WPlan wp0 = w_plan(M);
     Carver c(workspace);
-    double* G = c.take<double>(M * M);
     double* comp = c.take<double>(M * M);
     double* sig = c.take<double>(M);
     double* v0 = c.take<double>(M);
+    double* wbuf = c.take<double>(2 * M);
+    double* psum = c.take<double>(2 * wp0.NP);
+    int* flags = c.take<int>(2);
     if (!workspace || workspace_bytes < c.bytes()) {
         set_error("whitening_matrix: workspace too small (%lld < %lld)",
                   (long long)workspace_bytes, (long long)c.bytes());
@@ -565,10 +655,29 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     MSAMD_HIP(e1);
     MSAMD_HIP(e2);
     // XXt is symmetric: row-major and column-major views coincide.
-    MSAMD_HIP(hipMemcpyAsync(G, XXt, sizeof(double) * M * M, hipMemcpyDeviceToDevice, st));
-    const size_t lds = sizeof(double) * (2 * M + 33);
-    whitening_power_kernel<<<1, kWThreads, lds, st>>>(G, static_cast<int>(M), v0, comp, sig);
-    MSAMD_LAUNCH_CHECK();
+    const WPlan wp = w_plan(M);
+    WBufs wb{};
+    wb.wbuf = wbuf;
+    wb.psum = psum;
+    wb.counter = reinterpret_cast<unsigned*>(flags);
+    wb.err = flags + 1;
+    MSAMD_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), st));
+    const size_t lds = sizeof(double) * (static_cast<size_t>(wp.RW) * M + 2 * M + wp.RW + 2);
+    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    {
+        const double* xx = XXt;
+        void* args[] = {const_cast<double**>(&xx), const_cast<WPlan*>(&wp), &v0, &comp, &sig, &wb};
+        MSAMD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&whitening_power_kernel),
+                                             dim3(wp.NP), dim3(kWThreads), args, lds, st));
+    }
+    int herr = 0;
+    MSAMD_HIP(hipMemcpyAsync(&herr, flags + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    MSAMD_HIP(hipStreamSynchronize(st));
+    if (herr) {
+        set_error("whitening_matrix: workgroup barrier timed out");
+        return MSAMD_ERR_HIP;
+    }
     whitening_assemble_kernel<<<static_cast<unsigned>(ceil_div(M * M, 256)), 256, 0, st>>>(
         comp, sig, static_cast<int>(M), W);
     MSAMD_LAUNCH_CHECK();
