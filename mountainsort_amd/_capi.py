@@ -32,6 +32,8 @@ SIGNATURES = {
     "msamd_extract_clips": (cint, [vp, i64, i64, vp, i64, vp, i32, i32, vp, vp]),
     "msamd_extract_clips_batched": (cint, [vp, i64, i64, vp, vp, i64, vp, vp, i32, i32, vp, vp,
                                            vp]),
+    "msamd_pca_workspace_size": (i64, [i32, vp, vp, i32, i32, i32, i64]),
+    "msamd_pca_features": (cint, [vp, i32, vp, vp, i32, i32, i32, i64, vp, vp, vp, i64, vp]),
     "msamd_whiten_covariance_workspace_size": (i64, [i64, i64]),
     "msamd_whiten_covariance": (cint, [vp, i64, i64, vp, vp, i64, vp]),
     "msamd_whitening_matrix_workspace_size": (i64, [i64]),

@@ -1,0 +1,610 @@
+// Two-stage per-neighborhood PCA features (a12-a15), batched over all
+// neighborhoods.
+//
+// Reference: NeighborhoodSorterPrivate::dimension_reduce_clips
+// (packages/mountainsortalg/neighborhoodsorter.cpp:236-276) -> pca_subsampled
+// (pca.cpp:99-127) -> pca (pca.cpp:68-97), then the stage-2 pca_subsampled of
+// sort_clips_subset (sort_clips.cpp:58-70) whose features go to isosplit5.
+//
+// Stage 1, per neighborhood g and channel w: A = clips[w, :, :] (T x L);
+// stage 2, per g: R = reduced clips viewed as (nf1*W) x L.  Each PCA:
+//   columns: all L, or the pca_subsampled indices (int64)i for
+//            i = 0, inc, 2inc, ... < L, inc = L/max (double accumulation)
+//   Gram:    G = A_s A_s^T  -- split-BF16 3-term MFMA, FP32 accumulation
+//   power:   per component k < K: v = normalize(sin(1..n)); 10 times
+//            v <- G v / |G v| (sigma = |G v|); deflate G <- (I-vv')G(I-vv')
+//            (FP64; the reference iterates X X^T v on the data, equal in
+//            exact arithmetic)
+//   project: F = C^T A on all L columns (FP32 FMA).
+// Phases run as one launch each over every neighborhood (items sized for
+// load balance: Gram and power per neighborhood / channel, projections per
+// chunk of events).
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace msamd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kKC = 32;        // subsampled columns per MFMA K step
+constexpr int kMaxW = 16;      // channels per neighborhood supported
+constexpr int kMaxT = 64;      // clip length supported
+constexpr int kProjE1 = 32;    // events per stage-1 projection item
+constexpr int kProjE2 = 64;    // events per stage-2 projection item
+constexpr int kMaxTiles = 40;  // Gram tiles per wavefront
+
+struct PcaPlan {
+    int32_t G, T, nf1, nf2;
+    int64_t max_s;
+    int32_t maxW;
+    int64_t idx_stride;  // max_s + 2
+};
+
+struct PcaTables {
+    const int64_t* clip_off;   // [G] floats
+    const int64_t* red_off;    // [G] floats (W*nf1*L)
+    const int64_t* feat_off;   // [G] floats (nf2*L)
+    const int32_t* L;          // [G]
+    const int32_t* W;          // [G]
+    const int64_t* g1_off;     // [G] floats: W*T*T
+    const int64_t* g2_off;     // [G] floats: (nf1*W)^2
+    const int64_t* c1_off;     // [G] floats: W*T*nf1
+    const int64_t* c2_off;     // [G] floats: nf1*W*nf2
+    const int32_t* p1_items;   // [n1][2] (g, w)
+    const int32_t* j1_items;   // [nj1][2] (g, first event)
+    const int32_t* j2_items;   // [nj2][2] (g, first event)
+    int32_t* ns;               // [G] subsampled column count
+    int32_t* idx;              // [G][idx_stride]
+    float* g1;                 // Gram stage 1
+    float* g2;                 // Gram stage 2
+    float* c1;                 // components stage 1, [w][t][a]
+    float* c2;                 // components stage 2, [r][a]
+    const double* sin_tab;     // sin(i+1), i < 1024
+    double* work2;             // FP64 stage-2 Gram copies when n > kPowMaxN
+    const int64_t* w2_off;     // [G] doubles
+};
+
+// ----------------------------------------------------- subsample indices
+// pca_subsampled, pca.cpp:106-115: one thread per neighborhood, the same
+// sequential double accumulation as the reference.
+__global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= p.G) return;
+    const int64_t L = tb.L[g];
+    if (L <= p.max_s) {
+        tb.ns[g] = static_cast<int32_t>(L);
+        return;
+    }
+    const double increment = L * 1.0 / p.max_s;
+    int32_t* out = tb.idx + g * p.idx_stride;
+    int32_t c = 0;
+    for (double i = 0; i < L; i += increment) {
+        if (c < p.idx_stride) out[c] = static_cast<int32_t>(static_cast<int64_t>(i));
+        ++c;
+    }
+    tb.ns[g] = c < p.idx_stride ? c : static_cast<int32_t>(p.idx_stride);
+}
+
+// ------------------------------------------------------------------ Gram
+// One workgroup per neighborhood.  Each K step stages 32 subsampled columns
+// (S floats each: the W x T clip for stage 1, the nf1*W reduced vector for
+// stage 2) in LDS as bf16 hi/lo pieces; waves own Gram tiles (sub-matrix q,
+// row block I, column block J >= I).  Stage 1 has W sub-matrices of T rows
+// (row t of channel w at offset t*W + w); stage 2 one of nf1*W rows.
+template <int kStage>
+__global__ void __launch_bounds__(256) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
+                                                       PcaTables tb) {
+    extern __shared__ __bf16 gl[];
+    const int g = blockIdx.x;
+    const int L = tb.L[g];
+    const int W = tb.W[g];
+    const int ns = tb.ns[g];
+    const int nrows = kStage == 1 ? p.T : p.nf1 * W;
+    const int nsub = kStage == 1 ? W : 1;
+    const int S = kStage == 1 ? W * p.T : p.nf1 * W;
+    const float* base = src + (kStage == 1 ? tb.clip_off[g] : tb.red_off[g]);
+    const int32_t* idx = tb.idx + g * p.idx_stride;
+    const bool sub = L > p.max_s;
+    const int nb = (nrows + 15) / 16;
+    const int tpq = nb * (nb + 1) / 2;       // tiles per sub-matrix
+    const int ntiles = tpq * nsub;
+    __bf16* hi = gl;
+    __bf16* lo = gl + kKC * S;
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
+    // tile t of this wave: t = wid + nw * k
+    f32x4 acc[kMaxTiles];
+#pragma unroll
+    for (int k = 0; k < kMaxTiles; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto rowoff = [&](int q, int r) { return kStage == 1 ? r * W + q : r; };
+    for (int c0 = 0; c0 < ns; c0 += kKC) {
+        // stage columns c0..c0+31 (zero past ns)
+        for (int e = threadIdx.x; e < kKC * S; e += blockDim.x) {
+            const int k = e / S;
+            const int o = e - k * S;
+            float v = 0.0f;
+            const int c = c0 + k;
+            if (c < ns) {
+                const int64_t col = sub ? idx[c] : c;
+                v = base[col * S + o];
+            }
+            const __bf16 h = static_cast<__bf16>(v);
+            hi[e] = h;
+            lo[e] = static_cast<__bf16>(v - static_cast<float>(h));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kMaxTiles; ++k) {
+            const int t = wid + nw * k;
+            if (t < ntiles) {
+                const int q = t / tpq;
+                int tt = t - q * tpq;
+                int I = 0, rl = nb;
+                while (tt >= rl) {
+                    tt -= rl;
+                    ++I;
+                    --rl;
+                }
+                const int J = I + tt;
+                bf16x8 ah, al, bh, bl;
+                const int ra = I * 16 + (lane & 15);
+                const int rb = J * 16 + (lane & 15);
+#pragma unroll
+                for (int e2 = 0; e2 < 8; ++e2) {
+                    const int kc = 8 * (lane >> 4) + e2;
+                    const bool va = ra < nrows, vb = rb < nrows;
+                    const int oa = kc * S + rowoff(q, va ? ra : 0);
+                    const int ob = kc * S + rowoff(q, vb ? rb : 0);
+                    ah[e2] = va ? hi[oa] : static_cast<__bf16>(0.0f);
+                    al[e2] = va ? lo[oa] : static_cast<__bf16>(0.0f);
+                    bh[e2] = vb ? hi[ob] : static_cast<__bf16>(0.0f);
+                    bl[e2] = vb ? lo[ob] : static_cast<__bf16>(0.0f);
+                }
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[k], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
+#pragma unroll
+    for (int k = 0; k < kMaxTiles; ++k) {
+        const int t = wid + nw * k;
+        if (t < ntiles) {
+            const int q = t / tpq;
+            int tt = t - q * tpq;
+            int I = 0, rl = nb;
+            while (tt >= rl) {
+                tt -= rl;
+                ++I;
+                --rl;
+            }
+            const int J = I + tt;
+            float* oq = out + static_cast<int64_t>(q) * nrows * nrows;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ii = I * 16 + 4 * (lane >> 4) + r;
+                const int jj = J * 16 + (lane & 15);
+                if (ii < nrows && jj < nrows && !(I == J && ii > jj)) {
+                    oq[ii * nrows + jj] = acc[k][r];
+                    oq[jj * nrows + ii] = acc[k][r];
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------- power method
+// One workgroup per Gram matrix (n x n, n <= 128 held in LDS as FP64).
+// Components are written as float [row][component].
+constexpr int kPowMaxN = 128;
+
+__device__ double pow_block_sum(double x, double* red) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s = s + red[w];
+        red[8] = s;
+    }
+    __syncthreads();
+    const double s = red[8];
+    __syncthreads();
+    return s;
+}
+
+__device__ void power_method(double* Gs, int n, int K, const double* sin_tab, float* comp,
+                             double* v, double* w, double* red) {
+    for (int k = 0; k < K; ++k) {
+        // v = normalize(sin(1..n))
+        double part = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            v[i] = sin_tab[i];
+            part += v[i] * v[i];
+        }
+        double nrm = sqrt(pow_block_sum(part, red));
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = nrm != 0.0 ? v[i] / nrm : v[i];
+        __syncthreads();
+        for (int it = 0; it < 11; ++it) {
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                const double* row = Gs + r * n;
+                double a = 0.0;
+                for (int c = 0; c < n; ++c) a += row[c] * v[c];
+                w[r] = a;
+            }
+            __syncthreads();
+            if (it < 10) {
+                part = 0.0;
+                for (int i = threadIdx.x; i < n; i += blockDim.x) part += w[i] * w[i];
+                const double sigma = sqrt(pow_block_sum(part, red));
+                for (int i = threadIdx.x; i < n; i += blockDim.x)
+                    v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
+                __syncthreads();
+            } else {
+                // deflation with u = G v
+                part = 0.0;
+                for (int i = threadIdx.x; i < n; i += blockDim.x) part += v[i] * w[i];
+                const double alpha = pow_block_sum(part, red);
+                for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
+                    const int i = q / n;
+                    const int j = q - i * n;
+                    Gs[q] = (Gs[q] - (v[i] * w[j] + w[i] * v[j])) + alpha * (v[i] * v[j]);
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) comp[i * K + k] = static_cast<float>(v[i]);
+        __syncthreads();
+    }
+}
+
+template <int kStage>
+__global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb) {
+    extern __shared__ double psh[];
+    int g, q;
+    if (kStage == 1) {
+        g = tb.p1_items[2 * blockIdx.x];
+        q = tb.p1_items[2 * blockIdx.x + 1];
+    } else {
+        g = blockIdx.x;
+        q = 0;
+    }
+    const int W = tb.W[g];
+    const int n = kStage == 1 ? p.T : p.nf1 * W;
+    const int K = kStage == 1 ? p.nf1 : p.nf2;
+    const float* Gsrc = kStage == 1 ? tb.g1 + tb.g1_off[g] + static_cast<int64_t>(q) * n * n
+                                    : tb.g2 + tb.g2_off[g];
+    float* comp = kStage == 1 ? tb.c1 + tb.c1_off[g] + static_cast<int64_t>(q) * n * K
+                              : tb.c2 + tb.c2_off[g];
+    // the FP64 copy of G lives in LDS when it fits, otherwise in the workspace
+    const bool in_lds = kStage == 1 || p.nf1 * p.maxW <= kPowMaxN;  // decided per launch
+    double* Gs = in_lds ? psh : tb.work2 + tb.w2_off[g];
+    double* v = in_lds ? Gs + n * n : psh;
+    double* w = v + n;
+    double* red = w + n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Gs[e] = static_cast<double>(Gsrc[e]);
+    __syncthreads();
+    power_method(Gs, n, K, tb.sin_tab, comp, v, w, red);
+}
+
+// ------------------------------------------------------------ projections
+// Stage 1 item: kProjE1 events of one neighborhood; thread (event, channel)
+// forms nf1 dot products over T (clip and components staged in LDS).
+__global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restrict__ clips,
+                                                           PcaPlan p, PcaTables tb,
+                                                           float* __restrict__ reduced) {
+    extern __shared__ float sp[];
+    const int g = tb.j1_items[2 * blockIdx.x];
+    const int e0 = tb.j1_items[2 * blockIdx.x + 1];
+    const int L = tb.L[g];
+    const int W = tb.W[g];
+    const int T = p.T;
+    const int nf = p.nf1;
+    const int ne = (L - e0) < kProjE1 ? (L - e0) : kProjE1;
+    float* Cs = sp;                    // [W][T][nf]
+    float* Xs = sp + W * T * nf;       // [ne][T][W]
+    const float* cg = tb.c1 + tb.c1_off[g];
+    for (int e = threadIdx.x; e < W * T * nf; e += blockDim.x) Cs[e] = cg[e];
+    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T;
+    for (int e = threadIdx.x; e < ne * W * T; e += blockDim.x) Xs[e] = xg[e];
+    __syncthreads();
+    for (int o = threadIdx.x; o < ne * W; o += blockDim.x) {
+        const int ev = o / W;
+        const int wc = o - ev * W;
+        float acc[16];
+#pragma unroll
+        for (int a = 0; a < 16; ++a) acc[a] = 0.0f;
+        const float* x = Xs + ev * W * T + wc;
+        const float* c = Cs + wc * T * nf;
+        for (int t = 0; t < T; ++t) {
+            const float xv = x[t * W];
+#pragma unroll
+            for (int a = 0; a < 16; ++a)
+                if (a < nf) acc[a] = __builtin_fmaf(c[t * nf + a], xv, acc[a]);
+        }
+        float* out = reduced + tb.red_off[g] + static_cast<int64_t>(e0 + ev) * W * nf + wc;
+#pragma unroll
+        for (int a = 0; a < 16; ++a)
+            if (a < nf) out[W * a] = acc[a];
+    }
+}
+
+// Stage 2 item: kProjE2 events; thread (event, feature) dots nf1*W inputs.
+__global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restrict__ reduced,
+                                                           PcaPlan p, PcaTables tb,
+                                                           float* __restrict__ features) {
+    extern __shared__ float sq[];
+    const int g = tb.j2_items[2 * blockIdx.x];
+    const int e0 = tb.j2_items[2 * blockIdx.x + 1];
+    const int L = tb.L[g];
+    const int W = tb.W[g];
+    const int R = p.nf1 * W;
+    const int nf = p.nf2;
+    const int ne = (L - e0) < kProjE2 ? (L - e0) : kProjE2;
+    float* Cs = sq;            // [R][nf]
+    float* Xs = sq + R * nf;   // [ne][R]
+    const float* cg = tb.c2 + tb.c2_off[g];
+    for (int e = threadIdx.x; e < R * nf; e += blockDim.x) Cs[e] = cg[e];
+    const float* xg = reduced + tb.red_off[g] + static_cast<int64_t>(e0) * R;
+    for (int e = threadIdx.x; e < ne * R; e += blockDim.x) Xs[e] = xg[e];
+    __syncthreads();
+    for (int o = threadIdx.x; o < ne * nf; o += blockDim.x) {
+        const int ev = o / nf;
+        const int a = o - ev * nf;
+        float acc = 0.0f;
+        const float* x = Xs + ev * R;
+        for (int r = 0; r < R; ++r) acc = __builtin_fmaf(Cs[r * nf + a], x[r], acc);
+        features[tb.feat_off[g] + static_cast<int64_t>(e0 + ev) * nf + a] = acc;
+    }
+}
+
+// ------------------------------------------------------------- host plan
+struct HostPlan {
+    std::vector<int64_t> clip_off, red_off, feat_off, g1_off, g2_off, c1_off, c2_off;
+    std::vector<int32_t> L, W, p1, j1, j2;
+    std::vector<int64_t> w2_off;
+    int64_t g1_total = 0, g2_total = 0, c1_total = 0, c2_total = 0, w2_total = 0;
+    int32_t maxW = 0;
+};
+
+HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
+                   int32_t nf1, int32_t nf2) {
+    HostPlan h;
+    int32_t maxW = 0;
+    for (int32_t g = 0; g < G; ++g) maxW = widths[g] > maxW ? widths[g] : maxW;
+    const bool w2_global = static_cast<int64_t>(nf1) * maxW > kPowMaxN;
+    int64_t co = 0, ro = 0, fo = 0;
+    for (int32_t g = 0; g < G; ++g) {
+        const int64_t L = num_events[g];
+        const int32_t W = widths[g];
+        h.L.push_back(static_cast<int32_t>(L));
+        h.W.push_back(W);
+        h.maxW = W > h.maxW ? W : h.maxW;
+        h.clip_off.push_back(co);
+        h.red_off.push_back(ro);
+        h.feat_off.push_back(fo);
+        co += static_cast<int64_t>(W) * T * L;
+        ro += static_cast<int64_t>(W) * nf1 * L;
+        fo += static_cast<int64_t>(nf2) * L;
+        h.g1_off.push_back(h.g1_total);
+        h.g1_total += static_cast<int64_t>(W) * T * T;
+        const int64_t R = static_cast<int64_t>(nf1) * W;
+        h.g2_off.push_back(h.g2_total);
+        h.g2_total += R * R;
+        h.w2_off.push_back(h.w2_total);
+        if (w2_global) h.w2_total += R * R;
+        h.c1_off.push_back(h.c1_total);
+        h.c1_total += static_cast<int64_t>(W) * T * nf1;
+        h.c2_off.push_back(h.c2_total);
+        h.c2_total += R * nf2;
+        for (int32_t w = 0; w < W; ++w) {
+            h.p1.push_back(g);
+            h.p1.push_back(w);
+        }
+        for (int64_t e = 0; e < L; e += kProjE1) {
+            h.j1.push_back(g);
+            h.j1.push_back(static_cast<int32_t>(e));
+        }
+        for (int64_t e = 0; e < L; e += kProjE2) {
+            h.j2.push_back(g);
+            h.j2.push_back(static_cast<int32_t>(e));
+        }
+    }
+    return h;
+}
+
+template <typename T>
+T* take_copy(Carver& c, const std::vector<T>& v) {
+    return c.take<T>(static_cast<int64_t>(v.size() > 0 ? v.size() : 1));
+}
+
+}  // namespace
+}  // namespace msamd
+
+using namespace msamd;
+
+namespace {
+
+struct PcaLayout {
+    PcaTables tb{};
+    int64_t bytes = 0;
+    // device destinations of the host tables
+    int64_t* d_clip_off = nullptr;
+    int64_t* d_red_off = nullptr;
+    int64_t* d_feat_off = nullptr;
+    int32_t* d_L = nullptr;
+    int32_t* d_W = nullptr;
+    int64_t* d_g1_off = nullptr;
+    int64_t* d_g2_off = nullptr;
+    int64_t* d_c1_off = nullptr;
+    int64_t* d_c2_off = nullptr;
+    int32_t* d_p1 = nullptr;
+    int32_t* d_j1 = nullptr;
+    int32_t* d_j2 = nullptr;
+    double* d_sin = nullptr;
+    int64_t* d_w2_off = nullptr;
+};
+
+PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride) {
+    PcaLayout l;
+    Carver c(ws);
+    l.d_clip_off = take_copy(c, h.clip_off);
+    l.d_red_off = take_copy(c, h.red_off);
+    l.d_feat_off = take_copy(c, h.feat_off);
+    l.d_L = take_copy(c, h.L);
+    l.d_W = take_copy(c, h.W);
+    l.d_g1_off = take_copy(c, h.g1_off);
+    l.d_g2_off = take_copy(c, h.g2_off);
+    l.d_c1_off = take_copy(c, h.c1_off);
+    l.d_c2_off = take_copy(c, h.c2_off);
+    l.d_p1 = take_copy(c, h.p1);
+    l.d_j1 = take_copy(c, h.j1);
+    l.d_j2 = take_copy(c, h.j2);
+    l.d_sin = c.take<double>(1024);
+    l.d_w2_off = take_copy(c, h.w2_off);
+    double* work2 = c.take<double>(h.w2_total > 0 ? h.w2_total : 1);
+    int32_t* ns = c.take<int32_t>(G);
+    int32_t* idx = c.take<int32_t>(static_cast<int64_t>(G) * idx_stride);
+    float* g1 = c.take<float>(h.g1_total > 0 ? h.g1_total : 1);
+    float* g2 = c.take<float>(h.g2_total > 0 ? h.g2_total : 1);
+    float* c1 = c.take<float>(h.c1_total > 0 ? h.c1_total : 1);
+    float* c2 = c.take<float>(h.c2_total > 0 ? h.c2_total : 1);
+    l.bytes = c.bytes();
+    l.tb = PcaTables{l.d_clip_off, l.d_red_off, l.d_feat_off, l.d_L, l.d_W, l.d_g1_off,
+                     l.d_g2_off, l.d_c1_off, l.d_c2_off, l.d_p1, l.d_j1, l.d_j2, ns, idx,
+                     g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off};
+    return l;
+}
+
+int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
+                   int32_t nf1, int32_t nf2, int64_t max_s) {
+    MSAMD_REQUIRE(G >= 1 && num_events && widths, "pca_features: bad neighborhood tables");
+    MSAMD_REQUIRE(T >= 1 && T <= kMaxT, "pca_features: clip_size %d outside 1..%d", T, kMaxT);
+    MSAMD_REQUIRE(nf1 >= 1 && nf1 <= 16 && nf2 >= 1 && nf2 <= 16,
+                  "pca_features: feature counts must be 1..16");
+    MSAMD_REQUIRE(nf1 <= T, "pca_features: num_features_per_channel > clip_size");
+    MSAMD_REQUIRE(max_s >= 1 && max_s < (1LL << 30), "pca_features: bad max_pca_samples");
+    for (int32_t g = 0; g < G; ++g) {
+        MSAMD_REQUIRE(widths[g] >= 1 && widths[g] <= kMaxW,
+                      "pca_features: neighborhood %d has %d channels (1..%d supported)", g,
+                      widths[g], kMaxW);
+        MSAMD_REQUIRE(num_events[g] >= 0 && num_events[g] < (1LL << 31),
+                      "pca_features: bad event count");
+        MSAMD_REQUIRE(nf2 <= nf1 * widths[g], "pca_features: num_features > stage-2 dimension");
+    }
+    MSAMD_REQUIRE(T <= kPowMaxN, "pca_features: clip_size too large");
+    return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events_host,
+                                            const int32_t* widths_host, int32_t T,
+                                            int32_t num_features_per_channel,
+                                            int32_t num_features, int64_t max_pca_samples) {
+    if (check_pca_args(G, num_events_host, widths_host, T, num_features_per_channel,
+                       num_features, max_pca_samples) != MSAMD_OK)
+        return -1;
+    const HostPlan h = host_plan(G, num_events_host, widths_host, T, num_features_per_channel,
+                                 num_features);
+    return pca_layout(nullptr, h, G, max_pca_samples + 2).bytes;
+}
+
+extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_host,
+                                  const int32_t* widths_host, int32_t T,
+                                  int32_t num_features_per_channel, int32_t num_features,
+                                  int64_t max_pca_samples, float* reduced, float* features,
+                                  void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    const int rc = check_pca_args(G, num_events_host, widths_host, T, num_features_per_channel,
+                                  num_features, max_pca_samples);
+    if (rc != MSAMD_OK) return rc;
+    const int32_t nf1 = num_features_per_channel, nf2 = num_features;
+    const HostPlan h = host_plan(G, num_events_host, widths_host, T, nf1, nf2);
+    const int64_t idx_stride = max_pca_samples + 2;
+    PcaLayout l = pca_layout(workspace, h, G, idx_stride);
+    if (!workspace || workspace_bytes < l.bytes) {
+        set_error("pca_features: workspace too small (%lld < %lld)", (long long)workspace_bytes,
+                  (long long)l.bytes);
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    std::vector<double> sins(1024);
+    for (int i = 0; i < 1024; ++i) sins[i] = std::sin(static_cast<double>(i + 1));
+    auto up = [&](void* dst, const void* src, size_t n) {
+        return n ? hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    MSAMD_HIP(up(l.d_clip_off, h.clip_off.data(), h.clip_off.size() * 8));
+    MSAMD_HIP(up(l.d_red_off, h.red_off.data(), h.red_off.size() * 8));
+    MSAMD_HIP(up(l.d_feat_off, h.feat_off.data(), h.feat_off.size() * 8));
+    MSAMD_HIP(up(l.d_L, h.L.data(), h.L.size() * 4));
+    MSAMD_HIP(up(l.d_W, h.W.data(), h.W.size() * 4));
+    MSAMD_HIP(up(l.d_g1_off, h.g1_off.data(), h.g1_off.size() * 8));
+    MSAMD_HIP(up(l.d_g2_off, h.g2_off.data(), h.g2_off.size() * 8));
+    MSAMD_HIP(up(l.d_c1_off, h.c1_off.data(), h.c1_off.size() * 8));
+    MSAMD_HIP(up(l.d_c2_off, h.c2_off.data(), h.c2_off.size() * 8));
+    MSAMD_HIP(up(l.d_p1, h.p1.data(), h.p1.size() * 4));
+    MSAMD_HIP(up(l.d_j1, h.j1.data(), h.j1.size() * 4));
+    MSAMD_HIP(up(l.d_j2, h.j2.data(), h.j2.size() * 4));
+    MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * 8));
+    MSAMD_HIP(up(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8));
+    MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope after the call
+    PcaPlan p{G, T, nf1, nf2, max_pca_samples, h.maxW, idx_stride};
+    pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
+    MSAMD_LAUNCH_CHECK();
+    {  // stage 1 Gram
+        const size_t lds = sizeof(__bf16) * 2 * kKC * h.maxW * T;
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<1>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        pca_gram_kernel<1><<<G, 256, lds, st>>>(clips, p, l.tb);
+        MSAMD_LAUNCH_CHECK();
+    }
+    {  // stage 1 power method
+        const size_t lds = sizeof(double) * (T * T + 2 * T + 16);
+        pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, lds, st>>>(p, l.tb);
+        MSAMD_LAUNCH_CHECK();
+    }
+    if (!h.j1.empty()) {  // stage 1 projections
+        const size_t lds = sizeof(float) * (h.maxW * T * nf1 + kProjE1 * h.maxW * T);
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_project1_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        pca_project1_kernel<<<static_cast<unsigned>(h.j1.size() / 2), 256, lds, st>>>(clips, p, l.tb,
+                                                                                      reduced);
+        MSAMD_LAUNCH_CHECK();
+    }
+    {  // stage 2 Gram
+        const size_t lds = sizeof(__bf16) * 2 * kKC * nf1 * h.maxW;
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<2>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        pca_gram_kernel<2><<<G, 256, lds, st>>>(reduced, p, l.tb);
+        MSAMD_LAUNCH_CHECK();
+    }
+    {  // stage 2 power method
+        const int n = nf1 * h.maxW;
+        const size_t lds = sizeof(double) * ((n <= kPowMaxN ? n * n : 0) + 2 * n + 16);
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_power_kernel<2>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        pca_power_kernel<2><<<G, 256, lds, st>>>(p, l.tb);
+        MSAMD_LAUNCH_CHECK();
+    }
+    if (!h.j2.empty()) {  // stage 2 projections
+        const size_t lds = sizeof(float) * (nf1 * h.maxW * nf2 + kProjE2 * nf1 * h.maxW);
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_project2_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, lds, st>>>(reduced, p, l.tb,
+                                                                                      features);
+        MSAMD_LAUNCH_CHECK();
+    }
+    return MSAMD_OK;
+}

@@ -181,6 +181,46 @@ def extract_clips_batched(X, times, event_offsets, nbhd_channels, nbhd_offsets, 
 
 
 # --------------------------------------------------------------------------
+# PCA features (a12-a15)
+# --------------------------------------------------------------------------
+def pca_features(clips_flat, clip_offsets, event_offsets, nbhd_offsets, clip_size,
+                 num_features_per_channel=10, num_features=10, max_pca_samples=10000):
+    """Two-stage PCA features of every neighborhood (neighborhoodsorter.cpp:236-276,
+    sort_clips.cpp:58-70).
+
+    clips_flat/clip_offsets as returned by extract_clips_batched; event_offsets
+    and nbhd_offsets are the (G+1) CSR offsets of events and channels.  Returns
+    (reduced_flat, reduced_offsets, features_flat, feature_offsets): per
+    neighborhood g, reduced (L, nf1, W) at reduced_flat[reduced_offsets[g]:] and
+    features (L, nf2) at features_flat[feature_offsets[g]:].
+    """
+    import numpy as np
+    dev = clips_flat.device
+    eo = torch.as_tensor(event_offsets).cpu().numpy().astype(np.int64)
+    no = torch.as_tensor(nbhd_offsets).cpu().numpy().astype(np.int64)
+    L = np.ascontiguousarray(eo[1:] - eo[:-1], dtype=np.int64)
+    W = np.ascontiguousarray(no[1:] - no[:-1], dtype=np.int32)
+    G = len(L)
+    T, nf1, nf2 = int(clip_size), int(num_features_per_channel), int(num_features)
+    lib = _capi.lib()
+    pL = L.ctypes.data_as(ctypes.c_void_p)
+    pW = W.ctypes.data_as(ctypes.c_void_p)
+    nb = lib.msamd_pca_workspace_size(G, pL, pW, T, nf1, nf2, int(max_pca_samples))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_pca_workspace_size", 1, "unsupported neighborhood shapes")
+    ws = _workspace(nb, dev)
+    red_sizes = W.astype(np.int64) * nf1 * L
+    red_off = np.concatenate([[0], np.cumsum(red_sizes)])
+    feat_off = np.concatenate([[0], np.cumsum(nf2 * L)])
+    reduced = torch.empty(max(int(red_off[-1]), 1), dtype=torch.float32, device=dev)
+    features = torch.empty(max(int(feat_off[-1]), 1), dtype=torch.float32, device=dev)
+    _capi.call("msamd_pca_features", _ptr(clips_flat), G, pL, pW, T, nf1, nf2,
+               int(max_pca_samples), _ptr(reduced), _ptr(features), _ptr(ws), ws.numel(),
+               _stream(dev))
+    return reduced, red_off, features, feat_off
+
+
+# --------------------------------------------------------------------------
 # whitening (a4-a6)
 # --------------------------------------------------------------------------
 def whiten_covariance(X):

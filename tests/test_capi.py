@@ -67,3 +67,18 @@ def test_bandpass_host_entry_points():
     assert L.msamd_bandpass_supported(120000, 60000) == 1
     assert L.msamd_bandpass_supported(1278101, 60000) == 0
     assert L.msamd_bandpass_workspace_size(384, 9_000_000, 120000, 60000) > 0
+
+
+def test_pca_workspace_query():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libmsamd.so not built")
+    import numpy as np
+    L = _capi.lib()
+    ev = np.array([300, 0, 12345, 50, 1500], np.int64)
+    w = np.array([5, 3, 12, 1, 16], np.int32)
+    args = (ev.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p))
+    assert L.msamd_pca_workspace_size(5, *args, 50, 10, 10, 10000) > 0
+    w_bad = np.array([5, 3, 17, 1, 16], np.int32)
+    assert L.msamd_pca_workspace_size(5, args[0], w_bad.ctypes.data_as(ctypes.c_void_p),
+                                      50, 10, 10, 10000) < 0
+    assert L.msamd_pca_workspace_size(5, *args, 65, 10, 10, 10000) < 0
