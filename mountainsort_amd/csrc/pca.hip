@@ -19,6 +19,7 @@
 // Phases run as one launch each over every neighborhood (items sized for
 // load balance: Gram and power per neighborhood / channel, projections per
 // chunk of events).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -35,9 +36,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kKC = 32;        // subsampled columns per MFMA K step
 constexpr int kMaxW = 16;      // channels per neighborhood supported
 constexpr int kMaxT = 64;      // clip length supported
-constexpr int kProjE1 = 32;    // events per stage-1 projection item
+constexpr int kProjE1 = 64;    // events per stage-1 projection item
 constexpr int kProjE2 = 64;    // events per stage-2 projection item
-constexpr int kMaxTiles = 40;  // Gram tiles per wavefront
+constexpr int kMaxTiles = 20;  // Gram tiles per wavefront (8 waves per workgroup)
+constexpr int kGramThreads = 512;
 
 struct PcaPlan {
     int32_t G, T, nf1, nf2;
@@ -68,6 +70,10 @@ struct PcaTables {
     const double* sin_tab;     // sin(i+1), i < 1024
     double* work2;             // FP64 stage-2 Gram copies when n > kPowMaxN
     const int64_t* w2_off;     // [G] doubles
+    const int32_t* gram_items; // [ngi][2] (g, chunk)
+    const int32_t* gram_first; // [G+1] first item of each neighborhood
+    const int64_t* gp1_off;    // [ngi] floats: stage-1 partial of each item
+    const int64_t* gp2_off;    // [ngi] floats: stage-2 partial of each item
 };
 
 // ----------------------------------------------------- subsample indices
@@ -92,102 +98,137 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 }
 
 // ------------------------------------------------------------------ Gram
-// One workgroup per neighborhood.  Each K step stages 32 subsampled columns
-// (S floats each: the W x T clip for stage 1, the nf1*W reduced vector for
-// stage 2) in LDS as bf16 hi/lo pieces; waves own Gram tiles (sub-matrix q,
-// row block I, column block J >= I).  Stage 1 has W sub-matrices of T rows
-// (row t of channel w at offset t*W + w); stage 2 one of nf1*W rows.
+// Work item = (neighborhood, chunk of up to kGramChunk subsampled columns);
+// the chunk partial Grams are summed in FP64 by pca_gram_reduce_kernel in
+// chunk order.  Each K step stages 32 columns (S floats each: the W x T clip
+// for stage 1, the nf1*W reduced vector for stage 2) in LDS as bf16 hi/lo
+// pieces laid out [sub-matrix][row][k]: a thread loads element o of 8
+// consecutive columns (coalesced across threads) and writes them as one
+// 16-byte store, and an MFMA operand fragment (8 consecutive k of one row)
+// is one 16-byte load.  Rows have a 40-element stride and sub-matrices an odd
+// row count so the stores spread over the LDS banks.  Stage 1 has W
+// sub-matrices of T rows (row t of channel w is element t*W + w of a
+// column); stage 2 one sub-matrix of nf1*W rows.  Rows past the end read as
+// zero.  Waves own Gram tiles (q, I, J >= I).
+constexpr int kGramChunk = 2560;
+constexpr int kRowStride = 40;
+
+__device__ __forceinline__ int gram_rows_alloc(int nrows) { return nrows | 1; }
+
 template <int kStage>
-__global__ void __launch_bounds__(256) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
-                                                       PcaTables tb) {
+__global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
+                                                       PcaTables tb, float* __restrict__ part) {
     extern __shared__ __bf16 gl[];
-    const int g = blockIdx.x;
+    __shared__ int64_t colbase[kKC];
+    const int g = tb.gram_items[2 * blockIdx.x];
+    const int chunk = tb.gram_items[2 * blockIdx.x + 1];
     const int L = tb.L[g];
     const int W = tb.W[g];
     const int ns = tb.ns[g];
     const int nrows = kStage == 1 ? p.T : p.nf1 * W;
     const int nsub = kStage == 1 ? W : 1;
     const int S = kStage == 1 ? W * p.T : p.nf1 * W;
+    const int nb = (nrows + 15) / 16;
+    const int ra = gram_rows_alloc(nrows);
     const float* base = src + (kStage == 1 ? tb.clip_off[g] : tb.red_off[g]);
     const int32_t* idx = tb.idx + g * p.idx_stride;
     const bool sub = L > p.max_s;
-    const int nb = (nrows + 15) / 16;
-    const int tpq = nb * (nb + 1) / 2;       // tiles per sub-matrix
+    const int tpq = nb * (nb + 1) / 2;  // tiles per sub-matrix
     const int ntiles = tpq * nsub;
+    const int plane = nsub * ra * kRowStride;  // bf16 elements per piece
     __bf16* hi = gl;
-    __bf16* lo = gl + kKC * S;
+    __bf16* lo = gl + plane;
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
-    // tile t of this wave: t = wid + nw * k
-    f32x4 acc[kMaxTiles];
-#pragma unroll
-    for (int k = 0; k < kMaxTiles; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
-    auto rowoff = [&](int q, int r) { return kStage == 1 ? r * W + q : r; };
-    for (int c0 = 0; c0 < ns; c0 += kKC) {
-        // stage columns c0..c0+31 (zero past ns)
-        for (int e = threadIdx.x; e < kKC * S; e += blockDim.x) {
-            const int k = e / S;
-            const int o = e - k * S;
-            float v = 0.0f;
-            const int c = c0 + k;
-            if (c < ns) {
-                const int64_t col = sub ? idx[c] : c;
-                v = base[col * S + o];
-            }
-            const __bf16 h = static_cast<__bf16>(v);
-            hi[e] = h;
-            lo[e] = static_cast<__bf16>(v - static_cast<float>(h));
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kMaxTiles; ++k) {
-            const int t = wid + nw * k;
-            if (t < ntiles) {
-                const int q = t / tpq;
-                int tt = t - q * tpq;
-                int I = 0, rl = nb;
-                while (tt >= rl) {
-                    tt -= rl;
-                    ++I;
-                    --rl;
-                }
-                const int J = I + tt;
-                bf16x8 ah, al, bh, bl;
-                const int ra = I * 16 + (lane & 15);
-                const int rb = J * 16 + (lane & 15);
-#pragma unroll
-                for (int e2 = 0; e2 < 8; ++e2) {
-                    const int kc = 8 * (lane >> 4) + e2;
-                    const bool va = ra < nrows, vb = rb < nrows;
-                    const int oa = kc * S + rowoff(q, va ? ra : 0);
-                    const int ob = kc * S + rowoff(q, vb ? rb : 0);
-                    ah[e2] = va ? hi[oa] : static_cast<__bf16>(0.0f);
-                    al[e2] = va ? lo[oa] : static_cast<__bf16>(0.0f);
-                    bh[e2] = vb ? hi[ob] : static_cast<__bf16>(0.0f);
-                    bl[e2] = vb ? lo[ob] : static_cast<__bf16>(0.0f);
-                }
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[k], 0, 0, 0);
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[k], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-    }
-    float* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
+    // packed (q << 16) | (I << 8) | J
+    int tile[kMaxTiles];
 #pragma unroll
     for (int k = 0; k < kMaxTiles; ++k) {
         const int t = wid + nw * k;
+        int q = 0, I = 0, J = 0;
         if (t < ntiles) {
-            const int q = t / tpq;
+            q = t / tpq;
             int tt = t - q * tpq;
-            int I = 0, rl = nb;
+            int rl = nb;
             while (tt >= rl) {
                 tt -= rl;
                 ++I;
                 --rl;
             }
-            const int J = I + tt;
+            J = I + tt;
+        }
+        tile[k] = (q << 16) | (I << 8) | J;
+    }
+    f32x4 acc[kMaxTiles];
+#pragma unroll
+    for (int k = 0; k < kMaxTiles; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8* hv = reinterpret_cast<const bf16x8*>(hi);
+    const bf16x8* lv = reinterpret_cast<const bf16x8*>(lo);
+    const bf16x8 zero8 = {};
+    const int cbeg = chunk * kGramChunk;
+    int cend = cbeg + kGramChunk;
+    if (cend > ns) cend = ns;
+    for (int c0 = cbeg; c0 < cend; c0 += kKC) {
+        __syncthreads();  // previous step's fragments consumed
+        if (threadIdx.x < kKC) {
+            const int c = c0 + threadIdx.x;
+            colbase[threadIdx.x] = c < cend ? static_cast<int64_t>(sub ? idx[c] : c) * S : -1;
+        }
+        __syncthreads();
+        // item (kg, o): element o of columns 8kg..8kg+7
+        for (int it = threadIdx.x; it < 4 * S; it += blockDim.x) {
+            const int kg = it / S;
+            const int o = it - kg * S;
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t cb = colbase[8 * kg + e];
+                v[e] = cb >= 0 ? base[cb + o] : 0.0f;
+            }
+            bf16x8 h8, l8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const __bf16 h = static_cast<__bf16>(v[e]);
+                h8[e] = h;
+                l8[e] = static_cast<__bf16>(v[e] - static_cast<float>(h));
+            }
+            int q, r;
+            if (kStage == 1) {
+                q = o % W;
+                r = o / W;
+            } else {
+                q = 0;
+                r = o;
+            }
+            const int li = ((q * ra + r) * kRowStride + 8 * kg) / 8;
+            reinterpret_cast<bf16x8*>(hi)[li] = h8;
+            reinterpret_cast<bf16x8*>(lo)[li] = l8;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kMaxTiles; ++k) {
+            if (wid + nw * k < ntiles) {
+                // fragment of row block X: lane (i + 16b) reads row 16X + i, k = 8b..8b+7
+                const int q = tile[k] >> 16, I = (tile[k] >> 8) & 255, J = tile[k] & 255;
+                const int rA = I * 16 + (lane & 15), rB = J * 16 + (lane & 15);
+                const int fa = ((q * ra + rA) * kRowStride + 8 * (lane >> 4)) / 8;
+                const int fb = ((q * ra + rB) * kRowStride + 8 * (lane >> 4)) / 8;
+                const bool va = rA < nrows, vb = rB < nrows;
+                const bf16x8 ah = va ? hv[fa] : zero8, al = va ? lv[fa] : zero8;
+                const bf16x8 bh = vb ? hv[fb] : zero8, bl = vb ? lv[fb] : zero8;
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[k], 0, 0, 0);
+            }
+        }
+    }
+    // partial Gram of this chunk: [sub-matrix][row][row], upper triangle mirrored
+    float* out = part + (kStage == 1 ? tb.gp1_off : tb.gp2_off)[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kMaxTiles; ++k) {
+        if (wid + nw * k < ntiles) {
+            const int q = tile[k] >> 16, I = (tile[k] >> 8) & 255, J = tile[k] & 255;
             float* oq = out + static_cast<int64_t>(q) * nrows * nrows;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -200,6 +241,26 @@ __global__ void __launch_bounds__(256) pca_gram_kernel(const float* __restrict__
             }
         }
     }
+}
+
+// Sum of a neighborhood's chunk partials (FP64, chunk order) into its Gram.
+template <int kStage>
+__global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTables tb,
+                                                              const float* __restrict__ part) {
+    const int g = blockIdx.y;
+    const int W = tb.W[g];
+    const int nrows = kStage == 1 ? p.T : p.nf1 * W;
+    const int nsub = kStage == 1 ? W : 1;
+    const int64_t n = static_cast<int64_t>(nsub) * nrows * nrows;
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i0 = tb.gram_first[g];
+    const int nch = tb.gram_first[g + 1] - i0;
+    const int64_t* offs = kStage == 1 ? tb.gp1_off : tb.gp2_off;
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s = s + static_cast<double>(part[offs[i0 + c] + e]);
+    float* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
+    out[e] = static_cast<float>(s);
 }
 
 // ----------------------------------------------------------- power method
@@ -299,8 +360,13 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
 }
 
 // ------------------------------------------------------------ projections
-// Stage 1 item: kProjE1 events of one neighborhood; thread (event, channel)
-// forms nf1 dot products over T (clip and components staged in LDS).
+// Stage 1 item: up to kProjE1 events of one neighborhood.  The components
+// (W x T x nf1) sit in LDS with rows padded to 16 floats; thread (event,
+// channel) reads its clip samples straight from global memory (consecutive
+// threads -> consecutive channels -> contiguous addresses) and forms nf1
+// dot products over T.
+constexpr int kCPad = 16;
+
 __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restrict__ clips,
                                                            PcaPlan p, PcaTables tb,
                                                            float* __restrict__ reduced) {
@@ -312,30 +378,50 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
     const int T = p.T;
     const int nf = p.nf1;
     const int ne = (L - e0) < kProjE1 ? (L - e0) : kProjE1;
-    float* Cs = sp;                    // [W][T][nf]
-    float* Xs = sp + W * T * nf;       // [ne][T][W]
+    float* Cs = sp;  // [W][T][kCPad]
     const float* cg = tb.c1 + tb.c1_off[g];
-    for (int e = threadIdx.x; e < W * T * nf; e += blockDim.x) Cs[e] = cg[e];
-    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T;
-    for (int e = threadIdx.x; e < ne * W * T; e += blockDim.x) Xs[e] = xg[e];
+    for (int e = threadIdx.x; e < W * T * kCPad; e += blockDim.x) {
+        const int a = e % kCPad;
+        const int wt = e / kCPad;
+        Cs[e] = a < nf ? cg[wt * nf + a] : 0.0f;
+    }
     __syncthreads();
+    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T;
     for (int o = threadIdx.x; o < ne * W; o += blockDim.x) {
         const int ev = o / W;
         const int wc = o - ev * W;
-        float acc[16];
+        float acc[kCPad];
 #pragma unroll
-        for (int a = 0; a < 16; ++a) acc[a] = 0.0f;
-        const float* x = Xs + ev * W * T + wc;
-        const float* c = Cs + wc * T * nf;
-        for (int t = 0; t < T; ++t) {
+        for (int a = 0; a < kCPad; ++a) acc[a] = 0.0f;
+        const float* x = xg + ev * W * T + wc;
+        const float* c = Cs + wc * T * kCPad;
+        constexpr int U = 10;
+        int t = 0;
+        for (; t + U <= T; t += U) {
+            float xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[(t + u) * W];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float4* c4 = reinterpret_cast<const float4*>(c + (t + u) * kCPad);
+#pragma unroll
+                for (int q = 0; q < kCPad / 4; ++q) {
+                    const float4 cc = c4[q];
+                    acc[4 * q + 0] = __builtin_fmaf(cc.x, xv[u], acc[4 * q + 0]);
+                    acc[4 * q + 1] = __builtin_fmaf(cc.y, xv[u], acc[4 * q + 1]);
+                    acc[4 * q + 2] = __builtin_fmaf(cc.z, xv[u], acc[4 * q + 2]);
+                    acc[4 * q + 3] = __builtin_fmaf(cc.w, xv[u], acc[4 * q + 3]);
+                }
+            }
+        }
+        for (; t < T; ++t) {
             const float xv = x[t * W];
 #pragma unroll
-            for (int a = 0; a < 16; ++a)
-                if (a < nf) acc[a] = __builtin_fmaf(c[t * nf + a], xv, acc[a]);
+            for (int a = 0; a < kCPad; ++a) acc[a] = __builtin_fmaf(c[t * kCPad + a], xv, acc[a]);
         }
         float* out = reduced + tb.red_off[g] + static_cast<int64_t>(e0 + ev) * W * nf + wc;
 #pragma unroll
-        for (int a = 0; a < 16; ++a)
+        for (int a = 0; a < kCPad; ++a)
             if (a < nf) out[W * a] = acc[a];
     }
 }
@@ -373,13 +459,15 @@ __global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restri
 struct HostPlan {
     std::vector<int64_t> clip_off, red_off, feat_off, g1_off, g2_off, c1_off, c2_off;
     std::vector<int32_t> L, W, p1, j1, j2;
-    std::vector<int64_t> w2_off;
+    std::vector<int64_t> w2_off, gp1_off, gp2_off;
+    std::vector<int32_t> gram_items, gram_first;
     int64_t g1_total = 0, g2_total = 0, c1_total = 0, c2_total = 0, w2_total = 0;
+    int64_t gp1_total = 0, gp2_total = 0;
     int32_t maxW = 0;
 };
 
 HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
-                   int32_t nf1, int32_t nf2) {
+                   int32_t nf1, int32_t nf2, int64_t max_s) {
     HostPlan h;
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = widths[g] > maxW ? widths[g] : maxW;
@@ -404,6 +492,18 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
         h.g2_total += R * R;
         h.w2_off.push_back(h.w2_total);
         if (w2_global) h.w2_total += R * R;
+        // Gram work items: the subsampled column count is min(L, max_s + 1)
+        const int64_t ns_bound = L < max_s ? L : max_s + 1;
+        const int64_t nch = ns_bound > 0 ? ceil_div(ns_bound, kGramChunk) : 1;
+        h.gram_first.push_back(static_cast<int32_t>(h.gram_items.size() / 2));
+        for (int64_t c = 0; c < nch; ++c) {
+            h.gram_items.push_back(g);
+            h.gram_items.push_back(static_cast<int32_t>(c));
+            h.gp1_off.push_back(h.gp1_total);
+            h.gp1_total += static_cast<int64_t>(W) * T * T;
+            h.gp2_off.push_back(h.gp2_total);
+            h.gp2_total += R * R;
+        }
         h.c1_off.push_back(h.c1_total);
         h.c1_total += static_cast<int64_t>(W) * T * nf1;
         h.c2_off.push_back(h.c2_total);
@@ -421,6 +521,7 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
             h.j2.push_back(static_cast<int32_t>(e));
         }
     }
+    h.gram_first.push_back(static_cast<int32_t>(h.gram_items.size() / 2));
     return h;
 }
 
@@ -454,6 +555,11 @@ struct PcaLayout {
     int32_t* d_j2 = nullptr;
     double* d_sin = nullptr;
     int64_t* d_w2_off = nullptr;
+    int32_t* d_gram_items = nullptr;
+    int32_t* d_gram_first = nullptr;
+    int64_t* d_gp1_off = nullptr;
+    int64_t* d_gp2_off = nullptr;
+    float* part = nullptr;
 };
 
 PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride) {
@@ -474,6 +580,11 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     l.d_sin = c.take<double>(1024);
     l.d_w2_off = take_copy(c, h.w2_off);
     double* work2 = c.take<double>(h.w2_total > 0 ? h.w2_total : 1);
+    l.d_gram_items = take_copy(c, h.gram_items);
+    l.d_gram_first = take_copy(c, h.gram_first);
+    l.d_gp1_off = take_copy(c, h.gp1_off);
+    l.d_gp2_off = take_copy(c, h.gp2_off);
+    l.part = c.take<float>(std::max<int64_t>(std::max(h.gp1_total, h.gp2_total), 1));
     int32_t* ns = c.take<int32_t>(G);
     int32_t* idx = c.take<int32_t>(static_cast<int64_t>(G) * idx_stride);
     float* g1 = c.take<float>(h.g1_total > 0 ? h.g1_total : 1);
@@ -483,7 +594,8 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     l.bytes = c.bytes();
     l.tb = PcaTables{l.d_clip_off, l.d_red_off, l.d_feat_off, l.d_L, l.d_W, l.d_g1_off,
                      l.d_g2_off, l.d_c1_off, l.d_c2_off, l.d_p1, l.d_j1, l.d_j2, ns, idx,
-                     g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off};
+                     g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off, l.d_gram_items,
+                     l.d_gram_first, l.d_gp1_off, l.d_gp2_off};
     return l;
 }
 
@@ -517,7 +629,7 @@ extern "C" int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events
                        num_features, max_pca_samples) != MSAMD_OK)
         return -1;
     const HostPlan h = host_plan(G, num_events_host, widths_host, T, num_features_per_channel,
-                                 num_features);
+                                 num_features, max_pca_samples);
     return pca_layout(nullptr, h, G, max_pca_samples + 2).bytes;
 }
 
@@ -531,7 +643,7 @@ extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* 
                                   num_features, max_pca_samples);
     if (rc != MSAMD_OK) return rc;
     const int32_t nf1 = num_features_per_channel, nf2 = num_features;
-    const HostPlan h = host_plan(G, num_events_host, widths_host, T, nf1, nf2);
+    const HostPlan h = host_plan(G, num_events_host, widths_host, T, nf1, nf2, max_pca_samples);
     const int64_t idx_stride = max_pca_samples + 2;
     PcaLayout l = pca_layout(workspace, h, G, idx_stride);
     if (!workspace || workspace_bytes < l.bytes) {
@@ -559,15 +671,24 @@ extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* 
     MSAMD_HIP(up(l.d_j2, h.j2.data(), h.j2.size() * 4));
     MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * 8));
     MSAMD_HIP(up(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8));
+    MSAMD_HIP(up(l.d_gram_items, h.gram_items.data(), h.gram_items.size() * 4));
+    MSAMD_HIP(up(l.d_gram_first, h.gram_first.data(), h.gram_first.size() * 4));
+    MSAMD_HIP(up(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8));
+    MSAMD_HIP(up(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8));
     MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope after the call
     PcaPlan p{G, T, nf1, nf2, max_pca_samples, h.maxW, idx_stride};
     pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
-    {  // stage 1 Gram
-        const size_t lds = sizeof(__bf16) * 2 * kKC * h.maxW * T;
+    const unsigned ngi = static_cast<unsigned>(h.gram_items.size() / 2);
+    {  // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
+        const size_t lds = sizeof(__bf16) * 2 * h.maxW * (T | 1) * kRowStride;
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<1>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_gram_kernel<1><<<G, 256, lds, st>>>(clips, p, l.tb);
+        pca_gram_kernel<1><<<ngi, kGramThreads, lds, st>>>(clips, p, l.tb, l.part);
+        MSAMD_LAUNCH_CHECK();
+        const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(h.maxW) * T * T, 256)),
+                      static_cast<unsigned>(G));
+        pca_gram_reduce_kernel<1><<<rg, 256, 0, st>>>(p, l.tb, l.part);
         MSAMD_LAUNCH_CHECK();
     }
     {  // stage 1 power method
@@ -576,7 +697,7 @@ extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* 
         MSAMD_LAUNCH_CHECK();
     }
     if (!h.j1.empty()) {  // stage 1 projections
-        const size_t lds = sizeof(float) * (h.maxW * T * nf1 + kProjE1 * h.maxW * T);
+        const size_t lds = sizeof(float) * h.maxW * T * kCPad;
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_project1_kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         pca_project1_kernel<<<static_cast<unsigned>(h.j1.size() / 2), 256, lds, st>>>(clips, p, l.tb,
@@ -584,10 +705,15 @@ extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* 
         MSAMD_LAUNCH_CHECK();
     }
     {  // stage 2 Gram
-        const size_t lds = sizeof(__bf16) * 2 * kKC * nf1 * h.maxW;
+        const int R = nf1 * h.maxW;
+        const size_t lds = sizeof(__bf16) * 2 * (R | 1) * kRowStride;
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<2>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_gram_kernel<2><<<G, 256, lds, st>>>(reduced, p, l.tb);
+        pca_gram_kernel<2><<<ngi, kGramThreads, lds, st>>>(reduced, p, l.tb, l.part);
+        MSAMD_LAUNCH_CHECK();
+        const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(R) * R, 256)),
+                      static_cast<unsigned>(G));
+        pca_gram_reduce_kernel<2><<<rg, 256, 0, st>>>(p, l.tb, l.part);
         MSAMD_LAUNCH_CHECK();
     }
     {  // stage 2 power method
