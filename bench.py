@@ -106,6 +106,89 @@ class Pipeline:
         return feats
 
 
+def cpu_baseline(M, N_full, events_full, widths, log_fn=log):
+    """Time the oracle (the CPU restatement of the reference, single thread)
+    stage by stage on a bounded sample of the same workload and scale each
+    stage to the full recording by its own cost law:
+      bandpass  64 of the channels x 240000 samples        (linear in M*N)
+      mask      384 x 240000                               (linear in M*N)
+      covariance, apply  384 x 4000                        (linear in N at fixed M)
+      whitening matrix   the 96 x 96 leading block         (x (M/96)^4, naive power method)
+      detect    384 x 240000                               (linear in M*N)
+      clips     the sample's events                        (linear in events)
+      PCA       1 neighborhood with the full run's mean event count and width
+                                                           (x 384)
+    Later stages run on the 64 filtered channels tiled across the 384.
+    Returns the cpu_baseline JSON object."""
+    import numpy as np
+    from mountainsort_amd import synth
+    from oracle import core as oc
+    t_all = time.perf_counter()
+    Ns = 240000
+    cfg = synth.config2(N=Ns, seed=2)
+    raw = synth.generate(cfg, device="cpu").numpy()
+    secs = {}
+    t = time.perf_counter()
+    filt64 = oc.bandpass_filter(raw[:, :64], 30000, 300, 6000, 1000, num_threads=8)
+    secs["bandpass"] = (time.perf_counter() - t) * (M * N_full) / (64 * Ns)
+    # later stages run on the 64 filtered channels tiled across the probe
+    filt = np.ascontiguousarray(np.tile(filt64.astype(np.float32), (1, M // 64 + 1))[:, :M])
+    t = time.perf_counter()
+    masked, _, _ = oc.mask_out_artifacts(filt, 6.0, 2000)
+    secs["mask"] = (time.perf_counter() - t) * N_full / Ns
+    Nc = 4000
+    t = time.perf_counter()
+    C = oc.whiten_covariance(masked[:Nc])
+    secs["covariance"] = (time.perf_counter() - t) * N_full / Nc
+    t = time.perf_counter()
+    oc.whitening_matrix(C[:96, :96])
+    secs["whitening_matrix"] = (time.perf_counter() - t) * (M / 96.0) ** 4
+    w, V = np.linalg.eigh(oc.covariance_fp64(masked))
+    W = (V / np.sqrt(np.maximum(w, 1e-12))) @ V.T
+    t = time.perf_counter()
+    oc.whiten_apply(masked[:Nc], W)
+    secs["apply"] = (time.perf_counter() - t) * N_full / Nc
+    white = oc.quantize_1e4((masked.astype(np.float64) @ W.T).astype(np.float32))
+    t = time.perf_counter()
+    counts, times = oc.ms3alg_detect(white, 3.0, 10, -1, num_threads=8)
+    secs["detect"] = (time.perf_counter() - t) * N_full / Ns
+    nb = neighborhoods(M, cfg.columns, cfg.pitch, 40.0)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    t = time.perf_counter()
+    n_ev = 0
+    for m in range(M):
+        ev = times[off[m]:off[m + 1]]
+        oc.extract_clips(white, ev - oc.clip_tmid(50), nb[m], 50)
+        n_ev += len(ev)
+    secs["clips"] = (time.perf_counter() - t) * events_full / max(n_ev, 1)
+    rng = np.random.default_rng(0)
+    L = int(round(events_full / M))
+    Wm = int(round(float(np.mean(widths))))
+    clips = rng.normal(size=(L, 50, Wm)).astype(np.float32)
+    t = time.perf_counter()
+    red = oc.dimension_reduce_clips(clips, 10, 10000)
+    oc.stage2_features(red, 10, 10000)
+    secs["pca"] = (time.perf_counter() - t) * M
+    total = sum(secs.values())
+    wall = time.perf_counter() - t_all
+    log_fn("[bench] cpu baseline stage seconds (scaled to the full run): " +
+           json.dumps({k: round(v, 2) for k, v in secs.items()}) + f"; sample wall {wall:.1f}s")
+    return {
+        "value": M * N_full / total,
+        "unit": "channel-samples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle (C/numpy restatement of the reference), one thread, on a 384 x 240000 "
+                   "sample of the same synthetic workload; per-stage times scaled to 384 x "
+                   f"{N_full} (bandpass on 64 channels, covariance/apply on 4000 samples, "
+                   "whitening matrix on the 96-channel block x (384/96)^4, PCA on 1 "
+                   f"neighborhood of {L} events x {Wm} channels x 384)"),
+        "stage_seconds": {k: round(v, 3) for k, v in secs.items()},
+        "total_seconds": round(total, 2),
+        "host_threads_available": os.cpu_count(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +273,9 @@ def main():
                      "frac": rl[dom]["frac_hbm"], "traffic": None},
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        widths = pipe.widths
+        line["cpu_baseline"] = cpu_baseline(cfg.M, N, pipe.stats.get("events", 0), widths)
     print(json.dumps(line), flush=True)
 
 

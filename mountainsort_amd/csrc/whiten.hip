@@ -216,14 +216,17 @@ inline WPlan w_plan(int64_t M) {
 }
 
 __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave's stores complete (workgroup scope: a wait, no cache flush),
+    // then one agent-scope release by thread 0 publishes the workgroup's data
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     __shared__ int failed;
     if (threadIdx.x == 0) {
         failed = 0;
         __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // relaxed polling (coherent loads, no invalidate per poll)
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
                 ++spins > kBarrierSpinLimit) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -232,9 +235,11 @@ __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target,
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        // one acquire for the workgroup (the CU's caches are shared by its waves)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return failed == 0;
 }
 
