@@ -1,0 +1,734 @@
+// The processors of msamd.mp: the reference's hot-path processors with the
+// same names, versions, parameters and defaults
+// (packages/ms3/mountainsort3_main.cpp:145-198,343-350) and stage processors
+// for detection, neighborhood clips and PCA features that use the parameter
+// names of mountainsortalg.ms3alg (mountainsortalg_main.cpp:31-46,87-102;
+// p_mountainsort3.h:20-40).  Arrays are read into device memory, processed
+// by libmsamd (include/msamd.h) and written back.
+#include "processors.h"
+
+#include <sched.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "device.h"
+#include "mda.h"
+#include "msamd.h"
+
+namespace msamd_host {
+
+namespace {
+
+constexpr int64_t kBandpassOverlap = 60000;  // p_bandpass_filter.cpp:149
+constexpr int64_t kDetectPadding = 1000;     // p_mountainsort3.cpp:515
+
+// omp_get_max_threads() of the reference process: _request_num_threads when
+// given and nonzero (mountainsort3_main.cpp:428-434), else OMP_NUM_THREADS,
+// else the CPUs this process may run on.  It fixes the chunk plans.
+int reference_threads(const Ctx& c) {
+    const long long explicit_t = c.clp.get("reference_num_threads").to_long();
+    if (explicit_t > 0) return static_cast<int>(explicit_t);
+    const long long req = c.clp.get("_request_num_threads").to_long();
+    if (req > 0) return static_cast<int>(req);
+    if (const char* e = getenv("OMP_NUM_THREADS")) {
+        const int v = atoi(e);
+        if (v > 0) return v;
+    }
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) return std::max(1, CPU_COUNT(&set));
+    return std::max(1u, std::thread::hardware_concurrency());
+}
+
+bool lib_ok(Ctx& c, int status, const char* what) {
+    if (status == MSAMD_OK) return true;
+    const char* msg = msamd_last_error();
+    return c.fail(std::string(what) + ": " + (msg ? msg : "error"));
+}
+
+bool open_timeseries(Ctx& c, const std::string& param, TimeseriesSource& src) {
+    const std::vector<std::string> paths = c.param(param).to_string_list();
+    if (paths.empty()) return c.fail("missing parameter " + param);
+    return src.open(paths, c.err);
+}
+
+// Device-resident float32 copy of a timeseries.
+bool load_f32(Ctx& c, const TimeseriesSource& src, DeviceBuffer& X, hipStream_t s) {
+    return X.alloc(src.M * src.N * 4, c.err) && upload_timeseries(src, MDA_FLOAT32, X.get(), s, c.err);
+}
+
+bool write_device(Ctx& c, const std::string& path, int32_t type, const std::vector<int64_t>& dims,
+                  const void* dev, hipStream_t s) {
+    int64_t bytes = mda_bytes_per_entry(type);
+    for (int64_t d : dims) bytes *= d;
+    MdaWriter w;
+    return w.open(path, type, dims, c.err) && download_to_writer(dev, bytes, w, s, c.err) &&
+           w.close(c.err);
+}
+
+bool copy_file(Ctx& c, const std::string& from, const std::string& to) {
+    FILE* in = fopen(from.c_str(), "rb");
+    if (!in) return c.fail("unable to open " + from);
+    FILE* out = fopen((to + ".tmp").c_str(), "wb");
+    if (!out) {
+        fclose(in);
+        return c.fail("unable to open " + to + ".tmp");
+    }
+    std::vector<char> buf(1 << 20);
+    bool ok = true;
+    size_t n;
+    while ((n = fread(buf.data(), 1, buf.size(), in)) > 0) ok = ok && fwrite(buf.data(), 1, n, out) == n;
+    fclose(in);
+    ok = (fclose(out) == 0) && ok;
+    if (!ok || std::rename((to + ".tmp").c_str(), to.c_str()) != 0) {
+        std::remove((to + ".tmp").c_str());
+        return c.fail("unable to copy " + from + " to " + to);
+    }
+    return true;
+}
+
+// 1-based channel list -> 0-based, checked against M.
+bool channel_list(Ctx& c, const std::string& param, int64_t M, std::vector<int32_t>& out) {
+    out.clear();
+    for (int ch : int_list(c.str(param))) {
+        if (ch < 1 || ch > M)
+            return c.fail("channel " + std::to_string(ch) + " outside 1.." + std::to_string(M));
+        out.push_back(ch - 1);
+    }
+    return true;
+}
+
+// Neighborhood channel lists (0-based, central channel first) of every
+// channel: get_channels_from_geom (p_mountainsort3.cpp:473-491) with the
+// geometry of p_mountainsort3.cpp:159-168 (no geom file: all coordinates 0).
+bool neighborhoods(Ctx& c, int64_t M, std::vector<std::vector<int32_t>>& nb) {
+    std::vector<std::vector<double>> geom;
+    const std::string gpath = c.str("geom");
+    if (gpath.empty()) {
+        geom.assign(M, std::vector<double>(2, 0.0));
+    } else {
+        if (!read_geom_csv(gpath, geom, c.err)) return false;
+        if (static_cast<int64_t>(geom.size()) != M)
+            return c.fail("Error: Inconsistent number of channels in geometry file: " +
+                          std::to_string(geom.size()) + " <> " + std::to_string(M));
+    }
+    const double r = c.num("adjacency_radius");
+    nb.assign(M, {});
+    for (int64_t m = 0; m < M; ++m) {
+        nb[m].push_back(static_cast<int32_t>(m));
+        for (int64_t m2 = 0; m2 < M; ++m2) {
+            if (m2 == m) continue;
+            double s = 0.0;
+            for (size_t i = 0; i < geom[m].size(); ++i) {
+                const double d = geom[m][i] - geom[m2][i];
+                s += d * d;
+            }
+            const double dist = std::sqrt(s);
+            if ((r > 0 && dist <= r) || r < 0) nb[m].push_back(static_cast<int32_t>(m2));
+        }
+    }
+    return true;
+}
+
+// Events of a 2 x L (or R x L, R >= 2) firings-like array: row 0 central
+// channel (1-based), row 1 time.
+bool read_events(Ctx& c, int64_t M, std::vector<int32_t>& chan, std::vector<double>& time) {
+    std::vector<double> a;
+    MdaHeader h;
+    if (!mda_read_f64(c.str("event_times"), a, h, c.err)) return false;
+    if (h.dim(0) < 2) return c.fail("event_times must have rows (central channel, time)");
+    const int64_t R = h.dim(0), L = h.size() / R;
+    chan.resize(L);
+    time.resize(L);
+    for (int64_t i = 0; i < L; ++i) {
+        const double ch = a[R * i];
+        if (!(ch >= 1 && ch <= M) || ch != std::floor(ch))
+            return c.fail("event " + std::to_string(i) + ": central channel outside 1.." +
+                          std::to_string(M));
+        chan[i] = static_cast<int32_t>(ch) - 1;
+        time[i] = a[R * i + 1];
+    }
+    return true;
+}
+
+int clip_tmid(int64_t T) { return static_cast<int>((T + 1) / 2) - 1; }
+
+// ---------------------------------------------------------------- processors
+
+bool run_bandpass(Ctx& c) {
+    const double fs = c.num("samplerate"), fmin = c.num("freq_min"), fmax = c.num("freq_max");
+    const double fwid = c.num("freq_wid"), q = c.num("quantization_unit");
+    long long k = c.integer("subsample_factor");
+    if (k == 0) k = 1;  // p_bandpass_filter.cpp:121-122
+    if (k < 0) return c.fail("subsample_factor must be positive");
+    if (c.requirements_only) {
+        c.peak_ram_mb = staging_bytes() / (1024.0 * 1024.0);
+        return true;
+    }
+    if (fmax == 0) return copy_file(c, c.str("timeseries"), c.str("timeseries_out"));  // :117-119
+    TimeseriesSource src;
+    if (!open_timeseries(c, "timeseries", src)) return false;
+    const int64_t M = src.M, N = src.N, N2 = N / k;
+    const int64_t chunk = msamd_bandpass_chunk_size(M, reference_threads(c));
+    if (!msamd_bandpass_supported(chunk, kBandpassOverlap))
+        return c.fail("bandpass window length " + std::to_string(chunk + 2 * kBandpassOverlap) +
+                      " (chunk plan for M=" + std::to_string(M) + ", " +
+                      std::to_string(reference_threads(c)) + " threads) is not supported");
+    const bool in16 = src.common_type == MDA_INT16;
+    const int32_t out_type = q != 0 ? MDA_INT16 : MDA_FLOAT32;
+    Stream s;
+    DeviceBuffer X, Y, ws;
+    if (!s.create(c.err) || !X.alloc(M * N * (in16 ? 2 : 4), c.err) ||
+        !upload_timeseries(src, in16 ? MDA_INT16 : MDA_FLOAT32, X.get(), s.get(), c.err))
+        return false;
+    const int64_t wsb = msamd_bandpass_workspace_size(M, N, chunk, kBandpassOverlap);
+    if (!Y.alloc(M * N2 * mda_bytes_per_entry(out_type), c.err) || !ws.alloc(wsb, c.err))
+        return false;
+    if (!lib_ok(c,
+                msamd_bandpass_filter(X.get(), in16 ? MSAMD_DTYPE_INT16 : MSAMD_DTYPE_FLOAT32, M, N,
+                                      fs, fmin, fmax, fwid, chunk, kBandpassOverlap,
+                                      static_cast<int>(k), q, Y.get(), ws.get(), ws.bytes(),
+                                      s.get()),
+                "bandpass_filter"))
+        return false;
+    return write_device(c, c.str("timeseries_out"), out_type, {M, N2}, Y.get(), s.get());
+}
+
+// Covariance and whitening matrix of a device timeseries (M x N float32) into
+// a device M x M float64 column-major buffer.
+bool whitening_matrix_of(Ctx& c, const float* X, int64_t M, int64_t N, DeviceBuffer& Wd,
+                         hipStream_t s) {
+    DeviceBuffer C, ws1, ws2;
+    if (!C.alloc(M * M * 8, c.err) || !Wd.alloc(M * M * 8, c.err) ||
+        !ws1.alloc(msamd_whiten_covariance_workspace_size(M, N), c.err) ||
+        !ws2.alloc(msamd_whitening_matrix_workspace_size(M), c.err))
+        return false;
+    return lib_ok(c, msamd_whiten_covariance(X, M, N, C.as<double>(), ws1.get(), ws1.bytes(), s),
+                  "whiten_covariance") &&
+           lib_ok(c, msamd_whitening_matrix(C.as<double>(), M, Wd.as<double>(), ws2.get(),
+                                            ws2.bytes(), s),
+                  "whitening_matrix");
+}
+
+// Applies W (device, column-major M x M) and writes the result.
+bool apply_and_write(Ctx& c, const float* X, int64_t M, int64_t N, const double* W, double q,
+                     hipStream_t s) {
+    DeviceBuffer Y, ws;
+    const int32_t out_type = q > 0 ? MDA_INT16 : MDA_FLOAT32;
+    if (!Y.alloc(M * N * mda_bytes_per_entry(out_type), c.err) ||
+        !ws.alloc(msamd_whiten_apply_workspace_size(M), c.err))
+        return false;
+    const int st = q > 0 ? msamd_whiten_apply(X, M, N, W, nullptr, q, Y.as<int16_t>(), ws.get(),
+                                              ws.bytes(), s)
+                         : msamd_whiten_apply(X, M, N, W, Y.as<float>(), 0.0, nullptr, ws.get(),
+                                              ws.bytes(), s);
+    if (!lib_ok(c, st, "whiten_apply")) return false;
+    return write_device(c, c.str("timeseries_out"), out_type, {M, N}, Y.get(), s);
+}
+
+bool run_whiten(Ctx& c) {
+    if (c.requirements_only) {
+        c.peak_ram_mb = staging_bytes() / (1024.0 * 1024.0);
+        return true;
+    }
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X, W;
+    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err) || !load_f32(c, src, X, s.get()))
+        return false;
+    return whitening_matrix_of(c, X.as<float>(), src.M, src.N, W, s.get()) &&
+           apply_and_write(c, X.as<float>(), src.M, src.N, W.as<double>(),
+                           c.num("quantization_unit"), s.get());
+}
+
+bool run_compute_whitening_matrix(Ctx& c) {
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X, W;
+    if (!open_timeseries(c, "timeseries_list", src) || !s.create(c.err) ||
+        !load_f32(c, src, X, s.get()))
+        return false;
+    std::vector<int32_t> ch;
+    if (!channel_list(c, "channels", src.M, ch)) return false;
+    const float* Xs = X.as<float>();
+    int64_t M2 = src.M;
+    DeviceBuffer sub, dstarts, dch;
+    if (!ch.empty()) {
+        // The channel subset as an M2 x N timeseries: consecutive "clips" of
+        // Tb samples laid end to end are exactly that array.
+        M2 = static_cast<int64_t>(ch.size());
+        const int32_t Tb = 4096;
+        const int64_t L = (src.N + Tb - 1) / Tb;
+        std::vector<int64_t> starts(L);
+        for (int64_t i = 0; i < L; ++i) starts[i] = i * Tb;
+        if (!sub.alloc(std::max<int64_t>(1, M2 * Tb * L) * 4, c.err) ||
+            !dstarts.alloc(L * 8, c.err) || !dch.alloc(M2 * 4, c.err) ||
+            !to_device(dstarts.get(), starts.data(), L * 8, s.get(), c.err) ||
+            !to_device(dch.get(), ch.data(), M2 * 4, s.get(), c.err))
+            return false;
+        if (!lib_ok(c, msamd_extract_clips(Xs, src.M, src.N, dstarts.as<int64_t>(), L,
+                                           dch.as<int32_t>(), static_cast<int32_t>(M2), Tb,
+                                           sub.as<float>(), s.get()),
+                    "extract_clips"))
+            return false;
+        Xs = sub.as<float>();
+    }
+    if (!whitening_matrix_of(c, Xs, M2, src.N, W, s.get())) return false;
+    return write_device(c, c.str("whitening_matrix_out"), MDA_FLOAT64, {M2, M2}, W.get(), s.get());
+}
+
+bool run_apply_whitening_matrix(Ctx& c) {
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X, W;
+    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err)) return false;
+    std::vector<double> w;
+    MdaHeader h;
+    if (!mda_read_f64(c.str("whitening_matrix"), w, h, c.err)) return false;
+    if (h.dim(0) != src.M || h.size() != src.M * src.M)
+        return c.fail("whitening matrix is not " + std::to_string(src.M) + " x " +
+                      std::to_string(src.M));
+    if (!load_f32(c, src, X, s.get()) || !W.alloc(src.M * src.M * 8, c.err) ||
+        !to_device(W.get(), w.data(), src.M * src.M * 8, s.get(), c.err))
+        return false;
+    return apply_and_write(c, X.as<float>(), src.M, src.N, W.as<double>(),
+                           c.num("quantization_unit"), s.get());
+}
+
+bool run_mask(Ctx& c) {
+    const double thr = c.num("threshold");
+    const int64_t isz = static_cast<int64_t>(c.num("interval_size"));
+    if (thr == 0 || isz == 0)  // p_mask_out_artifacts.cpp:26-29
+        return c.fail("Problem with input parameters. Either threshold or interval_size is zero.");
+    if (isz < 0) return c.fail("interval_size must be positive");
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X, Y, ws, use;
+    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err) || !load_f32(c, src, X, s.get()))
+        return false;
+    const int64_t M = src.M, N = src.N;
+    if (!Y.alloc(M * N * 4, c.err) || !ws.alloc(msamd_mask_workspace_size(M, N, isz), c.err) ||
+        !use.alloc((N / isz + 1) * 4, c.err))
+        return false;
+    if (!lib_ok(c, msamd_mask_out_artifacts(X.as<float>(), M, N, thr, isz, Y.as<float>(),
+                                            use.as<int32_t>(), ws.get(), ws.bytes(), s.get()),
+                "mask_out_artifacts"))
+        return false;
+    return write_device(c, c.str("timeseries_out"), MDA_FLOAT32, {M, N}, Y.get(), s.get());
+}
+
+bool run_extract_clips(Ctx& c) {
+    std::vector<double> et;
+    MdaHeader h;
+    if (!mda_read_f64(c.str("event_times"), et, h, c.err)) return false;
+    if (h.dim(0) > 1 && h.dim(1) > 1)  // p_extract_clips.cpp:39-42
+        return c.fail("Error: the input (event times) must be a vector.");
+    const int64_t T = c.integer("clip_size");
+    if (!T) return c.fail("Unexpected: Clip size is zero.");
+    if (T < 0) return c.fail("clip_size must be positive");
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X, clips, dstarts, dch;
+    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err)) return false;
+    std::vector<int32_t> ch;
+    if (!channel_list(c, "channels", src.M, ch)) return false;
+    if (ch.empty()) {
+        ch.resize(src.M);
+        std::iota(ch.begin(), ch.end(), 0);
+    }
+    const int64_t L = static_cast<int64_t>(et.size()), W = static_cast<int64_t>(ch.size());
+    std::vector<int64_t> starts(L);
+    const int tmid = clip_tmid(T);
+    for (int64_t i = 0; i < L; ++i) starts[i] = static_cast<int64_t>(et[i] - tmid);  // :59
+    if (!load_f32(c, src, X, s.get()) || !clips.alloc(W * T * L * 4, c.err) ||
+        !dstarts.alloc(L * 8, c.err) || !dch.alloc(W * 4, c.err) ||
+        !to_device(dstarts.get(), starts.data(), L * 8, s.get(), c.err) ||
+        !to_device(dch.get(), ch.data(), W * 4, s.get(), c.err))
+        return false;
+    if (!lib_ok(c, msamd_extract_clips(X.as<float>(), src.M, src.N, dstarts.as<int64_t>(), L,
+                                       dch.as<int32_t>(), static_cast<int32_t>(W),
+                                       static_cast<int32_t>(T), clips.as<float>(), s.get()),
+                "extract_clips"))
+        return false;
+    return write_device(c, c.str("clips_out"), MDA_FLOAT32, {W, T, L}, clips.get(), s.get());
+}
+
+// Detection on every channel; returns (channel, time) pairs in the order of
+// the ms3alg firings: by time, ties in channel order (p_mountainsort3.cpp:245-263).
+bool detect_all(Ctx& c, const float* X, int64_t M, int64_t N, hipStream_t s,
+                std::vector<int32_t>& chan, std::vector<int64_t>& time) {
+    const double thr = c.num("detect_threshold");
+    const double interval = static_cast<double>(static_cast<int>(c.num("detect_interval")));
+    const int sign = c.param("detect_sign").to_int();
+    const int64_t t1 = static_cast<int64_t>(c.num("t1")), t2 = static_cast<int64_t>(c.num("t2"));
+    const int64_t t_start = t1 < 0 ? 0 : t1, t_end = t2 < 0 ? N - 1 : t2;
+    if (t_end >= N || t_start > t_end + 1) return c.fail("t1/t2 outside the timeseries");
+    const int64_t n_range = t_end - t_start + 1;
+    const int64_t chunk = msamd_detect_chunk_size(M, n_range, reference_threads(c));
+    std::vector<int32_t> all(M);
+    std::iota(all.begin(), all.end(), 0);
+    DeviceBuffer dch, ws, dtimes, doff;
+    const int64_t wsb = msamd_detect_workspace_size(n_range, chunk, static_cast<int32_t>(M), interval);
+    if (!dch.alloc(M * 4, c.err) || !to_device(dch.get(), all.data(), M * 4, s, c.err) ||
+        !ws.alloc(wsb, c.err))
+        return false;
+    int64_t total = 0;
+    if (!lib_ok(c, msamd_detect_events(X, M, N, t1, t2, chunk, kDetectPadding, dch.as<int32_t>(),
+                                       static_cast<int32_t>(M), thr, interval, sign, ws.get(),
+                                       ws.bytes(), &total, s),
+                "detect_events"))
+        return false;
+    if (!dtimes.alloc(std::max<int64_t>(total, 1) * 8, c.err) || !doff.alloc((M + 1) * 8, c.err))
+        return false;
+    if (!lib_ok(c, msamd_detect_fetch(ws.get(), static_cast<int32_t>(M), dtimes.as<int64_t>(),
+                                      doff.as<int64_t>(), s),
+                "detect_fetch"))
+        return false;
+    std::vector<int64_t> t(total), off(M + 1);
+    if (!to_host(t.data(), dtimes.get(), total * 8, s, c.err) ||
+        !to_host(off.data(), doff.get(), (M + 1) * 8, s, c.err))
+        return false;
+    std::vector<int32_t> ch(total);
+    for (int64_t m = 0; m < M; ++m)
+        for (int64_t i = off[m]; i < off[m + 1]; ++i) ch[i] = static_cast<int32_t>(m);
+    std::vector<int64_t> order(total);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return t[a] < t[b]; });
+    chan.resize(total);
+    time.resize(total);
+    for (int64_t i = 0; i < total; ++i) {
+        chan[i] = ch[order[i]];
+        time[i] = t[order[i]];
+    }
+    return true;
+}
+
+bool run_detect(Ctx& c) {
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X;
+    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err) || !load_f32(c, src, X, s.get()))
+        return false;
+    std::vector<int32_t> chan;
+    std::vector<int64_t> time;
+    if (!detect_all(c, X.as<float>(), src.M, src.N, s.get(), chan, time)) return false;
+    const int64_t L = static_cast<int64_t>(time.size());
+    std::vector<double> out(2 * L);
+    for (int64_t i = 0; i < L; ++i) {
+        out[2 * i] = chan[i] + 1;
+        out[2 * i + 1] = static_cast<double>(time[i]);
+    }
+    return mda_write(c.str("event_times_out"), MDA_FLOAT64, {2, L}, out.data(), c.err);
+}
+
+bool run_neighborhood_clips(Ctx& c) {
+    TimeseriesSource src;
+    if (!open_timeseries(c, "timeseries", src)) return false;
+    const int64_t M = src.M;
+    const long long cc = c.integer("central_channel");
+    if (cc < 1 || cc > M) return c.fail("central_channel outside 1.." + std::to_string(M));
+    const int64_t T = c.integer("clip_size");
+    if (T < 1) return c.fail("clip_size must be positive");
+    std::vector<std::vector<int32_t>> nb;
+    std::vector<int32_t> chan;
+    std::vector<double> time;
+    if (!neighborhoods(c, M, nb) || !read_events(c, M, chan, time)) return false;
+    const std::vector<int32_t>& ch = nb[cc - 1];
+    std::vector<int64_t> starts;
+    const int tmid = clip_tmid(T);
+    for (size_t i = 0; i < chan.size(); ++i)
+        if (chan[i] == cc - 1) starts.push_back(static_cast<int64_t>(time[i] - tmid));
+    const int64_t L = static_cast<int64_t>(starts.size()), W = static_cast<int64_t>(ch.size());
+    Stream s;
+    DeviceBuffer X, clips, dstarts, dch;
+    if (!s.create(c.err) || !load_f32(c, src, X, s.get()) || !clips.alloc(W * T * L * 4, c.err) ||
+        !dstarts.alloc(L * 8, c.err) || !dch.alloc(W * 4, c.err) ||
+        !to_device(dstarts.get(), starts.data(), L * 8, s.get(), c.err) ||
+        !to_device(dch.get(), ch.data(), W * 4, s.get(), c.err))
+        return false;
+    if (!lib_ok(c, msamd_extract_clips(X.as<float>(), M, src.N, dstarts.as<int64_t>(), L,
+                                       dch.as<int32_t>(), static_cast<int32_t>(W),
+                                       static_cast<int32_t>(T), clips.as<float>(), s.get()),
+                "extract_clips"))
+        return false;
+    return write_device(c, c.str("clips_out"), MDA_FLOAT32, {W, T, L}, clips.get(), s.get());
+}
+
+// Clips and two-stage PCA features of events grouped by central channel.
+// chan/time: events in any order; features come back in that order
+// (nf2 x L, column i for event i).
+bool features_of(Ctx& c, const float* X, int64_t M, int64_t N, const std::vector<int32_t>& chan,
+                 const std::vector<int64_t>& center, hipStream_t s, std::vector<float>& feat) {
+    const int64_t T = c.integer("clip_size");
+    const int64_t nf1 = c.integer("num_features_per_channel"), nf2 = c.integer("num_features");
+    const int64_t maxs = c.integer("max_pca_samples");
+    if (T < 1) return c.fail("clip_size must be positive");
+    std::vector<std::vector<int32_t>> nb;
+    if (!neighborhoods(c, M, nb)) return false;
+    const int32_t G = static_cast<int32_t>(M);
+    const int64_t L = static_cast<int64_t>(chan.size());
+    // events grouped by central channel, input order within a group
+    std::vector<int64_t> eo(G + 1, 0);
+    for (int32_t ch : chan) eo[ch + 1]++;
+    for (int32_t g = 0; g < G; ++g) eo[g + 1] += eo[g];
+    std::vector<int64_t> slot(eo.begin(), eo.end() - 1), pos(L), times(L);
+    for (int64_t i = 0; i < L; ++i) {
+        pos[i] = slot[chan[i]]++;
+        times[pos[i]] = center[i];
+    }
+    std::vector<int64_t> num(G), no(G + 1, 0), co(G + 1, 0);
+    std::vector<int32_t> width(G), nc;
+    for (int32_t g = 0; g < G; ++g) {
+        num[g] = eo[g + 1] - eo[g];
+        width[g] = static_cast<int32_t>(nb[g].size());
+        if (width[g] > 16)
+            return c.fail("neighborhood of channel " + std::to_string(g + 1) + " has " +
+                          std::to_string(width[g]) +
+                          " channels; the PCA kernels take at most 16 (reduce adjacency_radius)");
+        nc.insert(nc.end(), nb[g].begin(), nb[g].end());
+        no[g + 1] = no[g] + width[g];
+        co[g + 1] = co[g] + num[g] * width[g] * T;
+    }
+    const int64_t wsb = msamd_pca_workspace_size(G, num.data(), width.data(), static_cast<int32_t>(T),
+                                                 static_cast<int32_t>(nf1), static_cast<int32_t>(nf2),
+                                                 maxs);
+    if (wsb < 0) return lib_ok(c, MSAMD_ERR_ARG, "pca_workspace_size");
+    int64_t red = 0;
+    for (int32_t g = 0; g < G; ++g) red += width[g] * nf1 * num[g];
+    DeviceBuffer dtimes, deo, dnc, dno, dco, clips, reduced, features, ws;
+    if (!dtimes.alloc(L * 8, c.err) || !deo.alloc((G + 1) * 8, c.err) ||
+        !dnc.alloc(static_cast<int64_t>(nc.size()) * 4, c.err) || !dno.alloc((G + 1) * 8, c.err) ||
+        !dco.alloc((G + 1) * 8, c.err) || !clips.alloc(co[G] * 4, c.err) ||
+        !reduced.alloc(red * 4, c.err) || !features.alloc(nf2 * L * 4, c.err) ||
+        !ws.alloc(wsb, c.err))
+        return false;
+    if (!to_device(dtimes.get(), times.data(), L * 8, s, c.err) ||
+        !to_device(deo.get(), eo.data(), (G + 1) * 8, s, c.err) ||
+        !to_device(dnc.get(), nc.data(), static_cast<int64_t>(nc.size()) * 4, s, c.err) ||
+        !to_device(dno.get(), no.data(), (G + 1) * 8, s, c.err) ||
+        !to_device(dco.get(), co.data(), (G + 1) * 8, s, c.err))
+        return false;
+    if (!lib_ok(c, msamd_extract_clips_batched(X, M, N, dtimes.as<int64_t>(), deo.as<int64_t>(), L,
+                                               dnc.as<int32_t>(), dno.as<int64_t>(), G,
+                                               static_cast<int32_t>(T), dco.as<int64_t>(),
+                                               clips.as<float>(), s),
+                "extract_clips_batched") ||
+        !lib_ok(c, msamd_pca_features(clips.as<float>(), G, num.data(), width.data(),
+                                      static_cast<int32_t>(T), static_cast<int32_t>(nf1),
+                                      static_cast<int32_t>(nf2), maxs, reduced.as<float>(),
+                                      features.as<float>(), ws.get(), ws.bytes(), s),
+                "pca_features"))
+        return false;
+    std::vector<float> grouped(nf2 * L);
+    if (!to_host(grouped.data(), features.get(), nf2 * L * 4, s, c.err)) return false;
+    feat.assign(nf2 * L, 0.0f);
+    for (int64_t i = 0; i < L; ++i)
+        std::copy_n(&grouped[nf2 * pos[i]], nf2, &feat[nf2 * i]);
+    return true;
+}
+
+bool run_pca_features(Ctx& c) {
+    TimeseriesSource src;
+    if (!open_timeseries(c, "timeseries", src)) return false;
+    std::vector<int32_t> chan;
+    std::vector<double> time;
+    if (!read_events(c, src.M, chan, time)) return false;
+    std::vector<int64_t> center(time.size());
+    for (size_t i = 0; i < time.size(); ++i) center[i] = static_cast<int64_t>(time[i]);
+    Stream s;
+    DeviceBuffer X;
+    if (!s.create(c.err) || !load_f32(c, src, X, s.get())) return false;
+    std::vector<float> feat;
+    if (!features_of(c, X.as<float>(), src.M, src.N, chan, center, s.get(), feat)) return false;
+    const int64_t nf2 = c.integer("num_features");
+    return mda_write(c.str("features_out"), MDA_FLOAT32, {nf2, static_cast<int64_t>(time.size())},
+                     feat.data(), c.err);
+}
+
+// --------------------------------------------------------------- the registry
+
+std::vector<ProcessorSpec> make_registry() {
+    std::vector<ProcessorSpec> v;
+    {
+        ProcessorSpec p;
+        p.name = "ms3.bandpass_filter";
+        p.version = "0.20";
+        p.input("timeseries").output("timeseries_out");
+        p.required("samplerate").required("freq_min").required("freq_max");
+        p.optional("freq_wid", "", 1000LL).optional("quantization_unit", "", 0LL);
+        p.optional("subsample_factor", "", 1LL);
+        p.can_return_requirements = true;
+        p.run = run_bandpass;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.whiten";
+        p.version = "0.1";
+        p.input("timeseries").output("timeseries_out");
+        p.optional("quantization_unit", "", 0LL);
+        p.can_return_requirements = true;
+        p.run = run_whiten;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.compute_whitening_matrix";
+        p.version = "0.11";
+        p.input("timeseries_list").output("whitening_matrix_out");
+        p.optional("channels");
+        p.run = run_compute_whitening_matrix;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.apply_whitening_matrix";
+        p.version = "0.1";
+        p.input("timeseries").input("whitening_matrix").output("timeseries_out");
+        p.optional("quantization_unit", "", 0LL);
+        p.run = run_apply_whitening_matrix;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.extract_clips";
+        p.version = "0.11";
+        p.input("timeseries").input("event_times", "1xL array of event timestamps");
+        p.output("clips_out");
+        p.required("clip_size").optional("channels");
+        p.run = run_extract_clips;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.mask_out_artifacts";
+        p.version = "0.1";
+        p.input("timeseries").output("timeseries_out");
+        p.optional("threshold", "", 6LL).optional("interval_size", "", 2000LL);
+        p.run = run_mask;
+        v.push_back(p);
+    }
+    const char* kRefThreads =
+        "Thread count of the reference run whose time-chunk plan is reproduced (0: "
+        "_request_num_threads, else OMP_NUM_THREADS, else the available CPUs)";
+    {
+        ProcessorSpec p;
+        p.name = "msamd.detect_events";
+        p.version = "0.1";
+        p.description =
+            "Detection phase of mountainsortalg.ms3alg on every channel: rows 0-1 of its "
+            "firings (central channel, time) in firings order";
+        p.input("timeseries", "Preprocessed timeseries");
+        p.output("event_times_out", "2xL array: central channel (1-based), time (0-based)");
+        p.optional("detect_interval", "", 10LL).optional("detect_threshold", "", 3LL);
+        p.optional("detect_sign", "", 0LL);
+        p.optional("t1", "Start timepoint (default 0 means start at beginning)", 0LL);
+        p.optional("t2", "End timepoint (default -1 means go to end of timeseries)", -1LL);
+        p.optional("reference_num_threads", kRefThreads, 0LL);
+        p.run = run_detect;
+        v.push_back(p);
+    }
+    const char* kRadius =
+        "Radius of local sorting neighborhood, corresponding to the geometry file. 0 means "
+        "each channel is sorted independently. -1 means all channels are included in every "
+        "neighborhood.";
+    {
+        ProcessorSpec p;
+        p.name = "msamd.extract_neighborhood_clips";
+        p.version = "0.1";
+        p.description = "Clips of the events of one central channel over its neighborhood "
+                        "channels (central channel first, then ascending)";
+        p.input("timeseries").input("event_times", "2xL array (central channel, time)");
+        p.input("geom", "", true);
+        p.output("clips_out", "WxTxL float32");
+        p.required("central_channel");
+        p.optional("adjacency_radius", kRadius, -1LL).optional("clip_size", "", 50LL);
+        p.run = run_neighborhood_clips;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "msamd.pca_features";
+        p.version = "0.1";
+        p.description = "Two-stage per-neighborhood PCA features (the isosplit5 input of "
+                        "ms3alg) of every event";
+        p.input("timeseries").input("event_times", "2xL array (central channel, time)");
+        p.input("geom", "", true);
+        p.output("features_out", "num_features x L float32, column i for event i");
+        p.optional("adjacency_radius", kRadius, -1LL).optional("clip_size", "", 50LL);
+        p.optional("num_features", "", 10LL).optional("num_features_per_channel", "", 10LL);
+        p.optional("max_pca_samples", "", 10000LL);
+        p.run = run_pca_features;
+        v.push_back(p);
+    }
+    return v;
+}
+
+}  // namespace
+
+Value Ctx::param(const std::string& name) const {
+    if (clp.has(name)) return clp.get(name);
+    for (const auto& p : spec.params)
+        if (p.name == name && p.has_default) return p.default_value;
+    return Value();
+}
+
+const std::vector<ProcessorSpec>& processors() {
+    static const std::vector<ProcessorSpec> reg = make_registry();
+    return reg;
+}
+
+const ProcessorSpec* find_processor(const std::string& name) {
+    for (const auto& p : processors())
+        if (p.name == name) return &p;
+    return nullptr;
+}
+
+Json processor_json(const ProcessorSpec& p, const std::string& exe) {
+    Json j = Json::object();
+    j["name"] = p.name;
+    j["version"] = p.version;
+    j["description"] = p.description;
+    auto files = [](const std::vector<FileSpec>& fs) {
+        Json a = Json::array();
+        for (const auto& f : fs) {
+            Json o = Json::object();
+            o["name"] = f.name;
+            o["description"] = f.description;
+            o["optional"] = f.optional;
+            a.push_back(o);
+        }
+        return a;
+    };
+    j["inputs"] = files(p.inputs);
+    j["outputs"] = files(p.outputs);
+    Json ps = Json::array();
+    for (const auto& q : p.params) {
+        Json o = Json::object();
+        o["name"] = q.name;
+        o["description"] = q.description;
+        o["optional"] = q.optional;
+        if (q.has_default) o["default_value"] = q.default_json;
+        ps.push_back(o);
+    }
+    j["parameters"] = ps;
+    j["exe_command"] = exe + " " + p.name + " $(arguments)";
+    if (p.can_return_requirements)
+        j["requirements_command"] = exe + " requirements " + p.name + " $(arguments)";
+    return j;
+}
+
+Json spec_json(const std::string& exe) {
+    Json arr = Json::array();
+    for (const auto& p : processors()) arr.push_back(processor_json(p, exe));
+    Json j = Json::object();
+    j["processors"] = arr;
+    return j;
+}
+
+}  // namespace msamd_host
