@@ -106,6 +106,90 @@ class Pipeline:
         return feats
 
 
+def learned_leg(dev, pipe, cfg, reps=3, train_seconds=60.0, steps=4000):
+    """The learned detector: trained on a separate 60 s recording of the same
+    probe (seed 4) labeled by plain detection, then run on the benchmark's
+    whitened recording (seed 2) next to plain detection.  Reports recall and
+    precision against plain detection (+-1 sample), window classes, and the
+    detection and detect->clips->PCA times of both forms."""
+    import numpy as np
+    import torch
+    from mountainsort_amd import frontend as fe
+    from mountainsort_amd import learned, synth
+
+    fe_ = fe
+    Ntr = int(round(train_seconds * 30000))
+    cfg_tr = synth.config2(N=Ntr, seed=4)
+    raw_tr = synth.generate(cfg_tr, device=dev)
+    tr = Pipeline(dev, cfg_tr.M, Ntr, cfg_tr, with_pca=False)
+    tr.run(raw_tr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    params, trace = learned.train_detector(tr.white, 3.0, 10, -1, num_threads=pipe.num_threads,
+                                           steps=steps, seed=4)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    trace = trace.cpu().numpy()
+    del tr, raw_tr
+    torch.cuda.empty_cache()
+
+    X = pipe.white
+    det = learned.LearnedDetection(params)
+
+    def plain_detect():
+        return fe_.detect_events(X, 3.0, 10, -1, num_threads=pipe.num_threads)
+
+    def learned_detect():
+        off, t, _ = det(X, 3.0, 10, -1, num_threads=pipe.num_threads)
+        return off, t
+
+    def features(off, t):
+        clips, co = fe_.extract_clips_batched(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50)
+        if pipe.with_pca:
+            fe_.pca_features(clips, co, off, pipe.nb_offsets, 50)
+
+    def timed(fn, with_features):
+        ms_d, ms_all = [], []
+        for _ in range(reps + 1):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            c = torch.cuda.Event(enable_timing=True)
+            a.record()
+            off, t = fn()
+            b.record()
+            if with_features:
+                features(off, t)
+            c.record()
+            torch.cuda.synchronize()
+            ms_d.append(a.elapsed_time(b))
+            ms_all.append(a.elapsed_time(c))
+        return float(np.mean(ms_d[1:])), float(np.mean(ms_all[1:])), off, t
+
+    pd, pall, off_p, t_p = timed(plain_detect, True)
+    ld, lall, off_l, t_l = timed(learned_detect, True)
+    _, _, counts = det(X, 3.0, 10, -1, num_threads=pipe.num_threads)
+    hit_p, _, n_p = learned.match_events(off_l, t_l, off_p, t_p)
+    hit_l, _, n_l = learned.match_events(off_p, t_p, off_l, t_l)
+    out = {
+        "training": {"recording": f"384 ch x {train_seconds:g} s (synth.config2 seed 4)",
+                     "steps": steps, "batch": 256, "seg_windows": 16, "pos_weight": 20.0,
+                     "seconds": round(train_s, 3),
+                     "loss_first_last": [round(float(trace[:50].mean()), 4),
+                                         round(float(trace[-50:].mean()), 4)]},
+        "score_band": [0.2, 0.8],
+        "recall_vs_plain": round(hit_p / max(n_p, 1), 5),
+        "precision_vs_plain": round(hit_l / max(n_l, 1), 5),
+        "events_plain": n_p, "events_learned": n_l,
+        "windows_skipped_uncertain_present": counts,
+        "detect_ms": {"plain": round(pd, 3), "learned": round(ld, 3)},
+        "detect_clips_pca_ms": {"plain": round(pall, 3), "learned": round(lall, 3)},
+        "speedup_detect": round(pd / ld, 3),
+        "speedup_detect_clips_pca": round(pall / lall, 3),
+    }
+    log("[bench] learned: " + json.dumps(out))
+    return out
+
+
 def cpu_baseline(M, N_full, events_full, widths, log_fn=log):
     """Time the oracle (the CPU restatement of the reference, single thread)
     stage by stage on a bounded sample of the same workload and scale each
@@ -197,6 +281,7 @@ def main():
     ap.add_argument("--seconds", type=float, default=300.0, help="recording length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pca", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is "
@@ -273,6 +358,8 @@ def main():
                      "frac": rl[dom]["frac_hbm"], "traffic": None},
         "cpu_baseline": None,
     }
+    if not args.no_learned:
+        line["learned"] = learned_leg(dev, pipe, cfg)
     if not args.no_cpu_baseline:
         widths = pipe.widths
         line["cpu_baseline"] = cpu_baseline(cfg.M, N, pipe.stats.get("events", 0), widths)

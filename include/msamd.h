@@ -229,6 +229,82 @@ int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, double thresh
                              int64_t interval_size, float* Y, int32_t* use_it, void* workspace,
                              int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned spike-presence detector.  No counterpart in the reference: a
+ * 3-layer temporal convolution network (32 hidden channels, kernel 9, ReLU,
+ * strides 8 and 4) over the 7-channel window c-3..c+3 scores every
+ * 32-sample window [32w, 32w+32) of every channel; detection then only
+ * looks where the network proposes, with the reference's detection rule
+ * (detect_events.cpp:69-135, time-chunk plan p_mountainsort3.cpp:170-216)
+ * deciding uncertain windows.  Parameters: one float vector of
+ * msamd_tcn_num_params() values, W1 [32][7][9] | b1 [32] | W2 [32][32][9] |
+ * b2 [32] | W3 [32][9] | b3; weights are used as bf16, sums are fp32.
+ * Windows: nW = ceil(N / 32); window-indexed arrays are [nW][M].
+ * ------------------------------------------------------------------------ */
+
+int32_t msamd_tcn_num_params(void);
+
+/* He-normal weights from a counter-based generator, zero biases, b3 = -3. */
+int msamd_tcn_init(float* params, uint64_t seed, void* stream);
+
+/* Training labels from plain detection output (times/offsets as returned by
+ * msamd_detect_fetch for channels 0..M-1): labels[w][c] = 1 when window w of
+ * channel c holds an event, else 0.  positives (may be NULL): one entry
+ * w*M + c per event, in event order. */
+int msamd_tcn_labels(const int64_t* times, const int64_t* offsets, int32_t M, int64_t N,
+                     uint8_t* labels, int64_t* positives, void* stream);
+
+/* Workspace bytes for msamd_tcn_grad / msamd_tcn_train (-1 if invalid);
+ * seg_windows (windows per training segment) is 1..64. */
+int64_t msamd_tcn_train_workspace_size(int32_t batch, int32_t seg_windows);
+
+/* Mean weighted binary cross-entropy over the windows of `batch` segments
+ * (segment b: channel seg_channel[b], windows seg_window[b] ..
+ * + seg_windows - 1; windows outside the recording have label 0) and its
+ * gradient with respect to every parameter (grad: msamd_tcn_num_params()
+ * floats; loss: one float).  Positive windows weigh pos_weight. */
+int msamd_tcn_grad(const float* X, int64_t M, int64_t N, const uint8_t* labels,
+                   const float* params, const int32_t* seg_channel, const int64_t* seg_window,
+                   int32_t batch, int32_t seg_windows, double pos_weight, float* grad,
+                   float* loss, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* `steps` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on params (fp32
+ * master copy) from adam_state (2 * num_params floats, zero before the first
+ * step; step0 = steps already taken).  Each step samples `batch` segments
+ * (half at random windows, half around a random entry of positives) with a
+ * counter-based generator seeded by seed.  loss_trace (may be NULL): the
+ * loss of every step. */
+int msamd_tcn_train(const float* X, int64_t M, int64_t N, const uint8_t* labels,
+                    const int64_t* positives, int64_t num_positives, float* params,
+                    float* adam_state, int32_t steps, int32_t step0, int32_t batch,
+                    int32_t seg_windows, double lr, double pos_weight, uint64_t seed,
+                    float* loss_trace, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Inference logits of every window of every channel (logits: [nW][M]),
+ * computed by the bf16 MFMA inference kernel. */
+int64_t msamd_tcn_score_workspace_size(void);
+int msamd_tcn_score(const float* X, int64_t M, int64_t N, const float* params, float* logits,
+                    void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Workspace bytes for msamd_tcn_detect. */
+int64_t msamd_tcn_detect_workspace_size(int64_t N_file, int64_t n_range, int64_t chunk_size,
+                                        int32_t M, double detect_interval);
+
+/* Learned detection on every channel 0..M-1.  Statistics and chunk plan are
+ * those of msamd_detect_events.  Each window is classified by its score
+ * s = sigmoid(z): s < score_lo: no event; s > score_hi: the first maximum of
+ * v over the window's samples, kept when it exceeds the threshold and passes
+ * the detect_interval suppression stencil (one event per window); otherwise
+ * the plain rule decides every sample.
+ * Fetch the result with msamd_detect_fetch(workspace, M, ...).
+ * class_counts_host (may be NULL): windows per class {skipped, uncertain,
+ * present} over the detection range. */
+int msamd_tcn_detect(const float* X, int64_t M, int64_t N_file, int64_t t1, int64_t t2,
+                     int64_t chunk_size, int64_t padding, const float* params,
+                     double detect_threshold, double detect_interval, int detect_sign,
+                     double score_lo, double score_hi, void* workspace, int64_t workspace_bytes,
+                     int64_t* num_events_host, int64_t* class_counts_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

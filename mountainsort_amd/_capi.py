@@ -42,6 +42,18 @@ SIGNATURES = {
     "msamd_whiten_apply": (cint, [vp, i64, i64, vp, vp, f64, vp, vp, i64, vp]),
     "msamd_mask_workspace_size": (i64, [i64, i64, i64]),
     "msamd_mask_out_artifacts": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
+    "msamd_tcn_num_params": (i32, []),
+    "msamd_tcn_init": (cint, [vp, ctypes.c_uint64, vp]),
+    "msamd_tcn_labels": (cint, [vp, vp, i32, i64, vp, vp, vp]),
+    "msamd_tcn_train_workspace_size": (i64, [i32, i32]),
+    "msamd_tcn_grad": (cint, [vp, i64, i64, vp, vp, vp, vp, i32, i32, f64, vp, vp, vp, i64, vp]),
+    "msamd_tcn_train": (cint, [vp, i64, i64, vp, vp, i64, vp, vp, i32, i32, i32, i32, f64, f64,
+                               ctypes.c_uint64, vp, vp, i64, vp]),
+    "msamd_tcn_score_workspace_size": (i64, []),
+    "msamd_tcn_score": (cint, [vp, i64, i64, vp, vp, vp, i64, vp]),
+    "msamd_tcn_detect_workspace_size": (i64, [i64, i64, i64, i32, f64]),
+    "msamd_tcn_detect": (cint, [vp, i64, i64, i64, i64, i64, i64, vp, f64, f64, cint, f64, f64,
+                                vp, i64, ctypes.POINTER(i64), vp, vp]),
 }
 
 

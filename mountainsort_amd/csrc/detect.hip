@@ -16,6 +16,7 @@
 // bit-identical.  Contraction is off in this file: every double operation
 // is rounded as on the reference's x86-64 build.
 #include "common.h"
+#include "tcn.h"
 
 #pragma clang fp contract(off)
 
@@ -239,6 +240,112 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
     counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
 }
 
+// Gated scan of the learned detector: one thread per (segment, channel), the
+// channel list being 0..M-1.  Every 32-sample window w overlapping the
+// segment is handled by its class: 0 skips it; 1 applies the plain rule
+// above to its samples; 2 keeps the first maximum of v over the window's
+// samples inside the padded chunk, when it lies in this segment, exceeds
+// the threshold and passes the +-D stencil (one event per window).  Every
+// kept sample passes the stencil, so kept samples are more than D apart and
+// the slot capacity of the plain scan holds.
+__device__ __forceinline__ bool stencil_ok(const float* xp, int64_t t, double v, double mean,
+                                           int sign, int D, int64_t wa, int64_t wb, int64_t M,
+                                           int64_t N) {
+    for (int d = 1; d <= D; ++d) {
+        if (t - d < wa) break;
+        if (detect_value(load_x(xp, t - d, M, N), mean, sign) >= v) return false;
+    }
+    for (int d = 1; d <= D; ++d) {
+        if (t + d >= wb) break;
+        if (detect_value(load_x(xp, t + d, M, N), mean, sign) > v) return false;
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256) detect_gated_scan_kernel(
+    const float* __restrict__ X, DetectPlan p, int sign, const double* __restrict__ mean_in,
+    const double* __restrict__ thr_in, const uint8_t* __restrict__ cls,
+    int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= p.nseg * p.Mc) return;
+    const int j = static_cast<int>(g % p.Mc);
+    const int64_t seg = g / p.Mc;
+    const int64_t ck = seg / p.spc;
+    const int64_t s = seg % p.spc;
+    int64_t size = p.n_range - ck * p.cs;
+    if (size > p.cs) size = p.cs;
+    const int64_t r0 = s * kSeg;
+    int64_t r1 = r0 + kSeg;
+    if (r1 > size) r1 = size;
+    int32_t cnt = 0;
+    if (r0 < r1) {
+        const double mean = mean_in[ck * p.Mc + j];
+        const double thr = thr_in[ck * p.Mc + j];
+        const int64_t core0 = p.t_start + ck * p.cs;
+        const int64_t wa = core0 - p.pad;
+        const int64_t wb = core0 + size + p.pad;
+        const float* xp = X + j;
+        uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
+        const int64_t T0 = core0 + r0, T1 = core0 + r1;
+        constexpr int64_t W = tcn::kWin;
+        for (int64_t w = T0 / W; w * W < T1; ++w) {
+            const int c = cls[w * p.M + j];
+            if (c == 0) continue;
+            const int64_t lo = w * W > T0 ? w * W : T0;
+            const int64_t hi = w * W + W < T1 ? w * W + W : T1;
+            if (c == 1) {
+                for (int64_t t = lo; t < hi; ++t) {
+                    const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
+                    if (!(v > thr)) continue;
+                    if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) &&
+                        cnt < p.cap)
+                        my_slots[cnt++] = static_cast<uint16_t>(t - T0);
+                }
+            } else {
+                const int64_t a = w * W > wa ? w * W : wa;
+                const int64_t b = w * W + W < wb ? w * W + W : wb;
+                int64_t u = a;
+                double bv = detect_value(load_x(xp, a, p.M, p.N_file), mean, sign);
+                for (int64_t t = a + 1; t < b; ++t) {
+                    const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
+                    if (v > bv) {
+                        bv = v;
+                        u = t;
+                    }
+                }
+                if (u >= lo && u < hi && bv > thr &&
+                    stencil_ok(xp, u, bv, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
+                    my_slots[cnt++] = static_cast<uint16_t>(u - T0);
+            }
+        }
+    }
+    counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
+}
+
+// Class histogram over the windows that hold samples of [t_start, t_end].
+__global__ void __launch_bounds__(256) detect_class_count_kernel(
+    const uint8_t* __restrict__ cls, int64_t w_begin, int64_t w_end, int64_t M,
+    unsigned long long* __restrict__ out) {
+    __shared__ unsigned int h[3];
+    if (threadIdx.x < 3) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t n = (w_end - w_begin) * M;
+    unsigned int loc[3] = {0, 0, 0};
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int c = cls[w_begin * M + q];
+        loc[c < 3 ? c : 2]++;
+    }
+    for (int k = 0; k < 3; ++k) atomicAdd(&h[k], loc[k]);
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&out[threadIdx.x], static_cast<unsigned long long>(h[threadIdx.x]));
+}
+
+__global__ void iota_kernel(int32_t* __restrict__ a, int32_t n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
 // One block per channel: exclusive scan of the per-segment counts.
 __global__ void __launch_bounds__(1024) detect_seg_scan_kernel(
     const int32_t* __restrict__ counts, int64_t nseg, int64_t* __restrict__ seg_off,
@@ -427,5 +534,122 @@ extern "C" int msamd_detect_fetch(const void* workspace, int32_t num_channels, i
             p, b.counts, b.seg_off, b.chan_off, b.slots, times);
         MSAMD_LAUNCH_CHECK();
     }
+    return MSAMD_OK;
+}
+
+// ------------------------------------------------------- learned detector
+// Workspace: the plain detection buffers (same offsets, so msamd_detect_fetch
+// reads the result), then the channel list, the window classes [nW][M], the
+// network fragments and the class histogram.
+namespace {
+struct LearnedBuffers {
+    int32_t* channels;
+    uint8_t* classes;
+    void* frag;
+    unsigned long long* hist;
+};
+
+inline LearnedBuffers carve_learned(void* ws, const DetectPlan& p, int64_t* bytes) {
+    int64_t prefix = 0;
+    carve(nullptr, p, &prefix);
+    Carver c(ws ? static_cast<char*>(ws) + prefix : nullptr);
+    LearnedBuffers b{};
+    b.channels = c.take<int32_t>(p.M);
+    b.classes = c.take<uint8_t>(tcn::num_windows(p.N_file) * p.M);
+    b.frag = c.take<char>(tcn::frag_bytes());
+    b.hist = c.take<unsigned long long>(3);
+    if (bytes) *bytes = prefix + c.bytes();
+    return b;
+}
+}  // namespace
+
+extern "C" int64_t msamd_tcn_detect_workspace_size(int64_t N_file, int64_t n_range,
+                                                   int64_t chunk_size, int32_t M,
+                                                   double detect_interval) {
+    if (N_file < 0 || n_range < 0 || chunk_size < 1 || M < 1) return -1;
+    const DetectPlan p = make_plan(M, N_file, 0, n_range, chunk_size, 0, M, detect_interval);
+    int64_t bytes = 0;
+    carve_learned(nullptr, p, &bytes);
+    return bytes;
+}
+
+extern "C" int msamd_tcn_detect(const float* X, int64_t M, int64_t N_file, int64_t t1,
+                                int64_t t2, int64_t chunk_size, int64_t padding,
+                                const float* params, double detect_threshold,
+                                double detect_interval, int detect_sign, double score_lo,
+                                double score_hi, void* workspace, int64_t workspace_bytes,
+                                int64_t* num_events_host, int64_t* class_counts_host,
+                                void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && M <= INT32_MAX && N_file >= 0, "tcn_detect: bad shape");
+    MSAMD_REQUIRE(chunk_size >= 1 && padding >= 0, "tcn_detect: bad chunk plan");
+    MSAMD_REQUIRE(params && num_events_host, "tcn_detect: null argument");
+    MSAMD_REQUIRE(score_lo > 0.0 && score_hi < 1.0 && score_lo <= score_hi,
+                  "tcn_detect: score band must satisfy 0 < lo <= hi < 1");
+    const int64_t t_start = t1 < 0 ? 0 : t1;
+    const int64_t t_end = t2 < 0 ? N_file - 1 : t2;
+    const int64_t n_range = t_end - t_start + 1;
+    MSAMD_REQUIRE(n_range >= 0 && t_end < N_file + (n_range == 0 ? 1 : 0),
+                  "tcn_detect: time range outside the recording");
+    const int32_t Mc = static_cast<int32_t>(M);
+    const DetectPlan p = make_plan(M, N_file, t_start, n_range, chunk_size, padding, Mc,
+                                   detect_interval);
+    int64_t need = 0;
+    DetectBuffers b = carve(workspace, p, nullptr);
+    LearnedBuffers lb = carve_learned(workspace, p, &need);
+    if (!workspace || workspace_bytes < need) {
+        set_error("tcn_detect: workspace too small (%lld < %lld bytes)",
+                  (long long)workspace_bytes, (long long)need);
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    MSAMD_HIP(hipMemcpyAsync(b.header, &p, sizeof(DetectPlan), hipMemcpyHostToDevice, st));
+    if (n_range == 0) {
+        MSAMD_HIP(hipMemsetAsync(b.chan_off, 0, sizeof(int64_t) * (Mc + 1), st));
+        MSAMD_HIP(hipStreamSynchronize(st));
+        *num_events_host = 0;
+        if (class_counts_host) class_counts_host[0] = class_counts_host[1] = class_counts_host[2] = 0;
+        return MSAMD_OK;
+    }
+    iota_kernel<<<static_cast<unsigned>(ceil_div(M, 256)), 256, 0, st>>>(lb.channels, Mc);
+    MSAMD_LAUNCH_CHECK();
+    {
+        const int64_t n = p.nchunks * p.Mc;
+        detect_stats_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            X, p, lb.channels, detect_threshold, b.mean, b.thr);
+        MSAMD_LAUNCH_CHECK();
+    }
+    const float lo = static_cast<float>(log(score_lo / (1.0 - score_lo)));
+    const float hi = static_cast<float>(log(score_hi / (1.0 - score_hi)));
+    int rc = tcn::prepare(params, lb.frag, st);
+    if (rc != MSAMD_OK) return rc;
+    rc = tcn::classify(X, M, N_file, lb.frag, lo, hi, nullptr, lb.classes, st);
+    if (rc != MSAMD_OK) return rc;
+    {
+        const int64_t n = p.nseg * p.Mc;
+        detect_gated_scan_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            X, p, detect_sign, b.mean, b.thr, lb.classes, b.counts, b.slots);
+        MSAMD_LAUNCH_CHECK();
+    }
+    detect_seg_scan_kernel<<<dim3(p.Mc), 1024, 0, st>>>(b.counts, p.nseg, b.seg_off,
+                                                         b.chan_off);
+    MSAMD_LAUNCH_CHECK();
+    detect_chan_scan_kernel<<<1, 64, 0, st>>>(b.chan_off, p.Mc);
+    MSAMD_LAUNCH_CHECK();
+    unsigned long long hist[3] = {0, 0, 0};
+    if (class_counts_host) {
+        MSAMD_HIP(hipMemsetAsync(lb.hist, 0, sizeof(unsigned long long) * 3, st));
+        const int64_t wb0 = t_start / tcn::kWin, wb1 = t_end / tcn::kWin + 1;
+        detect_class_count_kernel<<<1024, 256, 0, st>>>(lb.classes, wb0, wb1, M, lb.hist);
+        MSAMD_LAUNCH_CHECK();
+        MSAMD_HIP(hipMemcpyAsync(hist, lb.hist, sizeof(hist), hipMemcpyDeviceToHost, st));
+    }
+    int64_t total = 0;
+    MSAMD_HIP(hipMemcpyAsync(&total, b.chan_off + p.Mc, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             st));
+    MSAMD_HIP(hipStreamSynchronize(st));
+    *num_events_host = total;
+    if (class_counts_host)
+        for (int k = 0; k < 3; ++k) class_counts_host[k] = static_cast<int64_t>(hist[k]);
     return MSAMD_OK;
 }
