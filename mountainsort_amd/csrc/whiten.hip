@@ -47,9 +47,21 @@ __device__ __forceinline__ f32x4 mfma3(const bf16x8& ah, const bf16x8& al, const
 constexpr int kCovTPW = 16;  // output tiles per wavefront
 constexpr int kCovWaves = 4;
 
+// Streaming form for nB <= 24 (M <= 384): G workgroups of 8 waves share a
+// time span and hold the output tiles of the upper triangle between them
+// (up to 20 per wave); each streams the span once, staging the fragments of
+// all channel blocks per 32 samples in a double-buffered LDS area while the
+// next 32 samples load.  The G workgroups of one span run on the same XCD,
+// so the span is read from HBM about once.
+constexpr int kCovAllWaves = 8;
+constexpr int kCovAllTPW = 20;
+constexpr int kCovAllKSteps = 600;   // 32-sample steps per workgroup (FP32 partial length)
+constexpr int kCovAllItems = 3;      // (channel, 8-sample group) items per thread (4*Mp <= 1536)
+
 struct CovPlan {
     int64_t M, N, span, nsplit;
     int32_t Mp, nB, nTiles, G;
+    int32_t all;  // single-pass kernel
 };
 
 inline CovPlan cov_plan(int64_t M, int64_t N) {
@@ -59,9 +71,21 @@ inline CovPlan cov_plan(int64_t M, int64_t N) {
     p.Mp = static_cast<int32_t>(align_up(M, 16));
     p.nB = p.Mp / 16;
     p.nTiles = p.nB * (p.nB + 1) / 2;
+    const int64_t ksteps = ceil_div(N, 32);
+    p.all = p.nB <= 24 && 4 * p.Mp <= kCovAllItems * 64 * kCovAllWaves;
+    if (p.all) {
+        p.G = static_cast<int32_t>(ceil_div(p.nTiles, kCovAllWaves * kCovAllTPW));
+        int64_t nsplit = ceil_div(ksteps, kCovAllKSteps);
+        if (nsplit < 256) nsplit = 256;
+        if (nsplit > ksteps) nsplit = ksteps;
+        if (nsplit < 1) nsplit = 1;
+        p.span = align_up(ceil_div(N, nsplit), 32);
+        p.nsplit = ceil_div(N, p.span);
+        if (p.nsplit < 1) p.nsplit = 1;
+        return p;
+    }
     p.G = static_cast<int32_t>(ceil_div(p.nTiles, kCovTPW * kCovWaves));
     int64_t nsplit = ceil_div(2048, p.G);
-    const int64_t ksteps = ceil_div(N, 32);
     if (nsplit > ksteps) nsplit = ksteps;
     if (nsplit < 1) nsplit = 1;
     p.span = align_up(ceil_div(N, nsplit), 32);
@@ -159,6 +183,100 @@ __global__ void __launch_bounds__(256) cov_kernel(const float* __restrict__ X, C
     }
 }
 
+template <int TPW>
+__global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float* __restrict__ X,
+                                                                   CovPlan p,
+                                                                   float* __restrict__ part) {
+    extern __shared__ bf16x8 lds[];  // [buffer][hi, lo][nB][64]
+    // block b runs on XCD b % 8; the G tile groups of a span share that XCD
+    const int64_t r = blockIdx.x / 8;
+    const int g = static_cast<int>(r % p.G);
+    const int64_t split = (r / p.G) * 8 + blockIdx.x % 8;
+    if (split >= p.nsplit) return;
+    const int64_t tb = split * p.span;
+    int64_t te = tb + p.span;
+    if (te > p.N) te = p.N;
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int tile0 = (g * kCovAllWaves + wid) * TPW;
+    int I0 = 0, J0 = 0;
+    if (tile0 < p.nTiles) tile_ij(tile0, p.nB, I0, J0);
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int nitems = 4 * p.Mp;
+    const int plane = p.nB * 64;
+    float v[kCovAllItems][8];
+    auto load = [&](int64_t t0) {
+#pragma unroll
+        for (int it = 0; it < kCovAllItems; ++it) {
+            const int q = threadIdx.x + it * kCovAllWaves * 64;
+            const int m = q % p.Mp;
+            const int b = q / p.Mp;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t t = t0 + 8 * b + e;
+                v[it][e] = (q < nitems && m < p.M && t < te)
+                               ? __builtin_nontemporal_load(X + t * p.M + m)
+                               : 0.0f;
+            }
+        }
+    };
+    load(tb);
+    int buf = 0;
+    for (int64_t t0 = tb; t0 < te; t0 += 32, buf ^= 1) {
+        bf16x8* fh = lds + buf * 2 * plane;
+        bf16x8* fl = fh + plane;
+#pragma unroll
+        for (int it = 0; it < kCovAllItems; ++it) {
+            const int q = threadIdx.x + it * kCovAllWaves * 64;
+            if (q < nitems) {
+                const int m = q % p.Mp;
+                const int b = q / p.Mp;
+                bf16x8 h, l;
+                split8(v[it], h, l);
+                const int idx = (m >> 4) * 64 + (m & 15) + 16 * b;
+                fh[idx] = h;
+                fl[idx] = l;
+            }
+        }
+        __syncthreads();
+        if (t0 + 32 < te) load(t0 + 32);  // next step's samples load during the MFMAs
+        int I = I0, J = J0;
+        bool needA = true;
+        bf16x8 ah = {}, al = {};
+#pragma unroll
+        for (int k = 0; k < TPW; ++k) {
+            if (tile0 + k < p.nTiles) {
+                if (needA) {
+                    ah = fh[I * 64 + lane];
+                    al = fl[I * 64 + lane];
+                    needA = false;
+                }
+                const bf16x8 bh = fh[J * 64 + lane];
+                const bf16x8 bl = fl[J * 64 + lane];
+                acc[k] = mfma3(ah, al, bh, bl, acc[k]);
+            }
+            if (++J == p.nB) {
+                ++I;
+                J = I;
+                needA = true;
+            }
+        }
+        // the other buffer is written next; the barrier after that write keeps
+        // every wave's reads of this buffer ahead of its next overwrite
+    }
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) {
+        if (tile0 + k < p.nTiles) {
+            float* dst = part + (split * p.nTiles + tile0 + k) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[k][r];
+        }
+    }
+}
+
 // FP64 sum of the block partials in split order, / (N-1), mirrored.
 __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict__ part,
                                                          CovPlan p, double* __restrict__ C) {
@@ -192,10 +310,17 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
 // rows in the rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal
 // to the reference's two M^3 products in exact arithmetic.
 //
-// The kernel is launched cooperatively (all workgroups co-resident).  Each
-// barrier wait is bounded; on timeout the error flag is raised and every
-// workgroup leaves, so the kernel always terminates.
+// The grid has kWXcd * NP workgroups of which only those with
+// blockIdx % kWXcd == 0 work: workgroups are dispatched to the XCDs in
+// round-robin order, so the NP working groups share one XCD and its L2.
+// The launch is an ordinary one (a cooperative launch is refused while
+// another process on the GPU holds the cooperative queue); the NP groups
+// are few enough to be co-resident on an otherwise idle device, and every
+// barrier wait is bounded: on timeout the error flag is raised, every
+// workgroup leaves and the call returns an error, so the kernel always
+// terminates.
 constexpr int kWThreads = 256;
+constexpr int kWXcd = 8;
 constexpr unsigned kBarrierSpinLimit = 1u << 22;
 
 struct WPlan {
@@ -279,7 +404,8 @@ __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
     double* __restrict__ comp, double* __restrict__ sig, WBufs bufs) {
     extern __shared__ double sh[];
     const int M = p.M;
-    const int wg = blockIdx.x;
+    if (blockIdx.x % kWXcd != 0) return;
+    const int wg = blockIdx.x / kWXcd;
     const int r0 = wg * p.RW;
     const int nrows = (M - r0) < p.RW ? (M - r0) : p.RW;
     double* Gl = sh;                                    // [RW][M]
@@ -595,13 +721,31 @@ extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, dou
                   (long long)workspace_bytes, (long long)c.bytes());
         return MSAMD_ERR_WORKSPACE;
     }
-    const size_t lds = sizeof(bf16x8) * 2 * p.nB * 64;
-    const unsigned grid = static_cast<unsigned>(p.nsplit * p.G);
-    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&cov_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)));
-    cov_kernel<<<grid, 256, lds, st>>>(X, p, part);
-    MSAMD_LAUNCH_CHECK();
+    if (p.all) {
+        const size_t lds = sizeof(bf16x8) * 4 * p.nB * 64;
+        const unsigned grid = static_cast<unsigned>(ceil_div(p.nsplit, 8) * 8 * p.G);
+        const int tpw = static_cast<int>(ceil_div(p.nTiles, kCovAllWaves * p.G));
+#define MSAMD_COV_ALL(T)                                                                     \
+    do {                                                                                     \
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&cov_all_kernel<T>),      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,            \
+                                      static_cast<int>(lds)));                               \
+        cov_all_kernel<T><<<grid, kCovAllWaves * 64, lds, st>>>(X, p, part);                  \
+    } while (0)
+        if (tpw <= 5) MSAMD_COV_ALL(5);
+        else if (tpw <= 10) MSAMD_COV_ALL(10);
+        else MSAMD_COV_ALL(kCovAllTPW);
+#undef MSAMD_COV_ALL
+        MSAMD_LAUNCH_CHECK();
+    } else {
+        const size_t lds = sizeof(bf16x8) * 2 * p.nB * 64;
+        const unsigned grid = static_cast<unsigned>(p.nsplit * p.G);
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&cov_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds)));
+        cov_kernel<<<grid, 256, lds, st>>>(X, p, part);
+        MSAMD_LAUNCH_CHECK();
+    }
     const int64_t nred = static_cast<int64_t>(p.nTiles) * 256;
     cov_reduce_kernel<<<static_cast<unsigned>(ceil_div(nred, 256)), 256, 0, st>>>(part, p, XXt);
     MSAMD_LAUNCH_CHECK();
@@ -670,12 +814,9 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     const size_t lds = sizeof(double) * (static_cast<size_t>(wp.RW) * M + 2 * M + wp.RW + 2);
     MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    {
-        const double* xx = XXt;
-        void* args[] = {const_cast<double**>(&xx), const_cast<WPlan*>(&wp), &v0, &comp, &sig, &wb};
-        MSAMD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&whitening_power_kernel),
-                                             dim3(wp.NP), dim3(kWThreads), args, lds, st));
-    }
+    whitening_power_kernel<<<static_cast<unsigned>(wp.NP * kWXcd), kWThreads, lds, st>>>(
+        XXt, wp, v0, comp, sig, wb);
+    MSAMD_LAUNCH_CHECK();
     int herr = 0;
     MSAMD_HIP(hipMemcpyAsync(&herr, flags + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     MSAMD_HIP(hipStreamSynchronize(st));

@@ -1,10 +1,16 @@
-"""Summaries of rocprofv3 kernel statistics for this library's kernels.
+"""Summaries of rocprofv3 measurements of this library's kernels.
 
-``python -m mountainsort_amd.profiling <kernel_stats.csv> <title> [command]``
+``python -m mountainsort_amd.profiling <kernel_stats.csv> <title> [command] [passes]``
 prints a markdown table of the msamd kernels in a rocprofv3
 ``--stats --output-format csv`` kernel_stats.csv (other kernels, e.g. torch's
 synthetic-data generation outside the timed region, are summed into one
 line).
+
+``python -m mountainsort_amd.profiling traffic FETCH.db WRITE.db KERNEL COPY_BYTES
+PREFIX "fetch command" "write command"`` turns two ``rocprofv3 --pmc
+FETCH_SIZE`` / ``--pmc WRITE_SIZE`` results databases into the HBM bytes per
+launch of KERNEL, each counter scaled by what it reads for a copy kernel of
+known size run in the same program (PREFIX.md and PREFIX.json).
 """
 import csv
 import re
@@ -38,7 +44,79 @@ def summarize(path, title, command=None, per=1):
     return "\n".join(out) + "\n"
 
 
+
+
+# --------------------------------------------------------------------------
+# PMC traffic: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (rocpd databases)
+# --------------------------------------------------------------------------
+def pmc_counters(db_path):
+    """{kernel: {counter: [values per dispatch]}, ...} and {kernel: [durations us]}
+    from a rocprofv3 results database."""
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    cur = con.cursor()
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for name, cn, cv in cur.execute(
+            "select name, counter_name, counter_value from pmc_events"):
+        agg[clean(name)][cn].append(float(cv))
+    dur = collections.defaultdict(list)
+    for name, d in cur.execute("select name, duration from kernels"):
+        dur[clean(name)].append(d / 1000.0)
+    return agg, dur
+
+
+def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, command_write):
+    """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by the factor that makes the same counter read the
+    known bytes of the calibration copy (the kernel whose name contains
+    'copy', reading and writing copy_bytes per launch).  Returns (dict, text)."""
+    fa, fdur = pmc_counters(fetch_db)
+    wa, _ = pmc_counters(write_db)
+
+    def avg(agg, pred, counter):
+        vals = [v for k, c in agg.items() if pred(k) for v in c.get(counter, [])]
+        return sum(vals) / len(vals) if vals else None
+
+    is_k = lambda k: k.endswith(kernel) or k.split("<")[0].endswith(kernel)
+    is_copy = lambda k: "copy" in k.lower() and "msamd" not in k
+    f_k, w_k = avg(fa, is_k, "FETCH_SIZE"), avg(wa, is_k, "WRITE_SIZE")
+    f_c, w_c = avg(fa, is_copy, "FETCH_SIZE"), avg(wa, is_copy, "WRITE_SIZE")
+    f_scale = copy_bytes / (f_c * 1024.0) if f_c else None
+    w_scale = copy_bytes / (w_c * 1024.0) if w_c else None
+    durs = [d for k, v in fdur.items() if is_k(k) for d in v]
+    out = {"kernel": kernel,
+           "fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k,
+           "copy_bytes": copy_bytes, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
+           "fetch_scale": f_scale, "write_scale": w_scale,
+           "avg_duration_us_fetch_run": sum(durs) / len(durs) if durs else None}
+    if None not in (f_k, w_k, f_scale, w_scale):
+        out["read_bytes"] = f_k * 1024.0 * f_scale
+        out["write_bytes"] = w_k * 1024.0 * w_scale
+        out["traffic_bytes"] = out["read_bytes"] + out["write_bytes"]
+    lines = [f"# HBM traffic of {kernel}", "", "Commands:", "", "```", command_fetch,
+             command_write, "```", "",
+             "| quantity | value |", "|---|---:|"]
+    for k, v in out.items():
+        lines.append(f"| {k} | {v:.6g} |" if isinstance(v, float) else f"| {k} | {v} |")
+    return out, "\n".join(lines) + "\n"
+
+
+def main_traffic(argv):
+    """python -m mountainsort_amd.profiling traffic FETCH.db WRITE.db KERNEL COPY_BYTES
+    OUT_PREFIX "fetch command" "write command" -> OUT_PREFIX.md and OUT_PREFIX.json"""
+    import json
+    fetch_db, write_db, kernel, copy_bytes, prefix, cf, cw = argv
+    out, text = traffic_report(fetch_db, write_db, kernel, float(copy_bytes), cf, cw)
+    open(prefix + ".md", "w").write(text)
+    json.dump(out, open(prefix + ".json", "w"), indent=1)
+    sys.stdout.write(text)
+
+
 if __name__ == "__main__":
-    cmd = sys.argv[3] if len(sys.argv) > 3 else None
-    per = int(sys.argv[4]) if len(sys.argv) > 4 else 1
-    sys.stdout.write(summarize(sys.argv[1], sys.argv[2], cmd, per))
+    if len(sys.argv) > 1 and sys.argv[1] == "traffic":
+        main_traffic(sys.argv[2:])
+    else:
+        cmd = sys.argv[3] if len(sys.argv) > 3 else None
+        per = int(sys.argv[4]) if len(sys.argv) > 4 else 1
+        sys.stdout.write(summarize(sys.argv[1], sys.argv[2], cmd, per))
