@@ -24,7 +24,7 @@ namespace msamd {
 namespace {
 
 constexpr int kSeg = 1024;   // samples per (segment, channel) scan (fits a uint16 offset)
-constexpr int kB = 16;        // register batch; neighbours within +-kB come from registers
+constexpr int kB = 16;        // register batch of the stencil scan
 
 struct DetectPlan {
     int64_t M, N_file, t_start, n_range, cs, pad, nchunks, spc, nseg;
@@ -135,23 +135,36 @@ __device__ __forceinline__ double detect_value(float x, double mean, int sign) {
 
 // One thread per (segment, central channel), lanes over consecutive channels
 // so every load is one coalesced row.  The thread streams its segment in
-// batches of kB samples, loading two batches ahead.  The previous, current
-// and next batches also sit in a per-thread LDS ring (stride 49 floats, no
-// bank conflicts), from which the rare threshold crossings read their +-D
-// neighbours; lags beyond kB (detect_interval > 16) fall back to global
-// loads.
+// register batches of kB samples, loading the next batch while it tests the
+// current one.  Threshold crossings are rare; their +-D neighbours are read
+// again through the cache (the rows were just loaded by the wavefront).
 __device__ __forceinline__ float load_x(const float* xp, int64_t t, int64_t M, int64_t N) {
     return (t >= 0 && t < N) ? xp[t * M] : 0.0f;
 }
 
-constexpr int kRing = 3 * kB + 1;
+// The +-D suppression test of sample t (windows cut at [wa, wb)): v must
+// beat every earlier value and be >= every later value within D.  The
+// neighbours were just streamed by the wavefront and come from the cache;
+// walking them in order keeps the kernel small enough for high occupancy,
+// which matters more here than the latency of the rare candidate.
+__device__ __forceinline__ bool stencil_ok(const float* xp, int64_t t, double v, double mean,
+                                           int sign, int D, int64_t wa, int64_t wb, int64_t M,
+                                           int64_t N) {
+    for (int d = 1; d <= D; ++d) {
+        if (t - d < wa) break;
+        if (detect_value(load_x(xp, t - d, M, N), mean, sign) >= v) return false;
+    }
+    for (int d = 1; d <= D; ++d) {
+        if (t + d >= wb) break;
+        if (detect_value(load_x(xp, t + d, M, N), mean, sign) > v) return false;
+    }
+    return true;
+}
 
 __global__ void __launch_bounds__(256) detect_scan_kernel(
     const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels, int sign,
     const double* __restrict__ mean_in, const double* __restrict__ thr_in,
     int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
-    __shared__ float ring_all[256 * kRing];
-    float* ring = ring_all + threadIdx.x * kRing;
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= p.nseg * p.Mc) return;
     const int j = static_cast<int>(g % p.Mc);
@@ -173,68 +186,27 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
         const float* xp = X + channels[j];
         uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
         const int64_t T0 = core0 + r0;
-        // ring slot of batch q (q = -1, 0, 1, ...) is ((q + 3) % 3) * kB
-        {
-            float a[kB], b[kB], c[kB];
+        const int64_t T1 = core0 + r1;
+        float cur[kB], nxt[kB];
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                a[i] = load_x(xp, T0 - kB + i, p.M, p.N_file);
-                b[i] = load_x(xp, T0 + i, p.M, p.N_file);
-                c[i] = load_x(xp, T0 + kB + i, p.M, p.N_file);
-            }
+        for (int i = 0; i < kB; ++i) cur[i] = load_x(xp, T0 + i, p.M, p.N_file);
+        for (int64_t tb = T0; tb < T1; tb += kB) {
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                ring[2 * kB + i] = a[i];
-                ring[i] = b[i];
-                ring[kB + i] = c[i];
-            }
-        }
-        const int D = p.D;
-        int q = 0;
-        for (int64_t rb = r0; rb < r1; rb += kB, ++q) {
-            const int64_t tb = core0 + rb;  // absolute time of cur[0]
-            float nn[kB];
+            for (int i = 0; i < kB; ++i) nxt[i] = load_x(xp, tb + kB + i, p.M, p.N_file);
+            uint32_t over = 0;  // threshold crossings of this batch
 #pragma unroll
-            for (int i = 0; i < kB; ++i) nn[i] = load_x(xp, tb + 2 * kB + i, p.M, p.N_file);
-            const int base_slot = q % 3;  // slot of the current batch
-            const float* curp = ring + base_slot * kB;
-            for (int i = 0; i < kB; ++i) {
-                if (rb + i >= r1) break;
-                const double v = detect_value(curp[i], mean, sign);
-                if (!(v > thr)) continue;
+            for (int i = 0; i < kB; ++i)
+                if (tb + i < T1 && detect_value(cur[i], mean, sign) > thr) over |= 1u << i;
+            while (over) {  // in time order; kept events are > D apart, so cnt < cap
+                const int i = __builtin_ctz(over);
+                over &= over - 1;
                 const int64_t t = tb + i;
-                bool keep = true;
-                for (int d = 1; keep && d <= D; ++d) {  // left: v must beat every earlier value
-                    if (t - d < wa) break;
-                    const int k = i - d;
-                    float xu;
-                    if (k >= -kB) {
-                        const int bq = k < 0 ? 2 : 0;  // batch offset -1 -> +2 (mod 3)
-                        xu = ring[((base_slot + bq) % 3) * kB + (k < 0 ? k + kB : k)];
-                    } else {
-                        xu = load_x(xp, t - d, p.M, p.N_file);
-                    }
-                    if (detect_value(xu, mean, sign) >= v) keep = false;
-                }
-                for (int d = 1; keep && d <= D; ++d) {  // right: v must be >= every later value
-                    if (t + d >= wb) break;
-                    const int k = i + d;
-                    float xu;
-                    if (k < 2 * kB) {
-                        const int bq = k >= kB ? 1 : 0;
-                        xu = ring[((base_slot + bq) % 3) * kB + (k >= kB ? k - kB : k)];
-                    } else {
-                        xu = load_x(xp, t + d, p.M, p.N_file);
-                    }
-                    if (detect_value(xu, mean, sign) > v) keep = false;
-                }
-                // kept events are > D apart, so cnt < cap always holds
-                if (keep && cnt < p.cap) my_slots[cnt++] = static_cast<uint16_t>(rb + i - r0);
+                const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
+                if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
+                    my_slots[cnt++] = static_cast<uint16_t>(t - T0);
             }
-            // batch q+2 replaces batch q-1 in the ring
-            const int ns = ((q + 2) % 3) * kB;
 #pragma unroll
-            for (int i = 0; i < kB; ++i) ring[ns + i] = nn[i];
+            for (int i = 0; i < kB; ++i) cur[i] = nxt[i];
         }
     }
     counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
@@ -248,20 +220,6 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
 // the threshold and passes the +-D stencil (one event per window).  Every
 // kept sample passes the stencil, so kept samples are more than D apart and
 // the slot capacity of the plain scan holds.
-__device__ __forceinline__ bool stencil_ok(const float* xp, int64_t t, double v, double mean,
-                                           int sign, int D, int64_t wa, int64_t wb, int64_t M,
-                                           int64_t N) {
-    for (int d = 1; d <= D; ++d) {
-        if (t - d < wa) break;
-        if (detect_value(load_x(xp, t - d, M, N), mean, sign) >= v) return false;
-    }
-    for (int d = 1; d <= D; ++d) {
-        if (t + d >= wb) break;
-        if (detect_value(load_x(xp, t + d, M, N), mean, sign) > v) return false;
-    }
-    return true;
-}
-
 __global__ void __launch_bounds__(256) detect_gated_scan_kernel(
     const float* __restrict__ X, DetectPlan p, int sign, const double* __restrict__ mean_in,
     const double* __restrict__ thr_in, const uint8_t* __restrict__ cls,
