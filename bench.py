@@ -322,8 +322,9 @@ def main():
     ap.add_argument("--no-pca", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS),
-                    help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB copy and "
-                         "the named stage three times each")
+                    help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
+                         "elementwise negation (counter calibration) and the named stage, "
+                         "three times each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is "
@@ -348,8 +349,8 @@ def main():
         copy_bytes = 1 << 30
         src = torch.ones(copy_bytes // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        for _ in range(3):  # calibration: reads and writes copy_bytes (torch's neg kernel)
+            torch.neg(src, out=dst)
         for _ in range(3):
             if args.pmc_pass == "apply":
                 fe.whiten_apply(pipe.masked, pipe.W, out=pipe.white)

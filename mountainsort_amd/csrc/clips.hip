@@ -37,7 +37,10 @@ __global__ void __launch_bounds__(256) clips_kernel(
 // Batched form: events of neighborhood g are times[eo[g]..eo[g+1]) (centers),
 // its channels nc[no[g]..no[g+1]), its clips start at clips + co[g].
 // blockIdx.y selects the neighborhood; the blocks along x stride over its
-// events, one wavefront per clip.
+// events, one wavefront per clip.  For W <= 64 each lane owns one channel
+// (w = lane % W) and a row offset (lane / W), so the channel index is loaded
+// once and the W x T clip is covered by rows of 64 / W time samples; the
+// loads of several rows are issued before their stores.
 __global__ void __launch_bounds__(256) clips_batched_kernel(
     const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ times,
     const int64_t* __restrict__ eo, const int32_t* __restrict__ nc,
@@ -53,6 +56,34 @@ __global__ void __launch_bounds__(256) clips_batched_kernel(
     float* base = clips + co[g];
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    if (W <= 64) {
+        const int rpi = 64 / W;           // rows per pass
+        const int w = lane % W;
+        const int tl = lane / W;
+        const bool active = tl < rpi;
+        const int64_t chw = active ? ch[w] : 0;
+        for (int64_t i = wave; i < L; i += nwaves) {
+            const int64_t t0 = times[e0 + i] - tmid;
+            float* out = base + i * WT;
+            if (!active) continue;
+            int t = tl;
+            for (; t + 3 * rpi < T; t += 4 * rpi) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int64_t tt = t0 + t + k * rpi;
+                    v[k] = (tt >= 0 && tt < N) ? X[tt * M + chw] : 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) out[(t + k * rpi) * W + w] = v[k];
+            }
+            for (; t < T; t += rpi) {
+                const int64_t tt = t0 + t;
+                out[t * W + w] = (tt >= 0 && tt < N) ? X[tt * M + chw] : 0.0f;
+            }
+        }
+        return;
+    }
     for (int64_t i = wave; i < L; i += nwaves) {
         const int64_t t0 = times[e0 + i] - tmid;
         float* out = base + i * WT;

@@ -506,15 +506,15 @@ __global__ void __launch_bounds__(256) whitening_assemble_kernel(
 // hh + hm + mh + hl + lh + mm; the dropped terms are O(2^-24) relative.
 //
 // Each workgroup keeps the three-piece fragments of CT column tiles (16*CT
-// outputs) of W in LDS and streams 8 waves x 32 samples per pass over its
-// 1024-sample time block.  Workgroups are numbered so that all column
+// outputs) of W in LDS and streams 8 waves x 64 samples per pass over its
+// 2048-sample time block.  Workgroups are numbered so that all column
 // blocks of one time block run back to back on the same XCD (blockIdx % 8
 // selects the XCD), and the time block is read once from HBM into that
 // XCD's L2.
 constexpr int kApplyWaves = 8;
-constexpr int kApplyRowTiles = 2;    // 16-sample row tiles per wave per pass
+constexpr int kApplyRowTiles = 4;    // 16-sample row tiles per wave per pass
 constexpr int kApplyPasses = 4;      // passes per workgroup
-constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 1024 samples
+constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 2048 samples
 constexpr int kXcd = 8;
 
 struct ApplyPlan {

@@ -10,7 +10,8 @@ line).
 PREFIX "fetch command" "write command"`` turns two ``rocprofv3 --pmc
 FETCH_SIZE`` / ``--pmc WRITE_SIZE`` results databases into the HBM bytes per
 launch of KERNEL, each counter scaled by what it reads for a copy kernel of
-known size run in the same program (PREFIX.md and PREFIX.json).
+known size (an elementwise negation) run in the same program (PREFIX.md and
+PREFIX.json).
 """
 import csv
 import re
@@ -69,8 +70,9 @@ def pmc_counters(db_path):
 def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, command_write):
     """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by the factor that makes the same counter read the
-    known bytes of the calibration copy (the kernel whose name contains
-    'copy', reading and writing copy_bytes per launch).  Returns (dict, text)."""
+    known bytes of the calibration kernel (an elementwise negation, whose name
+    contains 'neg', reading and writing copy_bytes per launch).  Returns
+    (dict, text)."""
     fa, fdur = pmc_counters(fetch_db)
     wa, _ = pmc_counters(write_db)
 
@@ -78,10 +80,21 @@ def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, comman
         vals = [v for k, c in agg.items() if pred(k) for v in c.get(counter, [])]
         return sum(vals) / len(vals) if vals else None
 
+    def calib(agg, counter):
+        # the calibration kernel: an elementwise negation reading and writing
+        # copy_bytes, the only negation in the measured program
+        vals = [v for k, c in agg.items() if "neg" in k.lower() and "msamd" not in k
+                for v in c.get(counter, [])]
+        if not vals:
+            return None
+        calib.used[counter] = len(vals)
+        return sum(vals) / len(vals)
+
+    calib.used = {}
+
     is_k = lambda k: k.endswith(kernel) or k.split("<")[0].endswith(kernel)
-    is_copy = lambda k: "copy" in k.lower() and "msamd" not in k
     f_k, w_k = avg(fa, is_k, "FETCH_SIZE"), avg(wa, is_k, "WRITE_SIZE")
-    f_c, w_c = avg(fa, is_copy, "FETCH_SIZE"), avg(wa, is_copy, "WRITE_SIZE")
+    f_c, w_c = calib(fa, "FETCH_SIZE"), calib(wa, "WRITE_SIZE")
     f_scale = copy_bytes / (f_c * 1024.0) if f_c else None
     w_scale = copy_bytes / (w_c * 1024.0) if w_c else None
     durs = [d for k, v in fdur.items() if is_k(k) for d in v]
@@ -89,6 +102,7 @@ def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, comman
            "fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k,
            "copy_bytes": copy_bytes, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
            "fetch_scale": f_scale, "write_scale": w_scale,
+           "copy_dispatches_used": dict(calib.used),
            "avg_duration_us_fetch_run": sum(durs) / len(durs) if durs else None}
     if None not in (f_k, w_k, f_scale, w_scale):
         out["read_bytes"] = f_k * 1024.0 * f_scale
