@@ -105,20 +105,21 @@ __global__ void tcn_prep_kernel(const float* __restrict__ P, Frag* __restrict__ 
 }
 
 // ------------------------------------------------------------ inference
-// One workgroup = kCB central channels x kWt windows.  The x tile (the kCB+6
-// channel rows the centrals need, kXT samples, bf16) is staged once; the
-// centrals are then processed one after another: layer 1 (16 N-tiles of 16
-// positions, two per wave, both M-tiles per B fragment), layer 2 (4 N-tiles
-// x 2 M-tiles, one per wave), layer 3 (8 threads per window).
+// One workgroup (4 waves) = kCB central channels x kWt windows.  The x tile
+// (the kCB+6 channel rows the centrals need, kXT samples, bf16) is staged
+// once; the centrals are then processed one after another: layer 1 (8
+// N-tiles of 16 positions, two per wave, both M-tiles per B fragment),
+// layer 2 (2 N-tiles x 2 M-tiles, one per wave), layer 3 (8 threads per
+// window).  The tile is sized so that three workgroups share a CU.
 constexpr int kCB = 8;
-constexpr int kWt = 54;                     // windows per tile
-constexpr int kNP2 = 64;                    // h2 positions (>= kWt + 8)
-constexpr int kNP1 = 256;                   // h1 positions (>= 4 * (kWt + 7) + 9)
-constexpr int kXT = 8 * (kNP1 - 1) + 16;    // samples per x row (2056)
+constexpr int kWt = 22;                     // windows per tile
+constexpr int kNP2 = 32;                    // h2 positions (>= kWt + 8)
+constexpr int kNP1 = 128;                   // h1 positions (>= 4 * (kWt + 7) + 9)
+constexpr int kXT = 8 * (kNP1 - 1) + 16;    // samples per x row (1032)
 constexpr int kRows = kCB + kCI - 1;        // 14 channel rows
 constexpr int kH1S = 40;                    // h1 row stride (bf16): 80 bytes
 constexpr int kH2S = 33;                    // h2 row stride (float)
-constexpr int kCThreads = 512;
+constexpr int kCThreads = 256;
 constexpr size_t kXBytes = sizeof(__bf16) * kRows * kXT;
 constexpr size_t kH1Bytes = sizeof(__bf16) * kNP1 * kH1S;
 constexpr size_t kH2Bytes = sizeof(float) * kNP2 * kH2S;
@@ -152,7 +153,7 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int s = 0; s < 4; ++s) a1[mt][s] = F->a1[mt][s][lane];
-    const int nt2 = wv & 3, mt2 = wv >> 2;
+    const int nt2 = wv & 1, mt2 = wv >> 1;
     bf16x8 a2[9];
 #pragma unroll
     for (int s = 0; s < 9; ++s) a2[s] = F->a2[mt2][s][lane];
@@ -187,10 +188,10 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
     for (int cc = 0; cc < kCB; ++cc) {
         const int64_t c = c0 + cc;
         if (c >= M) break;
-        // ---- layer 1: N-tiles wv and wv + 8
+        // ---- layer 1: N-tiles wv and wv + 4
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int nt = wv + 8 * h;
+            const int nt = wv + 4 * h;
             const int jl = 16 * nt + (lane & 15);
             f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

@@ -127,9 +127,33 @@ def main_traffic(argv):
     sys.stdout.write(text)
 
 
+
+def main_all(argv):
+    """python -m mountainsort_amd.profiling pmc-all FETCH.db WRITE.db OUT.md TITLE
+    -> per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch, raw counter
+    values) and durations of every msamd kernel in the runs."""
+    fetch_db, write_db, out, title = argv
+    fa, fdur = pmc_counters(fetch_db)
+    wa, _ = pmc_counters(write_db)
+    lines = [f"# {title}", "", "| kernel | dispatches | avg us | FETCH_SIZE KiB | WRITE_SIZE KiB |",
+             "|---|---:|---:|---:|---:|"]
+    names = sorted((k for k in fdur if "msamd" in k), key=lambda k: -sum(fdur[k]))
+    for k in names:
+        f = fa.get(k, {}).get("FETCH_SIZE", [])
+        w = wa.get(k, {}).get("WRITE_SIZE", [])
+        d = fdur[k]
+        lines.append(f"| `{k}` | {len(d)} | {sum(d) / len(d):.2f} | "
+                     f"{(sum(f) / len(f)) if f else float('nan'):.6g} | "
+                     f"{(sum(w) / len(w)) if w else float('nan'):.6g} |")
+    open(out, "w").write("\n".join(lines) + "\n")
+    sys.stdout.write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "traffic":
         main_traffic(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "pmc-all":
+        main_all(sys.argv[2:])
     else:
         cmd = sys.argv[3] if len(sys.argv) > 3 else None
         per = int(sys.argv[4]) if len(sys.argv) > 4 else 1
