@@ -292,10 +292,10 @@ int64_t msamd_tcn_detect_workspace_size(int64_t N_file, int64_t n_range, int64_t
 
 /* Learned detection on every channel 0..M-1.  Statistics and chunk plan are
  * those of msamd_detect_events.  Each window is classified by its score
- * s = sigmoid(z): s < score_lo: no event; s > score_hi: the first maximum of
- * v over the window's samples, kept when it exceeds the threshold and passes
- * the detect_interval suppression stencil (one event per window); otherwise
- * the plain rule decides every sample.
+ * s = sigmoid(z): s < score_lo: skipped (no event); score_lo <= s <= score_hi:
+ * uncertain; s > score_hi: present.  In uncertain and present windows the
+ * plain rule decides every sample, so every reported event is an event of
+ * plain detection and only skipped windows can lose events.
  * Fetch the result with msamd_detect_fetch(workspace, M, ...).
  * class_counts_host (may be NULL): windows per class {skipped, uncertain,
  * present} over the detection range. */

@@ -213,13 +213,10 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
 }
 
 // Gated scan of the learned detector: one thread per (segment, channel), the
-// channel list being 0..M-1.  Every 32-sample window w overlapping the
-// segment is handled by its class: 0 skips it; 1 applies the plain rule
-// above to its samples; 2 keeps the first maximum of v over the window's
-// samples inside the padded chunk, when it lies in this segment, exceeds
-// the threshold and passes the +-D stencil (one event per window).  Every
-// kept sample passes the stencil, so kept samples are more than D apart and
-// the slot capacity of the plain scan holds.
+// channel list being 0..M-1.  Windows of class 0 (no spike) are skipped; in
+// windows proposed by the network (class 1, uncertain, and class 2, present)
+// the plain rule above decides every sample, so every kept sample is an event
+// of plain detection and the slot capacity of the plain scan holds.
 __global__ void __launch_bounds__(256) detect_gated_scan_kernel(
     const float* __restrict__ X, DetectPlan p, int sign, const double* __restrict__ mean_in,
     const double* __restrict__ thr_in, const uint8_t* __restrict__ cls,
@@ -251,29 +248,11 @@ __global__ void __launch_bounds__(256) detect_gated_scan_kernel(
             if (c == 0) continue;
             const int64_t lo = w * W > T0 ? w * W : T0;
             const int64_t hi = w * W + W < T1 ? w * W + W : T1;
-            if (c == 1) {
-                for (int64_t t = lo; t < hi; ++t) {
-                    const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
-                    if (!(v > thr)) continue;
-                    if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) &&
-                        cnt < p.cap)
-                        my_slots[cnt++] = static_cast<uint16_t>(t - T0);
-                }
-            } else {
-                const int64_t a = w * W > wa ? w * W : wa;
-                const int64_t b = w * W + W < wb ? w * W + W : wb;
-                int64_t u = a;
-                double bv = detect_value(load_x(xp, a, p.M, p.N_file), mean, sign);
-                for (int64_t t = a + 1; t < b; ++t) {
-                    const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
-                    if (v > bv) {
-                        bv = v;
-                        u = t;
-                    }
-                }
-                if (u >= lo && u < hi && bv > thr &&
-                    stencil_ok(xp, u, bv, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
-                    my_slots[cnt++] = static_cast<uint16_t>(u - T0);
+            for (int64_t t = lo; t < hi; ++t) {
+                const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
+                if (!(v > thr)) continue;
+                if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
+                    my_slots[cnt++] = static_cast<uint16_t>(t - T0);
             }
         }
     }

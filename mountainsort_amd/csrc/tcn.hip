@@ -3,7 +3,8 @@
 //
 // This component has no counterpart in the reference.  It proposes the
 // windows in which detection (detect_events.cpp:69-135) has to look; the
-// plain rule stays the judge on uncertain windows (detect.hip, gated scan).
+// plain rule decides every sample of a proposed window (detect.hip, gated
+// scan), so skipped windows are the only source of missed events.
 //
 // Network (shared by all channels; exact index conventions):
 //   x[ci][t] = X[t, c-3+ci]  ci = 0..6, zero outside the recording / [0, M)

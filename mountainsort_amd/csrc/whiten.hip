@@ -310,9 +310,6 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
 // rows in the rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal
 // to the reference's two M^3 products in exact arithmetic.
 //
-// The grid has kWXcd * NP workgroups of which only those with
-// blockIdx % kWXcd == 0 work: workgroups are dispatched to the XCDs in
-// round-robin order, so the NP working groups share one XCD and its L2.
 // The launch is an ordinary one (a cooperative launch is refused while
 // another process on the GPU holds the cooperative queue); the NP groups
 // are few enough to be co-resident on an otherwise idle device, and every
@@ -320,7 +317,7 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
 // workgroup leaves and the call returns an error, so the kernel always
 // terminates.
 constexpr int kWThreads = 256;
-constexpr int kWXcd = 8;
+constexpr int kWRowsPerThread = 4;   // M <= kWThreads * kWRowsPerThread = 1024
 constexpr unsigned kBarrierSpinLimit = 1u << 22;
 
 struct WPlan {
@@ -404,8 +401,7 @@ __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
     double* __restrict__ comp, double* __restrict__ sig, WBufs bufs) {
     extern __shared__ double sh[];
     const int M = p.M;
-    if (blockIdx.x % kWXcd != 0) return;
-    const int wg = blockIdx.x / kWXcd;
+    const int wg = blockIdx.x;
     const int r0 = wg * p.RW;
     const int nrows = (M - r0) < p.RW ? (M - r0) : p.RW;
     double* Gl = sh;                                    // [RW][M]
@@ -432,7 +428,14 @@ __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
             if (threadIdx.x == 0) bufs.psum[par * p.NP + wg] = part;
             ++epoch;
             if (!grid_barrier(bufs.counter, epoch * static_cast<unsigned>(p.NP), bufs.err)) return;
-            // partials: loaded in parallel (after the acquire), summed in order
+            // published rows and partials are loaded together (after the
+            // acquire); the partials are summed in workgroup order
+            double wreg[kWRowsPerThread];
+#pragma unroll
+            for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
+                const int i = threadIdx.x + k2 * kWThreads;
+                wreg[k2] = i < M ? bufs.wbuf[par * M + i] : 0.0;
+            }
             for (int q = threadIdx.x; q < p.NP; q += blockDim.x) u[q] = bufs.psum[par * p.NP + q];
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -444,9 +447,10 @@ __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
             const double tot = red[1];
             if (!defl) {
                 sigma = sqrt(tot);
-                for (int i = threadIdx.x; i < M; i += blockDim.x) {
-                    const double w = bufs.wbuf[par * M + i];
-                    v[i] = sigma != 0.0 ? w / sigma : w;
+#pragma unroll
+                for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
+                    const int i = threadIdx.x + k2 * kWThreads;
+                    if (i < M) v[i] = sigma != 0.0 ? wreg[k2] / sigma : wreg[k2];
                 }
                 __syncthreads();
                 if (it == 9) {
@@ -456,7 +460,12 @@ __global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
                 }
             } else {
                 const double alpha = tot;
-                for (int i = threadIdx.x; i < M; i += blockDim.x) u[i] = bufs.wbuf[par * M + i];
+                __syncthreads();  // the partials in u are read before u is overwritten
+#pragma unroll
+                for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
+                    const int i = threadIdx.x + k2 * kWThreads;
+                    if (i < M) u[i] = wreg[k2];
+                }
                 __syncthreads();
                 for (int64_t q = threadIdx.x; q < static_cast<int64_t>(nrows) * M; q += blockDim.x) {
                     const int r = static_cast<int>(q / M);
@@ -814,7 +823,7 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     const size_t lds = sizeof(double) * (static_cast<size_t>(wp.RW) * M + 2 * M + wp.RW + 2);
     MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    whitening_power_kernel<<<static_cast<unsigned>(wp.NP * kWXcd), kWThreads, lds, st>>>(
+    whitening_power_kernel<<<static_cast<unsigned>(wp.NP), kWThreads, lds, st>>>(
         XXt, wp, v0, comp, sig, wb);
     MSAMD_LAUNCH_CHECK();
     int herr = 0;

@@ -1,9 +1,9 @@
 """Learned spike-presence detector on device-resident tensors (C ABI: msamd_tcn_*).
 
 The network scores 32-sample windows of every channel (see include/msamd.h).
-Detection then runs only where it proposes spikes, with the reference's
-detection rule (detect_events.cpp:69-135 over the ms3alg time-chunk plan,
-p_mountainsort3.cpp:170-216) deciding the uncertain windows.  Training labels
+Detection then runs only in the windows it proposes (score >= score_lo),
+where the reference's detection rule (detect_events.cpp:69-135 over the
+ms3alg time-chunk plan, p_mountainsort3.cpp:170-216) decides every sample.  Training labels
 come from plain detection (frontend.detect_events) on a training recording;
 training runs on the GPU through msamd_tcn_train.
 """

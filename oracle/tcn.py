@@ -176,12 +176,11 @@ def gated_detect(X, classes, detect_threshold, detect_interval, detect_sign, num
                  chunk_size=None, t1=-1, t2=-1, padding=1000):
     """Event times of the gated rule on every channel, per channel ascending.
 
-    classes (nW, M): 0 = no spike (skipped), 1 = uncertain (the plain rule of
-    detect_events decides every sample), 2 = spike present (the first maximum
-    of v over the window's samples in the padded chunk, kept when it exceeds
-    the threshold and passes the +-D suppression stencil: one event per
-    window).  Chunk plan, statistics and v are those of
-    ms3alg detection (p_mountainsort3.cpp:170-216,507-533; detect_events.cpp).
+    classes (nW, M): 0 = no spike (the window is skipped); 1 = uncertain and
+    2 = present (the window was proposed by the network): the plain rule of
+    detect_events decides every sample.  Chunk plan, statistics and v are
+    those of ms3alg detection (p_mountainsort3.cpp:170-216,507-533;
+    detect_events.cpp).
     Returns (counts (M,), times concatenated by channel)."""
     X = np.asarray(X, np.float32)
     N, M = X.shape
@@ -210,23 +209,16 @@ def gated_detect(X, classes, detect_threshold, detect_interval, detect_sign, num
             elif detect_sign == 0:
                 v = np.abs(v)
             n = len(v)
-            stencil = np.ones(n, bool)
+            keep = v > thr
             for d in range(1, D + 1):
                 left = np.full(n, -np.inf)
                 left[d:] = v[:-d]
                 right = np.full(n, -np.inf)
                 right[:-d] = v[d:]
-                stencil &= (v > left) & (v >= right)
-            keep = stencil & (v > thr)
-            for t in range(core0, core0 + size):
-                cl = classes[t // WIN, c]
-                if cl == 1 and keep[t - wa]:
-                    tc.append(t)
-                elif cl == 2 and keep[t - wa]:
-                    w = t // WIN
-                    lo, hi = max(WIN * w, wa), min(WIN * w + WIN, wb)
-                    if t == lo + int(np.argmax(v[lo - wa:hi - wa])):
-                        tc.append(t)
+                keep &= (v > left) & (v >= right)
+            t = np.arange(core0, core0 + size)
+            sel = keep[t - wa] & (classes[t // WIN, c] != 0)
+            tc.extend(t[sel].tolist())
         counts[c] = len(tc)
         out.extend(tc)
     return counts, np.asarray(out, np.int64)
