@@ -32,7 +32,7 @@ using fft::cf;
 using fft::cmul;
 using fft::cconjmul;
 
-constexpr int kPB = 16;          // channel pairs per workgroup
+constexpr int kPB = 8;           // channel pairs per workgroup
 constexpr int kTwLo = 512;       // four-step twiddle: W = hi[e >> 9] * lo[e & 511]
 
 struct BpPlan {

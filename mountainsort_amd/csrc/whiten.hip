@@ -55,7 +55,7 @@ constexpr int kCovWaves = 4;
 // so the span is read from HBM about once.
 constexpr int kCovAllWaves = 8;
 constexpr int kCovAllTPW = 20;
-constexpr int kCovAllKSteps = 600;   // 32-sample steps per workgroup (FP32 partial length)
+constexpr int kCovAllKSteps = 1200;  // 32-sample steps per workgroup (FP32 partial length)
 constexpr int kCovAllItems = 3;      // (channel, 8-sample group) items per thread (4*Mp <= 1536)
 
 struct CovPlan {
