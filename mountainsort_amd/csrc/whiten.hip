@@ -299,186 +299,309 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
 }
 
 // ------------------------------------------------------- whitening matrix
-// The power method is a chain of M * 11 dependent matrix-vector products, so
-// its speed is set by how fast one product and one exchange complete.  G is
-// split by rows over NP workgroups and held in their LDS (24 rows of 384
-// doubles = 74 KB each at M = 384); every power step each workgroup forms
-// its rows of w = G v plus its partial |w|^2, publishes them, and all
-// workgroups meet at a device-scope barrier before normalizing the full
-// vector (partials summed in workgroup order, so every workgroup holds the
-// same v).  Deflation G <- (I-vv')G(I-vv') is applied to each workgroup's
-// rows in the rank-1 form G - v u' - u v' + (v'u) v v' with u = G v, equal
-// to the reference's two M^3 products in exact arithmetic.
+// The power method is a chain of dependent matrix-vector products (10 per
+// component), so its speed is set by how fast one product and one exchange
+// complete.  G is split by rows over NP workgroups of one wavefront each and
+// held in their LDS (4 rows of 384 doubles in each of 96 workgroups at
+// M = 384).
+//
+// Each row is owned by a group of L lanes (L = 16 up to M = 384): lane q of
+// the group holds columns q, q+L, ... of the row in LDS and, in registers,
+// the same columns of the current vector v.  A power step is
+//   * the row's dot product with v: per-lane FMAs, then a DPP reduction over
+//     the L lanes (every lane ends with the same sum);
+//   * publication of the row's value of w = G v as one tagged 64-bit word:
+//     the value rounded to 48 fraction bits with the step number (mod 16)
+//     in the 4 low bits, written by a relaxed agent-scope atomic store;
+//   * every lane polls its columns of the published w with relaxed
+//     agent-scope atomic loads until each word carries the current tag.  A
+//     64-bit atomic word is never torn, so a word with the right tag holds
+//     this step's value: no flags, fences or workgroup barriers are needed;
+//   * every row group forms |w|^2 by the same per-lane sums and the same DPP
+//     reduction, so all row groups of all workgroups hold bit-identical
+//     v = w / |w|.
+// Published words alternate between two slots by step parity.  A lane
+// waiting for step e can find step e-2 in its slot (a different tag) but
+// never step e+2: the owner of a row publishes step e+1 only after its whole
+// wavefront has read step e, and step e+2 needs every row of step e+1.
+// Row groups without a row (padding of the last workgroup) neither publish
+// nor poll.  The 2^-48 rounding of the exchanged values is far below the
+// power method's own sensitivity to rounding.
+//
+// Deflation G <- (I-vv')G(I-vv') is applied to each workgroup's rows in the
+// rank-1 form G - v u' - u v' + (v'u) v v' (u = G v), equal to the
+// reference's two M^3 products in exact arithmetic.  Its exchange (u) also
+// carries y = G v0, from which the first power step of the next component
+// follows without another exchange:
+//   G' v0 = y - v (u'v0) - u (v'v0) + (v'u) v (v'v0).
+// So a component costs 10 exchanges instead of 11.
 //
 // The launch is an ordinary one (a cooperative launch is refused while
 // another process on the GPU holds the cooperative queue); the NP groups
 // are few enough to be co-resident on an otherwise idle device, and every
-// barrier wait is bounded: on timeout the error flag is raised, every
-// workgroup leaves and the call returns an error, so the kernel always
-// terminates.
-constexpr int kWThreads = 256;
-constexpr int kWRowsPerThread = 4;   // M <= kWThreads * kWRowsPerThread = 1024
-constexpr unsigned kBarrierSpinLimit = 1u << 22;
+// wait is bounded: on timeout the error flag is raised, later waits end at
+// once, every wavefront runs to the end and the call returns an error.
+constexpr int kWMaxThreads = 512;
+constexpr unsigned kWSpinLimit = 1u << 20;
 
 struct WPlan {
-    int M, NP, RW;
+    int M, Mp, S, L, J, RW, NP, T;
 };
 
 inline WPlan w_plan(int64_t M) {
     WPlan p{};
     p.M = static_cast<int>(M);
-    int rw = static_cast<int>(ceil_div(M, 16));
-    // rows per workgroup so that rows + two vectors fit in ~150 KB of LDS
-    const int cap = static_cast<int>((19200 - 2 * M) / M);
-    if (rw > cap) rw = cap;
+    // lanes per row and columns per lane: one kernel instance per (L, J)
+    if (M <= 64) { p.L = 16; p.J = 4; }
+    else if (M <= 128) { p.L = 16; p.J = 8; }
+    else if (M <= 256) { p.L = 16; p.J = 16; }
+    else if (M <= 384) { p.L = 16; p.J = 24; }
+    else if (M <= 768) { p.L = 32; p.J = 24; }
+    else { p.L = 64; p.J = 16; }
+    p.Mp = p.L * p.J;                 // zero-padded vector length
+    p.S = p.Mp + 16;                  // LDS row stride: adjacent rows 128 B apart
+    // one wavefront per workgroup (measured fastest at M = 384: 4 rows per
+    // workgroup, 96 workgroups), at most 128 workgroups
+    int rw = 64 / p.L;
+    if (rw < ceil_div(M, 128)) rw = static_cast<int>(ceil_div(M, 128));
+    const int cap_lds = static_cast<int>((150 * 1024 / 8 - 3 * p.Mp) / p.S);
+    const int cap_thr = kWMaxThreads / p.L;
+    if (rw > cap_lds) rw = cap_lds;
+    if (rw > cap_thr) rw = cap_thr;
     if (rw < 1) rw = 1;
     p.RW = rw;
     p.NP = static_cast<int>(ceil_div(M, rw));
+    p.T = static_cast<int>(align_up(static_cast<int64_t>(rw) * p.L, 64));
     return p;
 }
 
-__device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
-    // every wave's stores complete (workgroup scope: a wait, no cache flush),
-    // then one agent-scope release by thread 0 publishes the workgroup's data
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();
-    __shared__ int failed;
-    if (threadIdx.x == 0) {
-        failed = 0;
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        // relaxed polling (coherent loads, no invalidate per poll)
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                ++spins > kBarrierSpinLimit) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                failed = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+inline size_t w_lds_bytes(const WPlan& p) {
+    return sizeof(double) * (static_cast<size_t>(p.RW) * p.S + 3 * static_cast<size_t>(p.Mp));
+}
+
+// x read from the lane selected by the DPP control CTRL (all rows, all banks)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                          static_cast<unsigned>(lo));
+}
+
+// Sum over the L lanes of a row group; every lane of the group gets the same
+// value (each step adds a lane's value and its partner's, a + b == b + a).
+template <int L>
+__device__ __forceinline__ double group_sum(double x) {
+    x = x + dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+    x = x + dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+    x = x + dpp_f64<0x141>(x);  // row_half_mirror: quads 0<->1, 2<->3
+    x = x + dpp_f64<0x140>(x);  // row_mirror: half rows 0<->1
+    if (L >= 32) x = x + __shfl_xor(x, 16, 64);
+    if (L >= 64) x = x + __shfl_xor(x, 32, 64);
+    return x;
+}
+
+typedef unsigned long long u64;
+
+// x rounded to 48 fraction bits (nearest, ties away from zero) with tag in
+// the 4 low bits
+__device__ __forceinline__ u64 w_tagged(double x, unsigned tag) {
+    const u64 b = __builtin_bit_cast(u64, x);
+    return ((b + 8ull) & ~15ull) | tag;
+}
+
+__device__ __forceinline__ void w_publish(u64* slot, double x, unsigned tag) {
+    __hip_atomic_store(slot, w_tagged(x, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lane q's columns q + L*j of a published vector (pub points at column q of
+// the slot); words beyond M are never written and read as 0.  Returns false
+// after a timeout or when another wavefront has raised the error flag.
+template <int L, int J>
+__device__ __forceinline__ bool w_fetch(const u64* pub, int q, int M, unsigned tag, double* out,
+                                        int* err) {
+    unsigned spins = 0;
+    for (;;) {
+        bool ready = true;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const u64 b = __hip_atomic_load(const_cast<u64*>(pub) + j * L, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            ready = ready && (j * L + q >= M || (b & 15ull) == tag);
+            out[j] = __builtin_bit_cast(double, b & ~15ull);
         }
-        // one acquire for the workgroup (the CU's caches are shared by its waves)
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (ready) return true;
+        if ((++spins & 63u) == 0 &&
+            (spins > kWSpinLimit ||
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return failed == 0;
 }
 
 struct WBufs {
-    double* wbuf;     // [2][M] published rows of G v
-    double* psum;     // [2][NP] published partial sums
-    unsigned* counter;
+    u64* wbuf;  // [2][Mp] published rows of G v (tagged words)
+    u64* ybuf;  // [2][Mp] published rows of G v0 (deflation exchange)
     int* err;
 };
 
-// sum over this workgroup's rows of (G v)_r * y_r into w_out (rows), returns
-// the workgroup-local sum of w_r * z_r where z = w (norm) or v (v'u).
-__device__ double rows_matvec(const double* Gl, int M, int nrows, const double* v, double* w_out,
-                              const double* dotv, double* red) {
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    for (int r = wid; r < nrows; r += nw) {
-        const double* row = Gl + static_cast<int64_t>(r) * M;
-        double acc = 0.0;
-        for (int n = lane; n < M; n += 64) acc += row[n] * v[n];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (lane == 0) w_out[r] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int r = 0; r < nrows; ++r) s = s + w_out[r] * (dotv ? dotv[r] : w_out[r]);
-        red[0] = s;
-    }
-    __syncthreads();
-    return red[0];
-}
-
-__global__ void __launch_bounds__(kWThreads) whitening_power_kernel(
-    const double* __restrict__ XXt, WPlan p, const double* __restrict__ v0,
+template <int L, int J>
+__global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
+    const double* __restrict__ XXt, WPlan p, const double* __restrict__ v0g,
     double* __restrict__ comp, double* __restrict__ sig, WBufs bufs) {
     extern __shared__ double sh[];
-    const int M = p.M;
-    const int wg = blockIdx.x;
-    const int r0 = wg * p.RW;
+    const int M = p.M, S = p.S, Mp = p.Mp;
+    const int tid = threadIdx.x;
+    const int rg = tid / L;   // row group = local row
+    const int q = tid % L;
+    const int r0 = blockIdx.x * p.RW;
     const int nrows = (M - r0) < p.RW ? (M - r0) : p.RW;
-    double* Gl = sh;                                    // [RW][M]
-    double* v = Gl + static_cast<int64_t>(p.RW) * M;     // [M]
-    double* u = v + M;                                  // [M]
-    double* wl = u + M;                                 // [RW]
-    double* red = wl + p.RW;                            // [2]
-    for (int64_t q = threadIdx.x; q < static_cast<int64_t>(nrows) * M; q += blockDim.x) {
-        const int r = static_cast<int>(q / M);
-        const int n = static_cast<int>(q - static_cast<int64_t>(r) * M);
-        Gl[q] = XXt[static_cast<int64_t>(r0 + r) * M + n];
+    const int row = r0 + rg;
+    const bool has_row = rg < nrows;
+    double* Gl = sh;                                   // [RW][S]
+    double* v0l = Gl + static_cast<int64_t>(p.RW) * S; // [Mp]
+    double* vl = v0l + Mp;                             // [Mp] staging of v (deflation)
+    double* ul = vl + Mp;                              // [Mp] staging of u (deflation)
+    for (int64_t e = tid; e < static_cast<int64_t>(p.RW) * S; e += blockDim.x) {
+        const int r = static_cast<int>(e / S);
+        const int c = static_cast<int>(e - static_cast<int64_t>(r) * S);
+        Gl[e] = (r < nrows && c < M) ? XXt[static_cast<int64_t>(r0 + r) * M + c] : 0.0;
     }
+    for (int i = tid; i < Mp; i += blockDim.x) v0l[i] = i < M ? v0g[i] : 0.0;
+    __syncthreads();
+    double* Grow = Gl + static_cast<int64_t>(has_row ? rg : 0) * S + q;
+    u64* wpub = bufs.wbuf + q;
+    u64* ypub = bufs.ybuf + q;
+
+    double v[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) v[j] = v0l[j * L + q];
     unsigned epoch = 0;
+    double sigma = 0.0;
+    int s_first = 1;
     for (int k = 0; k < M; ++k) {
-        for (int i = threadIdx.x; i < M; i += blockDim.x) v[i] = v0[i];
-        __syncthreads();
-        double sigma = 0.0;
-        for (int it = 0; it < 11; ++it) {
-            // it < 10: power step w = G v; it == 10: u = G v for the deflation
-            const bool defl = (it == 10);
-            const int par = epoch & 1;
-            const double part = rows_matvec(Gl, M, nrows, v, wl, defl ? v + r0 : nullptr, red);
-            for (int r = threadIdx.x; r < nrows; r += blockDim.x) bufs.wbuf[par * M + r0 + r] = wl[r];
-            if (threadIdx.x == 0) bufs.psum[par * p.NP + wg] = part;
+        for (int s = s_first; s <= 10; ++s) {
             ++epoch;
-            if (!grid_barrier(bufs.counter, epoch * static_cast<unsigned>(p.NP), bufs.err)) return;
-            // published rows and partials are loaded together (after the
-            // acquire); the partials are summed in workgroup order
-            double wreg[kWRowsPerThread];
+            const int par = epoch & 1;
+            const unsigned tag = epoch & 15u;
+            if (has_row) {
+                // the row's J values first (independent LDS reads in flight together)
+                double g[J];
 #pragma unroll
-            for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
-                const int i = threadIdx.x + k2 * kWThreads;
-                wreg[k2] = i < M ? bufs.wbuf[par * M + i] : 0.0;
+                for (int j = 0; j < J; ++j) g[j] = Grow[j * L];
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < J; ++j) acc = fma(g[j], v[j], acc);
+                acc = group_sum<L>(acc);
+                if (q == 0) w_publish(bufs.wbuf + par * Mp + row, acc, tag);
+                w_fetch<L, J>(wpub + par * Mp, q, M, tag, v, bufs.err);
             }
-            for (int q = threadIdx.x; q < p.NP; q += blockDim.x) u[q] = bufs.psum[par * p.NP + q];
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double s = 0.0;
-                for (int q = 0; q < p.NP; ++q) s = s + u[q];
-                red[1] = s;
-            }
-            __syncthreads();
-            const double tot = red[1];
-            if (!defl) {
-                sigma = sqrt(tot);
+            double ss = 0.0;
 #pragma unroll
-                for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
-                    const int i = threadIdx.x + k2 * kWThreads;
-                    if (i < M) v[i] = sigma != 0.0 ? wreg[k2] / sigma : wreg[k2];
-                }
-                __syncthreads();
-                if (it == 9) {
-                    for (int r = threadIdx.x; r < nrows; r += blockDim.x)
-                        comp[static_cast<int64_t>(r0 + r) * M + k] = v[r0 + r];
-                    if (wg == 0 && threadIdx.x == 0) sig[k] = sigma;
-                }
-            } else {
-                const double alpha = tot;
-                __syncthreads();  // the partials in u are read before u is overwritten
+            for (int j = 0; j < J; ++j) ss = fma(v[j], v[j], ss);
+            ss = group_sum<L>(ss);
+            sigma = sqrt(ss);
+            if (sigma != 0.0) {
+                const double inv = 1.0 / sigma;
 #pragma unroll
-                for (int k2 = 0; k2 < kWRowsPerThread; ++k2) {
-                    const int i = threadIdx.x + k2 * kWThreads;
-                    if (i < M) u[i] = wreg[k2];
-                }
-                __syncthreads();
-                for (int64_t q = threadIdx.x; q < static_cast<int64_t>(nrows) * M; q += blockDim.x) {
-                    const int r = static_cast<int>(q / M);
-                    const int n = static_cast<int>(q - static_cast<int64_t>(r) * M);
-                    const int i = r0 + r;
-                    const double cross = v[i] * u[n] + u[i] * v[n];
-                    const double vv = v[i] * v[n];
-                    Gl[q] = (Gl[q] - cross) + alpha * vv;
-                }
-                __syncthreads();
+                for (int j = 0; j < J; ++j) v[j] = v[j] * inv;
             }
         }
+        // component k (pca.cpp:549-557): v and its last norm
+        if (rg == 0) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int n = j * L + q;
+                if (n >= r0 && n < r0 + nrows) comp[static_cast<int64_t>(n) * M + k] = v[j];
+            }
+        }
+        if (blockIdx.x == 0 && tid == 0) sig[k] = sigma;
+        if (k == M - 1) break;
+
+        // deflation exchange: u = G v and y = G v0
+        ++epoch;
+        const int par = epoch & 1;
+        const unsigned tag = epoch & 15u;
+        double u[J], y[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) u[j] = y[j] = 0.0;
+        if (has_row) {
+            double au = 0.0, ay = 0.0;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const double g = Grow[j * L];
+                au = fma(g, v[j], au);
+                ay = fma(g, v0l[j * L + q], ay);
+            }
+            au = group_sum<L>(au);
+            ay = group_sum<L>(ay);
+            if (q == 0) {
+                w_publish(bufs.wbuf + par * Mp + row, au, tag);
+                w_publish(bufs.ybuf + par * Mp + row, ay, tag);
+            }
+            w_fetch<L, J>(wpub + par * Mp, q, M, tag, u, bufs.err);
+            w_fetch<L, J>(ypub + par * Mp, q, M, tag, y, bufs.err);
+        }
+        double d_uv0 = 0.0, d_vv0 = 0.0, alpha = 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const double v0j = v0l[j * L + q];
+            d_uv0 = fma(u[j], v0j, d_uv0);
+            d_vv0 = fma(v[j], v0j, d_vv0);
+            alpha = fma(v[j], u[j], alpha);
+        }
+        d_uv0 = group_sum<L>(d_uv0);
+        d_vv0 = group_sum<L>(d_vv0);
+        alpha = group_sum<L>(alpha);
+        if (rg == 0) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                vl[j * L + q] = v[j];
+                ul[j * L + q] = u[j];
+            }
+        }
+        __syncthreads();
+        if (has_row) {
+            const double vi = vl[row], ui = ul[row];
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const double g = Grow[j * L];
+                const double cross = vi * u[j] + ui * v[j];
+                Grow[j * L] = (g - cross) + alpha * (vi * v[j]);
+            }
+        }
+        // staging reads complete before the next component's staging write
+        __syncthreads();
+        // first power step of component k+1: w = G' v0 from the exchanged vectors
+        const double c_v = alpha * d_vv0 - d_uv0;
+        double ss = 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const double z = (y[j] + c_v * v[j]) - d_vv0 * u[j];
+            y[j] = z;
+            ss = fma(z, z, ss);
+        }
+        ss = group_sum<L>(ss);
+        sigma = sqrt(ss);
+        const double inv = sigma != 0.0 ? 1.0 / sigma : 1.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) v[j] = y[j] * inv;
+        s_first = 2;
     }
+}
+
+template <int L, int J>
+int launch_power(dim3 grid, dim3 block, size_t lds, hipStream_t st, const double* XXt,
+                 const WPlan& wp, const double* v0, double* comp, double* sig, const WBufs& wb) {
+    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel<L, J>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
+    whitening_power_kernel<L, J><<<grid, block, lds, st>>>(XXt, wp, v0, comp, sig, wb);
+    return MSAMD_OK;
 }
 
 // W[m, n] = sum_l (C[m,l] d_l) C[n,l], l ascending (mult_AB then
@@ -765,12 +888,11 @@ extern "C" int64_t msamd_whitening_matrix_workspace_size(int64_t M) {
     if (M < 1) return -1;
     const WPlan wp = w_plan(M);
     Carver c(nullptr);
-    c.take<double>(M * M);  // components
-    c.take<double>(M);      // sigma
-    c.take<double>(M);      // v0
-    c.take<double>(2 * M);  // published G v rows
-    c.take<double>(2 * wp.NP);
-    c.take<int>(2);         // barrier counter, error flag
+    c.take<double>(M * M);       // components
+    c.take<double>(M);           // sigma
+    c.take<double>(M);           // v0
+    c.take<u64>(4 * wp.Mp);      // published rows of G v and of G v0, two slots each
+    c.take<int>(1);              // error flag
     return c.bytes();
 }
 
@@ -778,14 +900,14 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
                                       int64_t workspace_bytes, void* stream) {
     clear_error();
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whitening_matrix: M=%lld outside 1..1024", (long long)M);
-    const WPlan wp0 = w_plan(M);
+    const WPlan wp = w_plan(M);
     Carver c(workspace);
     double* comp = c.take<double>(M * M);
     double* sig = c.take<double>(M);
     double* v0 = c.take<double>(M);
-    double* wbuf = c.take<double>(2 * M);
-    double* psum = c.take<double>(2 * wp0.NP);
-    int* flags = c.take<int>(2);
+    u64* wbuf = c.take<u64>(4 * wp.Mp);
+    u64* ybuf = wbuf + 2 * wp.Mp;
+    int* err = c.take<int>(1);
     if (!workspace || workspace_bytes < c.bytes()) {
         set_error("whitening_matrix: workspace too small (%lld < %lld)",
                   (long long)workspace_bytes, (long long)c.bytes());
@@ -813,24 +935,36 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     MSAMD_HIP(e1);
     MSAMD_HIP(e2);
     // XXt is symmetric: row-major and column-major views coincide.
-    const WPlan wp = w_plan(M);
+    MSAMD_REQUIRE(wp.T <= kWMaxThreads && wp.Mp >= M,
+                  "whitening_matrix: plan out of range (M=%lld)", (long long)M);
     WBufs wb{};
     wb.wbuf = wbuf;
-    wb.psum = psum;
-    wb.counter = reinterpret_cast<unsigned*>(flags);
-    wb.err = flags + 1;
-    MSAMD_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), st));
-    const size_t lds = sizeof(double) * (static_cast<size_t>(wp.RW) * M + 2 * M + wp.RW + 2);
-    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    whitening_power_kernel<<<static_cast<unsigned>(wp.NP), kWThreads, lds, st>>>(
-        XXt, wp, v0, comp, sig, wb);
+    wb.ybuf = ybuf;
+    wb.err = err;
+    // published words start at 0.  A word is first written at step 1 or 2
+    // (G v slots) or 11 (G v0 slot), whose tags are not 0; later a waiting
+    // lane can only find the tag of an earlier use (step e-2, or e-10 for the
+    // G v0 slot), never equal to e mod 16.
+    MSAMD_HIP(hipMemsetAsync(wbuf, 0, 4 * sizeof(u64) * wp.Mp, st));
+    MSAMD_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    const size_t lds = w_lds_bytes(wp);
+    const dim3 grid(static_cast<unsigned>(wp.NP)), block(static_cast<unsigned>(wp.T));
+    int rc = MSAMD_OK;
+    switch (wp.L * 100 + wp.J) {
+    case 1604: rc = launch_power<16, 4>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    case 1608: rc = launch_power<16, 8>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    case 1616: rc = launch_power<16, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    case 1624: rc = launch_power<16, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    case 3224: rc = launch_power<32, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    default: rc = launch_power<64, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
+    }
+    if (rc != MSAMD_OK) return rc;
     MSAMD_LAUNCH_CHECK();
     int herr = 0;
-    MSAMD_HIP(hipMemcpyAsync(&herr, flags + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    MSAMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
     MSAMD_HIP(hipStreamSynchronize(st));
     if (herr) {
-        set_error("whitening_matrix: workgroup barrier timed out");
+        set_error("whitening_matrix: exchange between workgroups timed out");
         return MSAMD_ERR_HIP;
     }
     whitening_assemble_kernel<<<static_cast<unsigned>(ceil_div(M * M, 256)), 256, 0, st>>>(

@@ -139,6 +139,21 @@ def test_whitening_matrix_quality_m64(gpu):
     assert qg < max(2 * qo, 1e-6) + 0.05
 
 
+@pytest.mark.parametrize("M", [24, 100, 200, 384, 500, 1000])
+def test_whitening_matrix_spiked_spectrum(gpu, M):
+    """Every kernel instance (16, 32 and 64 lanes per row) at sizes where the
+    oracle's chain is chaotic on generic input: on a spiked spectrum W is the
+    exact C^-1/2 whatever basis the rounding picks in the identity block."""
+    from mountainsort_amd import frontend as fe
+    from tests.test_oracle import _spiked_covariance
+    say(f"whitening matrix spiked spectrum M={M}")
+    C, R = _spiked_covariance(M, 600 + M)
+    Wg = fe.whitening_matrix(torch.from_numpy(C).to(gpu)).cpu().numpy()
+    rel = np.max(np.abs(Wg - R)) / np.max(np.abs(R))
+    say(f"  max rel |W_gpu - C^-1/2| = {rel:.3e}")
+    assert rel < 1e-7
+
+
 @pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000)])
 def test_whiten_apply_within_one_quantum(gpu, M, N):
     from mountainsort_amd import frontend as fe

@@ -266,6 +266,29 @@ def test_whitening_matrix_converged_case():
     np.testing.assert_allclose(sig, lam, rtol=1e-9)
 
 
+def _spiked_covariance(M, seed):
+    """C = Q diag(lam) Q' with five separated eigenvalues 1e6, 1e5, 1e4, 1e3,
+    1e2 (ratio 0.1: ten power steps converge to 1e-10) above an identity
+    block.  On the identity block the restarted power method picks whatever
+    orthonormal basis the rounding leaves (as for C = c I), and every such
+    basis gives the same W.  Returns (C, C^-1/2)."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.normal(size=(M, M)))
+    lam = np.ones(M)
+    k = min(5, M)
+    lam[:k] = 10.0 ** np.arange(6, 6 - k, -1)
+    C = (Q * lam) @ Q.T
+    return (C + C.T) / 2, (Q / np.sqrt(lam)) @ Q.T
+
+
+def test_whitening_matrix_spiked_spectrum():
+    """Converged components followed by a degenerate block: the result must
+    be the exact inverse square root."""
+    C, R = _spiked_covariance(24, 6)
+    W = oc.whitening_matrix(C)
+    assert np.max(np.abs(W - R)) / np.max(np.abs(R)) < 1e-9
+
+
 def test_whiten_apply_vs_fp64():
     rng = np.random.default_rng(5)
     M, N = 16, 20000
