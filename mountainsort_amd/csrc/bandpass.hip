@@ -32,7 +32,8 @@ using fft::cf;
 using fft::cmul;
 using fft::cconjmul;
 
-constexpr int kPB = 8;           // channel pairs per workgroup
+constexpr int kPBAC = 8;         // channel pairs per workgroup, stages A and C
+constexpr int kPBB = 8;          // channel pairs per workgroup, stage B
 constexpr int kTwLo = 512;       // four-step twiddle: W = hi[e >> 9] * lo[e & 511]
 
 struct BpPlan {
@@ -61,7 +62,7 @@ constexpr Shape kShapes[] = {
 // ------------------------------------------------------------------ stage A
 // Workgroup: one n2, kPB pairs of one unit; thread (j, pl) with pl fastest.
 // pass 1: radix R1 over x[j + r*R2] (R2 tasks), pass 2: radix R2 (R1 tasks).
-template <int R1, int R2, bool kI16>
+template <int R1, int R2, bool kI16, int kPB>
 __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     const void* __restrict__ Xv, BpPlan p, int64_t unit0, const cf* __restrict__ twN1,
     const cf* __restrict__ twHi, const cf* __restrict__ twLo, cf* __restrict__ S) {
@@ -146,7 +147,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
 // Workgroup: one k1, kPB pairs of one unit.  Forward N2 = Q1*Q2 transform
 // (pass radix Q1 then Q2), gain, inverse (radix Q2 on the same registers,
 // then radix Q1), written back in place.
-template <int Q1, int Q2>
+template <int Q1, int Q2, int kPB>
 __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
     BpPlan p, int64_t unit0, const cf* __restrict__ twN2, const float* __restrict__ Ht,
     cf* __restrict__ S) {
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
 // Workgroup: one n2, kPB pairs.  Inverse over k1: pass 1 radix R2 over
 // k1 = j + r*R1 (R1 tasks, after the conjugate four-step twiddle), pass 2
 // radix R1 (Ns = R2, R2 tasks) -> n1 = j + r*R2.
-template <int R1, int R2, bool kOutI16>
+template <int R1, int R2, bool kOutI16, bool kSub1, int kPB>
 __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
     BpPlan p, int64_t unit0, const cf* __restrict__ twN1, const cf* __restrict__ twHi,
     const cf* __restrict__ twLo, const cf* __restrict__ S, void* __restrict__ Yv) {
@@ -268,9 +269,14 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
                 const int64_t n1 = j + r * R2;
                 const int64_t tl = n1 * p.N2 + n2 - p.ov;   // core-relative
                 if (tl < 0 || tl >= p.cs) continue;
-                if (tl % p.k_sub) continue;
-                if (tl / p.k_sub >= p.cs / p.k_sub) continue;
-                const int64_t to = (w * p.cs) / p.k_sub + tl / p.k_sub;
+                int64_t to;
+                if (kSub1) {
+                    to = w * p.cs + tl;
+                } else {
+                    if (tl % p.k_sub) continue;
+                    if (tl / p.k_sub >= p.cs / p.k_sub) continue;
+                    to = (w * p.cs) / p.k_sub + tl / p.k_sub;
+                }
                 if (to >= p.N_out) continue;
                 const float vals[2] = {y[r].x, y[r].y};
                 if ((p.M % 2) == 0) {  // both channels of the pair in one store
@@ -330,7 +336,7 @@ BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double
     p.N1 = sh.R1 * sh.R2;
     p.N2 = sh.Q1 * sh.Q2;
     p.P = static_cast<int32_t>((M + 1) / 2);
-    p.CG = static_cast<int32_t>(align_up(p.P < 32 ? p.P : 32, kPB));
+    p.CG = static_cast<int32_t>(align_up(p.P < 32 ? p.P : 32, kPBAC));
     p.nGroups = static_cast<int32_t>(ceil_div(p.P, p.CG));
     p.nWin = ceil_div(N, cs);
     p.nUnits = p.nWin * p.nGroups;
@@ -486,27 +492,33 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         MSAMD_HIP(hipMemcpyAsync(b.twLo, tl.data(), sizeof(cf) * kTwLo, hipMemcpyHostToDevice, st));
         MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
     }
-    const int npb = p.CG / kPB;
     for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
         const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
-        const unsigned gridAC = static_cast<unsigned>(nu * npb * p.N2);
-        const unsigned gridB = static_cast<unsigned>(nu * npb * p.N1);
+        const unsigned gridAC = static_cast<unsigned>(nu * (p.CG / kPBAC) * p.N2);
+        const unsigned gridB = static_cast<unsigned>(nu * (p.CG / kPBB) * p.N1);
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
             if (p.in_i16)
-                bp_stage_a<20, 20, true><<<gridAC, kPB * 20, 0, st>>>(X, p, u0, b.twN1, b.twHi,
-                                                                      b.twLo, b.S);
+                bp_stage_a<20, 20, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(X, p, u0, b.twN1,
+                                                                             b.twHi, b.twLo, b.S);
             else
-                bp_stage_a<20, 20, false><<<gridAC, kPB * 20, 0, st>>>(X, p, u0, b.twN1, b.twHi,
-                                                                       b.twLo, b.S);
+                bp_stage_a<20, 20, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(X, p, u0, b.twN1,
+                                                                              b.twHi, b.twLo, b.S);
             MSAMD_LAUNCH_CHECK();
-            bp_stage_b<24, 25><<<gridB, kPB * 25, 0, st>>>(p, u0, b.twN2, b.Ht, b.S);
+            bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, st>>>(p, u0, b.twN2, b.Ht, b.S);
             MSAMD_LAUNCH_CHECK();
-            if (p.out_i16)
-                bp_stage_c<20, 20, true><<<gridAC, kPB * 20, 0, st>>>(p, u0, b.twN1, b.twHi,
-                                                                      b.twLo, b.S, Y);
+            const bool s1 = p.k_sub == 1;
+            if (p.out_i16 && s1)
+                bp_stage_c<20, 20, true, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+            else if (p.out_i16)
+                bp_stage_c<20, 20, true, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+            else if (s1)
+                bp_stage_c<20, 20, false, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             else
-                bp_stage_c<20, 20, false><<<gridAC, kPB * 20, 0, st>>>(p, u0, b.twN1, b.twHi,
-                                                                       b.twLo, b.S, Y);
+                bp_stage_c<20, 20, false, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             MSAMD_LAUNCH_CHECK();
         } else {
             set_error("bandpass: shape not instantiated");
