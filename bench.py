@@ -54,7 +54,7 @@ class Pipeline:
         self.dev = dev
         self.M, self.N = M, N
         self.num_threads = num_threads
-        self.with_pca = with_pca and hasattr(fe, "pca_features")
+        self.with_pca = with_pca
         nb = neighborhoods(M, cfg.columns, cfg.pitch, radius)
         self.nb_channels = torch.tensor(np.concatenate(nb), dtype=torch.int32, device=dev)
         self.nb_offsets = torch.tensor(np.concatenate([[0], np.cumsum([len(c) for c in nb])]),
@@ -319,7 +319,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seconds", type=float, default=300.0, help="recording length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-pca", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS),
                     help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
@@ -342,7 +341,7 @@ def main():
     raw = synth.generate(cfg, device=dev)
     torch.cuda.synchronize()
     log(f"[bench] generated {cfg.M} x {N} int16 in {time.time() - t0:.1f}s")
-    pipe = Pipeline(dev, cfg.M, N, cfg, with_pca=not args.no_pca)
+    pipe = Pipeline(dev, cfg.M, N, cfg)
     if args.pmc_pass:
         from mountainsort_amd import frontend as fe
         pipe.run(raw)

@@ -32,8 +32,37 @@ using fft::cf;
 using fft::cmul;
 using fft::cconjmul;
 
-constexpr int kPBAC = 8;         // channel pairs per workgroup, stages A and C
-constexpr int kPBB = 8;          // channel pairs per workgroup, stage B
+// Tuning (compile-time; the defaults are the measured best on MI355X).
+#ifndef MSAMD_BP_PBA
+#define MSAMD_BP_PBA 8
+#endif
+#ifndef MSAMD_BP_PBB
+#define MSAMD_BP_PBB 8
+#endif
+#ifndef MSAMD_BP_PBC
+#define MSAMD_BP_PBC 8
+#endif
+#ifndef MSAMD_BP_CGMAX
+#define MSAMD_BP_CGMAX 32
+#endif
+#ifndef MSAMD_BP_SCRATCH_MB
+#define MSAMD_BP_SCRATCH_MB 128
+#endif
+#ifndef MSAMD_BP_PB_FAST
+#define MSAMD_BP_PB_FAST 0
+#endif
+constexpr int kPBA = MSAMD_BP_PBA;  // channel pairs per workgroup, stage A
+constexpr int kPBB = MSAMD_BP_PBB;  // channel pairs per workgroup, stage B
+constexpr int kPBC = MSAMD_BP_PBC;  // channel pairs per workgroup, stage C
+constexpr int kCGMax = MSAMD_BP_CGMAX;  // channel pairs per unit (a multiple of every kPB*)
+// Block order of stages A and C: n2 fastest (0), or the pair blocks of one
+// (unit, n2) adjacent (1) so that they share the same input rows and output
+// rows in time.
+constexpr bool kPbFast = MSAMD_BP_PB_FAST != 0;
+constexpr int kCGAlign = kPBA > kPBB ? (kPBA > kPBC ? kPBA : kPBC) : (kPBB > kPBC ? kPBB : kPBC);
+static_assert(kCGMax % kPBA == 0 && kCGMax % kPBB == 0 && kCGMax % kPBC == 0 &&
+                  kCGAlign % kPBA == 0 && kCGAlign % kPBB == 0 && kCGAlign % kPBC == 0,
+              "pairs per unit must be a multiple of the pairs per workgroup of every stage");
 constexpr int kTwLo = 512;       // four-step twiddle: W = hi[e >> 9] * lo[e & 511]
 
 struct BpPlan {
@@ -59,6 +88,22 @@ constexpr Shape kShapes[] = {
     {20, 20, 24, 25},  // 400 x 600 = 240000 (chunk 120000, overlap 60000)
 };
 
+// Workgroup index -> (unit within the batch, pair block, column n2).
+__device__ __forceinline__ void block_unit_pb_col(int64_t b, int npb, int N2, int64_t* u, int* pb,
+                                                  int* n2) {
+    if (kPbFast) {
+        *pb = static_cast<int>(b % npb);
+        b /= npb;
+        *n2 = static_cast<int>(b % N2);
+        *u = b / N2;
+    } else {
+        *n2 = static_cast<int>(b % N2);
+        b /= N2;
+        *pb = static_cast<int>(b % npb);
+        *u = b / npb;
+    }
+}
+
 // ------------------------------------------------------------------ stage A
 // Workgroup: one n2, kPB pairs of one unit; thread (j, pl) with pl fastest.
 // pass 1: radix R1 over x[j + r*R2] (R2 tasks), pass 2: radix R2 (R1 tasks).
@@ -76,11 +121,10 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     const int j = tid / kPB;
     // block -> (unit, pair block, n2)
     const int npb = p.CG / kPB;
-    int64_t b = blockIdx.x;
-    const int n2 = static_cast<int>(b % p.N2);
-    b /= p.N2;
-    const int pb = static_cast<int>(b % npb);
-    const int64_t u = unit0 + b / npb;
+    int n2, pb;
+    int64_t u;
+    block_unit_pb_col(blockIdx.x, npb, p.N2, &u, &pb, &n2);
+    u += unit0;
     const int g = static_cast<int>(u / p.nWin);
     const int64_t w = u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;   // global pair index
@@ -231,11 +275,10 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
     const int pl = tid % kPB;
     const int j = tid / kPB;
     const int npb = p.CG / kPB;
-    int64_t b = blockIdx.x;
-    const int n2 = static_cast<int>(b % p.N2);
-    b /= p.N2;
-    const int pb = static_cast<int>(b % npb);
-    const int64_t u = unit0 + b / npb;
+    int n2, pb;
+    int64_t u;
+    block_unit_pb_col(blockIdx.x, npb, p.N2, &u, &pb, &n2);
+    u += unit0;
     const int g = static_cast<int>(u / p.nWin);
     const int64_t w = u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;
@@ -336,7 +379,7 @@ BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double
     p.N1 = sh.R1 * sh.R2;
     p.N2 = sh.Q1 * sh.Q2;
     p.P = static_cast<int32_t>((M + 1) / 2);
-    p.CG = static_cast<int32_t>(align_up(p.P < 32 ? p.P : 32, kPBAC));
+    p.CG = static_cast<int32_t>(align_up(p.P < kCGMax ? p.P : kCGMax, kCGAlign));
     p.nGroups = static_cast<int32_t>(ceil_div(p.P, p.CG));
     p.nWin = ceil_div(N, cs);
     p.nUnits = p.nWin * p.nGroups;
@@ -371,7 +414,7 @@ BpBuffers bp_carve(void* ws, const BpPlan& p, int64_t* bytes) {
     return b;
 }
 
-constexpr int64_t kScratchBudget = 128LL << 20;  // spectra kept in the last-level cache
+constexpr int64_t kScratchBudget = static_cast<int64_t>(MSAMD_BP_SCRATCH_MB) << 20;  // spectra kept in the last-level cache
 
 }  // namespace
 }  // namespace msamd
@@ -494,30 +537,31 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     }
     for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
         const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
-        const unsigned gridAC = static_cast<unsigned>(nu * (p.CG / kPBAC) * p.N2);
+        const unsigned gridA = static_cast<unsigned>(nu * (p.CG / kPBA) * p.N2);
+        const unsigned gridC = static_cast<unsigned>(nu * (p.CG / kPBC) * p.N2);
         const unsigned gridB = static_cast<unsigned>(nu * (p.CG / kPBB) * p.N1);
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
             if (p.in_i16)
-                bp_stage_a<20, 20, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(X, p, u0, b.twN1,
+                bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, b.twN1,
                                                                              b.twHi, b.twLo, b.S);
             else
-                bp_stage_a<20, 20, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(X, p, u0, b.twN1,
+                bp_stage_a<20, 20, false, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, b.twN1,
                                                                               b.twHi, b.twLo, b.S);
             MSAMD_LAUNCH_CHECK();
             bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, st>>>(p, u0, b.twN2, b.Ht, b.S);
             MSAMD_LAUNCH_CHECK();
             const bool s1 = p.k_sub == 1;
             if (p.out_i16 && s1)
-                bp_stage_c<20, 20, true, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                bp_stage_c<20, 20, true, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
                     p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             else if (p.out_i16)
-                bp_stage_c<20, 20, true, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                bp_stage_c<20, 20, true, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
                     p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             else if (s1)
-                bp_stage_c<20, 20, false, true, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                bp_stage_c<20, 20, false, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
                     p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             else
-                bp_stage_c<20, 20, false, false, kPBAC><<<gridAC, kPBAC * 20, 0, st>>>(
+                bp_stage_c<20, 20, false, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
                     p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
             MSAMD_LAUNCH_CHECK();
         } else {
