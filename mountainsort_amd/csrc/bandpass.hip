@@ -48,8 +48,14 @@ using fft::cconjmul;
 #ifndef MSAMD_BP_SCRATCH_MB
 #define MSAMD_BP_SCRATCH_MB 128
 #endif
-#ifndef MSAMD_BP_PB_FAST
-#define MSAMD_BP_PB_FAST 0
+#ifndef MSAMD_BP_ORDER
+#define MSAMD_BP_ORDER 2
+#endif
+#ifndef MSAMD_BP_LDS_PAD
+#define MSAMD_BP_LDS_PAD 1
+#endif
+#ifndef MSAMD_BP_TW_LDS
+#define MSAMD_BP_TW_LDS 1
 #endif
 constexpr int kPBA = MSAMD_BP_PBA;  // channel pairs per workgroup, stage A
 constexpr int kPBB = MSAMD_BP_PBB;  // channel pairs per workgroup, stage B
@@ -58,7 +64,14 @@ constexpr int kCGMax = MSAMD_BP_CGMAX;  // channel pairs per unit (a multiple of
 // Block order of stages A and C: n2 fastest (0), or the pair blocks of one
 // (unit, n2) adjacent (1) so that they share the same input rows and output
 // rows in time.
-constexpr bool kPbFast = MSAMD_BP_PB_FAST != 0;
+constexpr int kOrder = MSAMD_BP_ORDER;
+// LDS rows of the register-pass exchange padded to an odd length: the rows a
+// half-wavefront writes then start on different banks.
+__host__ __device__ constexpr int lds_row(int R) { return MSAMD_BP_LDS_PAD ? (R | 1) : R; }
+// Four-step twiddles W_n^{n2*k1} of a column n2 at k1 = j + r*R1 are formed as
+// W_n^{n2*j} * W_n^{n2*R1*r}: one table product per thread plus a per-workgroup
+// LDS table over r (instead of two table gathers per element).
+constexpr bool kTwLds = MSAMD_BP_TW_LDS != 0;
 constexpr int kCGAlign = kPBA > kPBB ? (kPBA > kPBC ? kPBA : kPBC) : (kPBB > kPBC ? kPBB : kPBC);
 static_assert(kCGMax % kPBA == 0 && kCGMax % kPBB == 0 && kCGMax % kPBC == 0 &&
                   kCGAlign % kPBA == 0 && kCGAlign % kPBB == 0 && kCGAlign % kPBC == 0,
@@ -89,9 +102,21 @@ constexpr Shape kShapes[] = {
 };
 
 // Workgroup index -> (unit within the batch, pair block, column n2).
+// Order 2 is XCD-aware: workgroups are dispatched round-robin over the eight
+// XCDs, so the pair blocks of one column are given ids that differ by
+// multiples of 8 inside one run of 8*npb ids -- the same XCD (and L2) at
+// nearly the same time.  The grid is rounded up to whole runs; ids past the
+// last column return *u = -1.
 __device__ __forceinline__ void block_unit_pb_col(int64_t b, int npb, int N2, int64_t* u, int* pb,
                                                   int* n2) {
-    if (kPbFast) {
+    if (kOrder == 2) {
+        const int64_t run = b / (8 * npb);
+        const int q = static_cast<int>(b % (8 * npb));
+        *pb = q / 8;
+        const int64_t col = run * 8 + (q % 8);
+        *n2 = static_cast<int>(col % N2);
+        *u = col / N2;
+    } else if (kOrder == 1) {
         *pb = static_cast<int>(b % npb);
         b /= npb;
         *n2 = static_cast<int>(b % N2);
@@ -109,11 +134,12 @@ __device__ __forceinline__ void block_unit_pb_col(int64_t b, int npb, int N2, in
 // pass 1: radix R1 over x[j + r*R2] (R2 tasks), pass 2: radix R2 (R1 tasks).
 template <int R1, int R2, bool kI16, int kPB>
 __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
-    const void* __restrict__ Xv, BpPlan p, int64_t unit0, const cf* __restrict__ twN1,
+    const void* __restrict__ Xv, BpPlan p, int64_t unit0, int64_t nu, const cf* __restrict__ twN1,
     const cf* __restrict__ twHi, const cf* __restrict__ twLo, cf* __restrict__ S) {
     constexpr int N1 = R1 * R2;
     constexpr int TPS = R1 > R2 ? R1 : R2;
-    __shared__ cf buf[N1 * kPB];
+    constexpr int RA = lds_row(R1);
+    __shared__ cf buf[R2 * RA * kPB];
     __shared__ cf tw1[N1];
     const int tid = threadIdx.x;
     for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
@@ -124,7 +150,13 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     int n2, pb;
     int64_t u;
     block_unit_pb_col(blockIdx.x, npb, p.N2, &u, &pb, &n2);
+    if (u >= nu) return;  // whole workgroup: past the last column of the batch
     u += unit0;
+    __shared__ cf stp[R2];  // W_n^{n2*R1*r}, r < R2 (n2*R1*r < n)
+    if (kTwLds && tid < R2) {
+        const int e = n2 * R1 * tid;
+        stp[tid] = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+    }
     const int g = static_cast<int>(u / p.nWin);
     const int64_t w = u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;   // global pair index
@@ -165,23 +197,33 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
         }
         fft::Dft<R1, -1>::run(v);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) buf[(j * R1 + r) * kPB + pl] = v[r];
+        for (int r = 0; r < R1; ++r) buf[(j * RA + r) * kPB + pl] = v[r];
     }
     __syncthreads();
     if (j < R1) {
         cf y[R2];
 #pragma unroll
         for (int r = 0; r < R2; ++r) {
-            const cf x = buf[(j + r * R1) * kPB + pl];
+            const cf x = buf[(r * RA + j) * kPB + pl];
             y[r] = (r == 0 || j == 0) ? x : cmul(x, tw1[r * j]);
         }
         fft::Dft<R2, -1>::run(y);
         cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+        cf wj = {1.f, 0.f};
+        if (kTwLds) {
+            const int e = n2 * j;
+            wj = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+        }
 #pragma unroll
         for (int r = 0; r < R2; ++r) {
             const int k1 = j + r * R1;
-            const int e = n2 * k1;  // < n, no reduction needed
-            const cf w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            cf w4;
+            if (kTwLds) {
+                w4 = cmul(wj, stp[r]);
+            } else {
+                const int e = n2 * k1;  // < n, no reduction needed
+                w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            }
             Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kPB + pl] = cmul(y[r], w4);
         }
     }
@@ -193,21 +235,23 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
 // then radix Q1), written back in place.
 template <int Q1, int Q2, int kPB>
 __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
-    BpPlan p, int64_t unit0, const cf* __restrict__ twN2, const float* __restrict__ Ht,
+    BpPlan p, int64_t unit0, int64_t nu, const cf* __restrict__ twN2, const float* __restrict__ Ht,
     cf* __restrict__ S) {
     constexpr int N2 = Q1 * Q2;
-    __shared__ cf buf[N2 * kPB];
+    constexpr int QA = lds_row(Q1), QB = lds_row(Q2);
+    constexpr int kBuf = (Q2 * QA > Q1 * QB ? Q2 * QA : Q1 * QB) * kPB;
+    __shared__ cf buf[kBuf];
     __shared__ cf tw2[N2];
     const int tid = threadIdx.x;
     for (int q = tid; q < N2; q += blockDim.x) tw2[q] = twN2[q];
     const int pl = tid % kPB;
     const int j = tid / kPB;
     const int npb = p.CG / kPB;
-    int64_t b = blockIdx.x;
-    const int k1 = static_cast<int>(b % p.N1);
-    b /= p.N1;
-    const int pb = static_cast<int>(b % npb);
-    const int64_t u = unit0 + b / npb;
+    int k1, pb;
+    int64_t u;
+    block_unit_pb_col(blockIdx.x, npb, p.N1, &u, &pb, &k1);
+    if (u >= nu) return;  // whole workgroup: past the last row of the batch
+    u += unit0;
     cf* Srow = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG +
                static_cast<int64_t>(k1) * N2 * p.CG + pb * kPB + pl;
     // forward pass 1: radix Q1 over n2 = j + r*Q2 (Q2 tasks)
@@ -217,7 +261,7 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
         for (int r = 0; r < Q1; ++r) v[r] = Srow[static_cast<int64_t>(j + r * Q2) * p.CG];
         fft::Dft<Q1, -1>::run(v);
 #pragma unroll
-        for (int r = 0; r < Q1; ++r) buf[(j * Q1 + r) * kPB + pl] = v[r];
+        for (int r = 0; r < Q1; ++r) buf[(j * QA + r) * kPB + pl] = v[r];
     }
     __syncthreads();
     // forward pass 2: radix Q2 (Q1 tasks) -> k2 = j + r*Q1; gain; inverse pass 1
@@ -226,7 +270,7 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
     if (j < Q1) {
 #pragma unroll
         for (int r = 0; r < Q2; ++r) {
-            const cf x = buf[(j + r * Q1) * kPB + pl];
+            const cf x = buf[(r * QA + j) * kPB + pl];
             y[r] = (r == 0 || j == 0) ? x : cmul(x, tw2[r * j]);
         }
         fft::Dft<Q2, -1>::run(y);
@@ -241,7 +285,7 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
     __syncthreads();
     if (j < Q1) {
 #pragma unroll
-        for (int r = 0; r < Q2; ++r) buf[(j * Q2 + r) * kPB + pl] = y[r];
+        for (int r = 0; r < Q2; ++r) buf[(j * QB + r) * kPB + pl] = y[r];
     }
     __syncthreads();
     // inverse pass 2: radix Q1, Ns = Q2 (Q2 tasks): inputs j + r*Q2, twiddle
@@ -250,7 +294,7 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
         cf v[Q1];
 #pragma unroll
         for (int r = 0; r < Q1; ++r) {
-            const cf x = buf[(j + r * Q2) * kPB + pl];
+            const cf x = buf[(r * QB + j) * kPB + pl];
             v[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw2[r * j]);
         }
         fft::Dft<Q1, 1>::run(v);
@@ -265,10 +309,11 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
 // radix R1 (Ns = R2, R2 tasks) -> n1 = j + r*R2.
 template <int R1, int R2, bool kOutI16, bool kSub1, int kPB>
 __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
-    BpPlan p, int64_t unit0, const cf* __restrict__ twN1, const cf* __restrict__ twHi,
+    BpPlan p, int64_t unit0, int64_t nu, const cf* __restrict__ twN1, const cf* __restrict__ twHi,
     const cf* __restrict__ twLo, const cf* __restrict__ S, void* __restrict__ Yv) {
     constexpr int N1 = R1 * R2;
-    __shared__ cf buf[N1 * kPB];
+    constexpr int RC = lds_row(R2);
+    __shared__ cf buf[R1 * RC * kPB];
     __shared__ cf tw1[N1];
     const int tid = threadIdx.x;
     for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
@@ -278,31 +323,50 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
     int n2, pb;
     int64_t u;
     block_unit_pb_col(blockIdx.x, npb, p.N2, &u, &pb, &n2);
+    if (u >= nu) return;  // whole workgroup: past the last column of the batch
     u += unit0;
     const int g = static_cast<int>(u / p.nWin);
     const int64_t w = u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;
     const int ch0 = 2 * pair;
     const cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+    __shared__ cf stp[R2];  // W_n^{n2*R1*r}, r < R2 (n2*R1*r < n)
+    if (kTwLds) {
+        if (tid < R2) {
+            const int e = n2 * R1 * tid;
+            stp[tid] = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+        }
+        __syncthreads();
+    }
     if (j < R1) {
         cf v[R2];
+        cf wj = {1.f, 0.f};
+        if (kTwLds) {
+            const int e = n2 * j;
+            wj = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+        }
 #pragma unroll
         for (int r = 0; r < R2; ++r) {
             const int k1 = j + r * R1;
-            const int e = n2 * k1;
-            const cf w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            cf w4;
+            if (kTwLds) {
+                w4 = cmul(wj, stp[r]);
+            } else {
+                const int e = n2 * k1;
+                w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            }
             v[r] = cconjmul(Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kPB + pl], w4);
         }
         fft::Dft<R2, 1>::run(v);
 #pragma unroll
-        for (int r = 0; r < R2; ++r) buf[(j * R2 + r) * kPB + pl] = v[r];
+        for (int r = 0; r < R2; ++r) buf[(j * RC + r) * kPB + pl] = v[r];
     }
     __syncthreads();
     if (j < R2) {
         cf y[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
-            const cf x = buf[(j + r * R2) * kPB + pl];
+            const cf x = buf[(r * RC + j) * kPB + pl];
             y[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw1[r * j]);
         }
         fft::Dft<R1, 1>::run(y);
@@ -537,32 +601,37 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     }
     for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
         const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
-        const unsigned gridA = static_cast<unsigned>(nu * (p.CG / kPBA) * p.N2);
-        const unsigned gridC = static_cast<unsigned>(nu * (p.CG / kPBC) * p.N2);
-        const unsigned gridB = static_cast<unsigned>(nu * (p.CG / kPBB) * p.N1);
+        auto grid = [&](int pb_per_wg, int64_t cols_per_unit) {
+            const int64_t cols = nu * cols_per_unit;
+            const int64_t npb = p.CG / pb_per_wg;
+            return static_cast<unsigned>(kOrder == 2 ? ceil_div(cols, 8) * 8 * npb : cols * npb);
+        };
+        const unsigned gridA = grid(kPBA, p.N2);
+        const unsigned gridC = grid(kPBC, p.N2);
+        const unsigned gridB = grid(kPBB, p.N1);
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
             if (p.in_i16)
-                bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, b.twN1,
+                bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, nu, b.twN1,
                                                                              b.twHi, b.twLo, b.S);
             else
-                bp_stage_a<20, 20, false, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, b.twN1,
+                bp_stage_a<20, 20, false, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, nu, b.twN1,
                                                                               b.twHi, b.twLo, b.S);
             MSAMD_LAUNCH_CHECK();
-            bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, st>>>(p, u0, b.twN2, b.Ht, b.S);
+            bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, st>>>(p, u0, nu, b.twN2, b.Ht, b.S);
             MSAMD_LAUNCH_CHECK();
             const bool s1 = p.k_sub == 1;
             if (p.out_i16 && s1)
                 bp_stage_c<20, 20, true, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
             else if (p.out_i16)
                 bp_stage_c<20, 20, true, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
             else if (s1)
                 bp_stage_c<20, 20, false, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
             else
                 bp_stage_c<20, 20, false, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, b.twN1, b.twHi, b.twLo, b.S, Y);
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
             MSAMD_LAUNCH_CHECK();
         } else {
             set_error("bandpass: shape not instantiated");
