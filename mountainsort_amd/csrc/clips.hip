@@ -36,26 +36,39 @@ __global__ void __launch_bounds__(256) clips_kernel(
 
 // Batched form: events of neighborhood g are times[eo[g]..eo[g+1]) (centers),
 // its channels nc[no[g]..no[g+1]), its clips start at clips + co[g].
-// blockIdx.y selects the neighborhood; the blocks along x stride over its
-// events, one wavefront per clip.  For W <= 64 each lane owns one channel
+// Work item (g, j) takes the j-th of P equal slices of g's events, one
+// wavefront per clip.  Events are time-sorted per neighborhood, so slice j of
+// every neighborhood lies near time j/P of the recording.  Workgroups are
+// dispatched round-robin over the eight XCDs; XCD k is given the contiguous
+// neighborhoods [k*gx, (k+1)*gx) and walks them slice by slice, so the
+// overlapping source rows of neighboring channels' clips meet in its L2.  For W <= 64 each lane owns one channel
 // (w = lane % W) and a row offset (lane / W), so the channel index is loaded
 // once and the W x T clip is covered by rows of 64 / W time samples; the
 // loads of several rows are issued before their stores.
 __global__ void __launch_bounds__(256) clips_batched_kernel(
     const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ times,
     const int64_t* __restrict__ eo, const int32_t* __restrict__ nc,
-    const int64_t* __restrict__ no, int32_t T, int32_t tmid, const int64_t* __restrict__ co,
-    float* __restrict__ clips) {
-    const int g = blockIdx.y;
+    const int64_t* __restrict__ no, int32_t G, int32_t P, int32_t T, int32_t tmid,
+    const int64_t* __restrict__ co, float* __restrict__ clips) {
+    const int64_t b = blockIdx.x;
+    const int gx = (G + 7) / 8;                 // neighborhoods per XCD
+    const int64_t sx = b / 8;                   // position in this XCD's sequence
+    const int g = static_cast<int>((b % 8) * gx + sx % gx);
+    const int64_t j = sx / gx;                  // slice
+    if (g >= G || j >= P) return;
     const int lane = threadIdx.x & 63;
-    const int64_t e0 = eo[g];
-    const int64_t L = eo[g + 1] - e0;
+    const int64_t eg = eo[g];
+    const int64_t Lg = eo[g + 1] - eg;
+    const int64_t s0 = Lg * j / P;              // this slice: events [s0, s1) of g
+    const int64_t s1 = Lg * (j + 1) / P;
+    const int64_t e0 = eg + s0;
+    const int64_t L = s1 - s0;
     const int32_t* ch = nc + no[g];
     const int32_t W = static_cast<int32_t>(no[g + 1] - no[g]);
     const int32_t WT = W * T;
-    float* base = clips + co[g];
-    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    float* base = clips + co[g] + s0 * WT;
+    const int64_t wave = threadIdx.x >> 6;
+    const int64_t nwaves = blockDim.x >> 6;
     if (W <= 64) {
         const int rpi = 64 / W;           // rows per pass
         const int w = lane % W;
@@ -134,14 +147,15 @@ extern "C" int msamd_extract_clips_batched(const float* X, int64_t M, int64_t N,
     MSAMD_REQUIRE(T >= 1, "extract_clips_batched: clip_size is zero");
     if (total_events == 0) return MSAMD_OK;
     const int32_t tmid = (T + 1) / 2 - 1;
-    // blocks per neighborhood sized for the mean event count (4 clips per block pass)
-    int64_t per = ceil_div(ceil_div(total_events, G), 16);
-    if (per < 1) per = 1;
-    if (per > 4096) per = 4096;
-    const dim3 grid(static_cast<unsigned>(per), static_cast<unsigned>(G));
-    clips_batched_kernel<<<grid, 256, 0, as_stream(stream)>>>(X, M, N, times, event_offsets,
-                                                              nbhd_channels, nbhd_offsets, T,
-                                                              tmid, clip_offsets, clips);
+    // slices per neighborhood sized for ~16 clips (4 per wavefront) at the mean event count
+    int64_t P = ceil_div(ceil_div(total_events, G), 16);
+    if (P < 1) P = 1;
+    const int64_t gx = ceil_div(G, 8);
+    const int64_t blocks = 8 * gx * P;
+    MSAMD_REQUIRE(blocks < (1LL << 31), "extract_clips_batched: too many work items");
+    clips_batched_kernel<<<static_cast<unsigned>(blocks), 256, 0, as_stream(stream)>>>(
+        X, M, N, times, event_offsets, nbhd_channels, nbhd_offsets, G, static_cast<int32_t>(P), T,
+        tmid, clip_offsets, clips);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }
