@@ -154,7 +154,9 @@ def test_whitening_matrix_spiked_spectrum(gpu, M):
     assert rel < 1e-7
 
 
-@pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000)])
+@pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000),
+                                 (128, 3000), (192, 100_000), (200, 4001), (256, 2500),
+                                 (320, 3333), (130, 2000)])
 def test_whiten_apply_within_one_quantum(gpu, M, N):
     from mountainsort_amd import frontend as fe
     say(f"whiten apply M={M} N={N}")
