@@ -229,6 +229,82 @@ def learned_leg(dev, pipe, cfg, reps=3, train_seconds=60.0, steps=4000):
     return out
 
 
+def pca_batching_leg(dev, reps=3):
+    """Batched PCA workload: 384 neighborhoods of 5-12 channels with
+    log-uniform 100-200000 events (synth.pca_batch_shapes seed 3), 50-sample
+    clips (synth.pca_batch_clips), 10 + 10 features.  The tiles come from the
+    autotuner; the batched plan is timed as plain launches and as one HIP
+    graph launch, against one plan per neighborhood launched in turn (same
+    tiles, so the features are bit-identical)."""
+    import numpy as np
+    import torch
+    from mountainsort_amd import frontend as fe
+    from mountainsort_amd import synth
+    L, W = synth.pca_batch_shapes(G=384, seed=3)
+    clips, co = synth.pca_batch_clips(L, W, T=50, seed=3, device=dev)
+    torch.cuda.synchronize()
+    tiles, tune_ms = fe.pca_autotune(clips, L, W, 50)
+    plan = fe.PcaPlan(L, W, 50, tiles=tiles, device=dev)
+    dplan = fe.PcaPlan(L, W, 50, device=dev)
+    singles = [fe.PcaPlan(L[g:g + 1], W[g:g + 1], 50, tiles=tiles, device=dev)
+               for g in range(len(L))]
+    views = [clips[co[g]:co[g + 1]] for g in range(len(L))]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            out.append(a.elapsed_time(b))
+        return float(np.median(out))
+
+    ms_launch = timed(lambda: plan.run(clips))
+    ms_graph = timed(lambda: plan.run(clips, graph=True))
+    ms_default = timed(lambda: dplan.run(clips))
+
+    def per_neighborhood():
+        for g, p1 in enumerate(singles):
+            if L[g]:
+                p1.run(views[g])
+
+    ms_single = timed(per_neighborhood)
+    _, ro, feat, fo = plan.run(clips, graph=True)
+    torch.cuda.synchronize()
+    same = all(torch.equal(singles[g].features[:fo[g + 1] - fo[g]], feat[fo[g]:fo[g + 1]])
+               for g in range(len(L)) if L[g])
+    events = int(L.sum())
+    clip_bytes = int((L * W).sum()) * 50 * 4
+    # algorithmic bytes: clips read once, reduced written and read once, features written
+    alg = clip_bytes + 2 * int((L * W).sum()) * 10 * 4 + events * 10 * 4
+    out = {
+        "workload": "384 neighborhoods, widths U{5..12}, events log-U(100, 200000) "
+                    "(synth.pca_batch_shapes seed 3), T = 50, 10 + 10 features",
+        "events": events, "clip_GB": round(clip_bytes / 1e9, 3),
+        "tiles": {"gram_columns_per_item": tiles[0], "stage1_events_per_item": tiles[1],
+                  "stage2_events_per_item": tiles[2], "autotune_best_ms": round(tune_ms, 3)},
+        "ms": {"batched_graph": round(ms_graph, 3), "batched_launches": round(ms_launch, 3),
+               "batched_default_tiles": round(ms_default, 3),
+               "per_neighborhood_launches": round(ms_single, 3)},
+        "speedup_batched_vs_per_neighborhood": round(ms_single / ms_graph, 3),
+        "achieved_GBs_batched_graph": round(alg / (ms_graph * 1e-3) / 1e9, 1),
+        "per_neighborhood_features_identical": bool(same),
+    }
+    for p1 in singles:
+        p1.close()
+    plan.close()
+    dplan.close()
+    del clips
+    torch.cuda.empty_cache()
+    log("[bench] pca batching: " + json.dumps(out))
+    return out
+
+
 def cpu_baseline(M, N_full, events_full, widths, log_fn=log):
     """Time the oracle (the CPU restatement of the reference, single thread)
     stage by stage on a bounded sample of the same workload and scale each
@@ -422,6 +498,7 @@ def main():
     }
     if not args.no_learned:
         line["learned"] = learned_leg(dev, pipe, cfg)
+    line["pca_batching"] = pca_batching_leg(dev)
     if not args.no_cpu_baseline:
         widths = pipe.widths
         line["cpu_baseline"] = cpu_baseline(cfg.M, N, pipe.stats.get("events", 0), widths)

@@ -171,6 +171,53 @@ int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_
                        int32_t num_features, int64_t max_pca_samples, float* reduced,
                        float* features, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* The same computation as a reusable plan: the host work (neighborhood
+ * tables, work items, their upload into the workspace) is done once by
+ * msamd_pca_plan_create, and msamd_pca_plan_run only enqueues the nine
+ * batched kernels, so it may be captured into a HIP graph.
+ * msamd_pca_plan_launch_graph captures that sequence once per (clips,
+ * reduced, features) triple on a private stream and replays it with one
+ * hipGraphLaunch on the given stream.  tiles (nullable) = {Gram columns per
+ * work item (multiple of 32), stage-1 events per projection item, stage-2
+ * events per projection item}; null selects {2560, 64, 64}.  The plan keeps
+ * pointers into the workspace, which must outlive it.  No reference
+ * counterpart: the reference computes each neighborhood's PCA serially on
+ * the CPU (neighborhoodsorter.cpp:236-276); callers that process many
+ * recordings of one probe reuse a plan. */
+typedef struct msamd_pca_plan msamd_pca_plan;
+int64_t msamd_pca_plan_workspace_size(int32_t G, const int64_t* num_events_host,
+                                      const int32_t* widths_host, int32_t T,
+                                      int32_t num_features_per_channel, int32_t num_features,
+                                      int64_t max_pca_samples, const int32_t* tiles);
+int msamd_pca_plan_create(int32_t G, const int64_t* num_events_host, const int32_t* widths_host,
+                          int32_t T, int32_t num_features_per_channel, int32_t num_features,
+                          int64_t max_pca_samples, const int32_t* tiles, void* workspace,
+                          int64_t workspace_bytes, void* stream, msamd_pca_plan** plan);
+int msamd_pca_plan_run(msamd_pca_plan* plan, const float* clips, float* reduced, float* features,
+                       void* stream);
+int msamd_pca_plan_launch_graph(msamd_pca_plan* plan, const float* clips, float* reduced,
+                                float* features, void* stream);
+void msamd_pca_plan_tiles(const msamd_pca_plan* plan, int32_t* tiles);
+void msamd_pca_plan_destroy(msamd_pca_plan* plan);
+
+/* Autotuner of the plan's tiles for one workload: a coordinate search over
+ * Gram columns per item {1280, 2560, 5120, 10240}, stage-1 events per item
+ * {32, 64, 128, 256} and stage-2 events per item {16, 32, 64, 128}, each
+ * candidate timed with HIP events on the stream (second of two runs).
+ * Writes the best tiles to tiles_out[3] and their time to best_ms_out
+ * (nullable).  reduced/features receive the last candidate's results.  The
+ * workspace must hold msamd_pca_autotune_workspace_size bytes.  Synchronizes
+ * the stream. */
+int64_t msamd_pca_autotune_workspace_size(int32_t G, const int64_t* num_events_host,
+                                          const int32_t* widths_host, int32_t T,
+                                          int32_t num_features_per_channel, int32_t num_features,
+                                          int64_t max_pca_samples);
+int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_host,
+                       const int32_t* widths_host, int32_t T, int32_t num_features_per_channel,
+                       int32_t num_features, int64_t max_pca_samples, float* reduced,
+                       float* features, void* workspace, int64_t workspace_bytes, void* stream,
+                       int32_t* tiles_out, float* best_ms_out);
+
 /* ------------------------------------------------------------------------
  * Whitening (a4-a6) -- replaces p_whiten (packages/ms3/p_whiten.cpp:45-206)
  * and whitening_matrix_from_XXt (packages/mountainsortalg/pca.cpp:634-650).

@@ -36,8 +36,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kKC = 32;        // subsampled columns per MFMA K step
 constexpr int kMaxW = 16;      // channels per neighborhood supported
 constexpr int kMaxT = 64;      // clip length supported
-constexpr int kProjE1 = 64;    // events per stage-1 projection item
-constexpr int kProjE2 = 64;    // events per stage-2 projection item
 constexpr int kMaxTiles = 20;  // Gram tiles per wavefront (8 waves per workgroup)
 constexpr int kGramThreads = 512;
 
@@ -46,6 +44,10 @@ struct PcaPlan {
     int64_t max_s;
     int32_t maxW;
     int64_t idx_stride;  // max_s + 2
+    // tiles (chosen per workload, see msamd_pca_autotune)
+    int32_t gram_chunk;  // subsampled columns per Gram work item (multiple of kKC)
+    int32_t pe1;         // events per stage-1 projection item
+    int32_t pe2;         // events per stage-2 projection item
 };
 
 struct PcaTables {
@@ -98,7 +100,7 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 }
 
 // ------------------------------------------------------------------ Gram
-// Work item = (neighborhood, chunk of up to kGramChunk subsampled columns);
+// Work item = (neighborhood, chunk of up to gram_chunk subsampled columns);
 // the chunk partial Grams are summed in FP64 by pca_gram_reduce_kernel in
 // chunk order.  Each K step stages 32 columns (S floats each: the W x T clip
 // for stage 1, the nf1*W reduced vector for stage 2) in LDS as bf16 hi/lo
@@ -110,7 +112,6 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 // sub-matrices of T rows (row t of channel w is element t*W + w of a
 // column); stage 2 one sub-matrix of nf1*W rows.  Rows past the end read as
 // zero.  Waves own Gram tiles (q, I, J >= I).
-constexpr int kGramChunk = 2560;
 constexpr int kRowStride = 40;
 
 __device__ __forceinline__ int gram_rows_alloc(int nrows) { return nrows | 1; }
@@ -166,8 +167,8 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
     const bf16x8* hv = reinterpret_cast<const bf16x8*>(hi);
     const bf16x8* lv = reinterpret_cast<const bf16x8*>(lo);
     const bf16x8 zero8 = {};
-    const int cbeg = chunk * kGramChunk;
-    int cend = cbeg + kGramChunk;
+    const int cbeg = chunk * p.gram_chunk;
+    int cend = cbeg + p.gram_chunk;
     if (cend > ns) cend = ns;
     for (int c0 = cbeg; c0 < cend; c0 += kKC) {
         __syncthreads();  // previous step's fragments consumed
@@ -360,7 +361,7 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
 }
 
 // ------------------------------------------------------------ projections
-// Stage 1 item: up to kProjE1 events of one neighborhood.  The components
+// Stage 1 item: up to pe1 events of one neighborhood.  The components
 // (W x T x nf1) sit in LDS with rows padded to 16 floats; thread (event,
 // channel) reads its clip samples straight from global memory (consecutive
 // threads -> consecutive channels -> contiguous addresses) and forms nf1
@@ -377,7 +378,7 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
     const int W = tb.W[g];
     const int T = p.T;
     const int nf = p.nf1;
-    const int ne = (L - e0) < kProjE1 ? (L - e0) : kProjE1;
+    const int ne = (L - e0) < p.pe1 ? (L - e0) : p.pe1;
     float* Cs = sp;  // [W][T][kCPad]
     const float* cg = tb.c1 + tb.c1_off[g];
     for (int e = threadIdx.x; e < W * T * kCPad; e += blockDim.x) {
@@ -426,7 +427,7 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
     }
 }
 
-// Stage 2 item: kProjE2 events; thread (event, feature) dots nf1*W inputs.
+// Stage 2 item: pe2 events; thread (event, feature) dots nf1*W inputs.
 __global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restrict__ reduced,
                                                            PcaPlan p, PcaTables tb,
                                                            float* __restrict__ features) {
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restri
     const int W = tb.W[g];
     const int R = p.nf1 * W;
     const int nf = p.nf2;
-    const int ne = (L - e0) < kProjE2 ? (L - e0) : kProjE2;
+    const int ne = (L - e0) < p.pe2 ? (L - e0) : p.pe2;
     float* Cs = sq;            // [R][nf]
     float* Xs = sq + R * nf;   // [ne][R]
     const float* cg = tb.c2 + tb.c2_off[g];
@@ -466,8 +467,12 @@ struct HostPlan {
     int32_t maxW = 0;
 };
 
+struct Tiles {
+    int32_t gram_chunk = 2560, pe1 = 64, pe2 = 64;
+};
+
 HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
-                   int32_t nf1, int32_t nf2, int64_t max_s) {
+                   int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl) {
     HostPlan h;
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = widths[g] > maxW ? widths[g] : maxW;
@@ -494,7 +499,7 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
         if (w2_global) h.w2_total += R * R;
         // Gram work items: the subsampled column count is min(L, max_s + 1)
         const int64_t ns_bound = L < max_s ? L : max_s + 1;
-        const int64_t nch = ns_bound > 0 ? ceil_div(ns_bound, kGramChunk) : 1;
+        const int64_t nch = ns_bound > 0 ? ceil_div(ns_bound, tl.gram_chunk) : 1;
         h.gram_first.push_back(static_cast<int32_t>(h.gram_items.size() / 2));
         for (int64_t c = 0; c < nch; ++c) {
             h.gram_items.push_back(g);
@@ -512,11 +517,11 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
             h.p1.push_back(g);
             h.p1.push_back(w);
         }
-        for (int64_t e = 0; e < L; e += kProjE1) {
+        for (int64_t e = 0; e < L; e += tl.pe1) {
             h.j1.push_back(g);
             h.j1.push_back(static_cast<int32_t>(e));
         }
-        for (int64_t e = 0; e < L; e += kProjE2) {
+        for (int64_t e = 0; e < L; e += tl.pe2) {
             h.j2.push_back(g);
             h.j2.push_back(static_cast<int32_t>(e));
         }
@@ -619,41 +624,95 @@ int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, 
     return MSAMD_OK;
 }
 
-}  // namespace
+constexpr int64_t kMaxLds = 160 * 1024;
 
-extern "C" int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events_host,
-                                            const int32_t* widths_host, int32_t T,
-                                            int32_t num_features_per_channel,
-                                            int32_t num_features, int64_t max_pca_samples) {
-    if (check_pca_args(G, num_events_host, widths_host, T, num_features_per_channel,
-                       num_features, max_pca_samples) != MSAMD_OK)
-        return -1;
-    const HostPlan h = host_plan(G, num_events_host, widths_host, T, num_features_per_channel,
-                                 num_features, max_pca_samples);
-    return pca_layout(nullptr, h, G, max_pca_samples + 2).bytes;
+// Dynamic LDS of the stage-2 projection for a tile choice.
+int64_t project2_lds(int32_t nf1, int32_t maxW, int32_t nf2, int32_t pe2) {
+    return static_cast<int64_t>(sizeof(float)) * (static_cast<int64_t>(nf1) * maxW * nf2 +
+                                                  static_cast<int64_t>(pe2) * nf1 * maxW);
 }
 
-extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_host,
-                                  const int32_t* widths_host, int32_t T,
-                                  int32_t num_features_per_channel, int32_t num_features,
-                                  int64_t max_pca_samples, float* reduced, float* features,
-                                  void* workspace, int64_t workspace_bytes, void* stream) {
-    clear_error();
-    const int rc = check_pca_args(G, num_events_host, widths_host, T, num_features_per_channel,
-                                  num_features, max_pca_samples);
+int check_tiles(const Tiles& t, int32_t nf1, int32_t maxW, int32_t nf2) {
+    MSAMD_REQUIRE(t.gram_chunk >= kKC && t.gram_chunk <= (1 << 20) && t.gram_chunk % kKC == 0,
+                  "pca tiles: gram_chunk %d must be a multiple of %d in [%d, 2^20]",
+                  t.gram_chunk, kKC, kKC);
+    MSAMD_REQUIRE(t.pe1 >= 1 && t.pe1 <= 4096, "pca tiles: stage-1 events per item %d", t.pe1);
+    MSAMD_REQUIRE(t.pe2 >= 1 && t.pe2 <= 4096 && project2_lds(nf1, maxW, nf2, t.pe2) <= kMaxLds,
+                  "pca tiles: stage-2 events per item %d exceed the LDS", t.pe2);
+    return MSAMD_OK;
+}
+
+Tiles tiles_from(const int32_t* t) {
+    Tiles r;
+    if (t) {
+        r.gram_chunk = t[0];
+        r.pe1 = t[1];
+        r.pe2 = t[2];
+    }
+    return r;
+}
+
+}  // namespace
+
+// A batched PCA over fixed neighborhood shapes: host tables, their device
+// copies in the caller's workspace, and the launch sequence.  Built once,
+// run many times; the run itself only enqueues kernels.
+struct msamd_pca_plan {
+    HostPlan h;
+    PcaPlan p{};
+    PcaLayout l;
+    Tiles tiles;
+    size_t lds_gram1 = 0, lds_power1 = 0, lds_proj1 = 0, lds_gram2 = 0, lds_power2 = 0,
+           lds_proj2 = 0;
+    // hipGraph of the launch sequence, keyed by the buffers it was captured with
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture = nullptr;
+    const float* g_clips = nullptr;
+    float* g_reduced = nullptr;
+    float* g_features = nullptr;
+};
+
+namespace {
+
+int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W, int32_t T,
+               int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl) {
+    int rc = check_pca_args(G, L, W, T, nf1, nf2, max_s);
     if (rc != MSAMD_OK) return rc;
-    const int32_t nf1 = num_features_per_channel, nf2 = num_features;
-    const HostPlan h = host_plan(G, num_events_host, widths_host, T, nf1, nf2, max_pca_samples);
-    const int64_t idx_stride = max_pca_samples + 2;
-    PcaLayout l = pca_layout(workspace, h, G, idx_stride);
+    int32_t maxW = 0;
+    for (int32_t g = 0; g < G; ++g) maxW = W[g] > maxW ? W[g] : maxW;
+    rc = check_tiles(tl, nf1, maxW, nf2);
+    if (rc != MSAMD_OK) return rc;
+    P->h = host_plan(G, L, W, T, nf1, nf2, max_s, tl);
+    P->tiles = tl;
+    P->p = PcaPlan{G, T, nf1, nf2, max_s, P->h.maxW, max_s + 2, tl.gram_chunk, tl.pe1, tl.pe2};
+    P->l = pca_layout(nullptr, P->h, G, max_s + 2);
+    const int n2 = nf1 * P->h.maxW;
+    P->lds_gram1 = sizeof(__bf16) * 2 * P->h.maxW * (T | 1) * kRowStride;
+    P->lds_power1 = sizeof(double) * (T * T + 2 * T + 16);
+    P->lds_proj1 = sizeof(float) * P->h.maxW * T * kCPad;
+    P->lds_gram2 = sizeof(__bf16) * 2 * (n2 | 1) * kRowStride;
+    P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * n2 : 0) + 2 * n2 + 16);
+    P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, tl.pe2));
+    return MSAMD_OK;
+}
+
+// Copies the host tables into the carved workspace (stream-ordered; the
+// tables live as long as the plan).
+int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hipStream_t st) {
+    const HostPlan& h = P->h;
+    P->l = pca_layout(workspace, h, P->p.G, P->p.idx_stride);
+    PcaLayout& l = P->l;
     if (!workspace || workspace_bytes < l.bytes) {
         set_error("pca_features: workspace too small (%lld < %lld)", (long long)workspace_bytes,
                   (long long)l.bytes);
         return MSAMD_ERR_WORKSPACE;
     }
-    hipStream_t st = as_stream(stream);
-    std::vector<double> sins(1024);
-    for (int i = 0; i < 1024; ++i) sins[i] = std::sin(static_cast<double>(i + 1));
+    static const std::vector<double> sins = [] {
+        std::vector<double> v(1024);
+        for (int i = 0; i < 1024; ++i) v[i] = std::sin(static_cast<double>(i + 1));
+        return v;
+    }();
     auto up = [&](void* dst, const void* src, size_t n) {
         return n ? hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st) : hipSuccess;
     };
@@ -669,68 +728,281 @@ extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* 
     MSAMD_HIP(up(l.d_p1, h.p1.data(), h.p1.size() * 4));
     MSAMD_HIP(up(l.d_j1, h.j1.data(), h.j1.size() * 4));
     MSAMD_HIP(up(l.d_j2, h.j2.data(), h.j2.size() * 4));
-    MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * 8));
+    MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * sizeof(double)));
     MSAMD_HIP(up(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8));
     MSAMD_HIP(up(l.d_gram_items, h.gram_items.data(), h.gram_items.size() * 4));
     MSAMD_HIP(up(l.d_gram_first, h.gram_first.data(), h.gram_first.size() * 4));
     MSAMD_HIP(up(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8));
     MSAMD_HIP(up(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8));
-    MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope after the call
-    PcaPlan p{G, T, nf1, nf2, max_pca_samples, h.maxW, idx_stride};
+    MSAMD_HIP(hipStreamSynchronize(st));  // pageable sources: copies complete before return
+    auto lds_attr = [](const void* f, size_t bytes) {
+        return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(bytes));
+    };
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<1>), P->lds_gram1));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_project1_kernel), P->lds_proj1));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<2>), P->lds_gram2));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_power_kernel<2>), P->lds_power2));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_project2_kernel), P->lds_proj2));
+    return MSAMD_OK;
+}
+
+// The launch sequence: nine kernels, each one launch over every neighborhood.
+int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, float* features,
+                 hipStream_t st) {
+    const HostPlan& h = P->h;
+    const PcaPlan& p = P->p;
+    const PcaLayout& l = P->l;
+    const int32_t G = p.G, T = p.T, nf1 = p.nf1;
     pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     const unsigned ngi = static_cast<unsigned>(h.gram_items.size() / 2);
-    {  // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
-        const size_t lds = sizeof(__bf16) * 2 * h.maxW * (T | 1) * kRowStride;
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<1>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_gram_kernel<1><<<ngi, kGramThreads, lds, st>>>(clips, p, l.tb, l.part);
-        MSAMD_LAUNCH_CHECK();
+    // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
+    pca_gram_kernel<1><<<ngi, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part);
+    MSAMD_LAUNCH_CHECK();
+    {
         const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(h.maxW) * T * T, 256)),
                       static_cast<unsigned>(G));
         pca_gram_reduce_kernel<1><<<rg, 256, 0, st>>>(p, l.tb, l.part);
         MSAMD_LAUNCH_CHECK();
     }
-    {  // stage 1 power method
-        const size_t lds = sizeof(double) * (T * T + 2 * T + 16);
-        pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, lds, st>>>(p, l.tb);
-        MSAMD_LAUNCH_CHECK();
-    }
+    // stage 1 power method
+    pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, P->lds_power1, st>>>(p, l.tb);
+    MSAMD_LAUNCH_CHECK();
     if (!h.j1.empty()) {  // stage 1 projections
-        const size_t lds = sizeof(float) * h.maxW * T * kCPad;
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_project1_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_project1_kernel<<<static_cast<unsigned>(h.j1.size() / 2), 256, lds, st>>>(clips, p, l.tb,
-                                                                                      reduced);
+        pca_project1_kernel<<<static_cast<unsigned>(h.j1.size() / 2), 256, P->lds_proj1, st>>>(
+            clips, p, l.tb, reduced);
         MSAMD_LAUNCH_CHECK();
     }
-    {  // stage 2 Gram
+    // stage 2 Gram
+    pca_gram_kernel<2><<<ngi, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
+    MSAMD_LAUNCH_CHECK();
+    {
         const int R = nf1 * h.maxW;
-        const size_t lds = sizeof(__bf16) * 2 * (R | 1) * kRowStride;
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_gram_kernel<2>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_gram_kernel<2><<<ngi, kGramThreads, lds, st>>>(reduced, p, l.tb, l.part);
-        MSAMD_LAUNCH_CHECK();
         const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(R) * R, 256)),
                       static_cast<unsigned>(G));
         pca_gram_reduce_kernel<2><<<rg, 256, 0, st>>>(p, l.tb, l.part);
         MSAMD_LAUNCH_CHECK();
     }
-    {  // stage 2 power method
-        const int n = nf1 * h.maxW;
-        const size_t lds = sizeof(double) * ((n <= kPowMaxN ? n * n : 0) + 2 * n + 16);
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_power_kernel<2>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_power_kernel<2><<<G, 256, lds, st>>>(p, l.tb);
-        MSAMD_LAUNCH_CHECK();
-    }
+    // stage 2 power method
+    pca_power_kernel<2><<<G, 256, P->lds_power2, st>>>(p, l.tb);
+    MSAMD_LAUNCH_CHECK();
     if (!h.j2.empty()) {  // stage 2 projections
-        const size_t lds = sizeof(float) * (nf1 * h.maxW * nf2 + kProjE2 * nf1 * h.maxW);
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pca_project2_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, lds, st>>>(reduced, p, l.tb,
-                                                                                      features);
+        pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, P->lds_proj2, st>>>(
+            reduced, p, l.tb, features);
         MSAMD_LAUNCH_CHECK();
     }
+    return MSAMD_OK;
+}
+
+void plan_free_graph(msamd_pca_plan* P) {
+    if (P->exec) hipGraphExecDestroy(P->exec);
+    if (P->graph) hipGraphDestroy(P->graph);
+    P->exec = nullptr;
+    P->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events_host,
+                                            const int32_t* widths_host, int32_t T,
+                                            int32_t num_features_per_channel,
+                                            int32_t num_features, int64_t max_pca_samples) {
+    return msamd_pca_plan_workspace_size(G, num_events_host, widths_host, T,
+                                         num_features_per_channel, num_features, max_pca_samples,
+                                         nullptr);
+}
+
+extern "C" int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_host,
+                                  const int32_t* widths_host, int32_t T,
+                                  int32_t num_features_per_channel, int32_t num_features,
+                                  int64_t max_pca_samples, float* reduced, float* features,
+                                  void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    msamd_pca_plan P;
+    int rc = plan_build(&P, G, num_events_host, widths_host, T, num_features_per_channel,
+                        num_features, max_pca_samples, Tiles{});
+    if (rc != MSAMD_OK) return rc;
+    hipStream_t st = as_stream(stream);
+    rc = plan_upload(&P, workspace, workspace_bytes, st);
+    if (rc != MSAMD_OK) return rc;
+    return plan_enqueue(&P, clips, reduced, features, st);
+}
+
+extern "C" int64_t msamd_pca_plan_workspace_size(int32_t G, const int64_t* num_events_host,
+                                                 const int32_t* widths_host, int32_t T,
+                                                 int32_t num_features_per_channel,
+                                                 int32_t num_features, int64_t max_pca_samples,
+                                                 const int32_t* tiles) {
+    msamd_pca_plan P;
+    if (plan_build(&P, G, num_events_host, widths_host, T, num_features_per_channel,
+                   num_features, max_pca_samples, tiles_from(tiles)) != MSAMD_OK)
+        return -1;
+    return P.l.bytes;
+}
+
+extern "C" int msamd_pca_plan_create(int32_t G, const int64_t* num_events_host,
+                                     const int32_t* widths_host, int32_t T,
+                                     int32_t num_features_per_channel, int32_t num_features,
+                                     int64_t max_pca_samples, const int32_t* tiles,
+                                     void* workspace, int64_t workspace_bytes, void* stream,
+                                     msamd_pca_plan** plan) {
+    clear_error();
+    MSAMD_REQUIRE(plan, "pca_plan_create: null plan pointer");
+    *plan = nullptr;
+    msamd_pca_plan* P = new msamd_pca_plan();
+    int rc = plan_build(P, G, num_events_host, widths_host, T, num_features_per_channel,
+                        num_features, max_pca_samples, tiles_from(tiles));
+    if (rc == MSAMD_OK) rc = plan_upload(P, workspace, workspace_bytes, as_stream(stream));
+    if (rc != MSAMD_OK) {
+        delete P;
+        return rc;
+    }
+    *plan = P;
+    return MSAMD_OK;
+}
+
+extern "C" int msamd_pca_plan_run(msamd_pca_plan* plan, const float* clips, float* reduced,
+                                  float* features, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(plan && clips && reduced && features, "pca_plan_run: null argument");
+    return plan_enqueue(plan, clips, reduced, features, as_stream(stream));
+}
+
+extern "C" int msamd_pca_plan_launch_graph(msamd_pca_plan* plan, const float* clips,
+                                           float* reduced, float* features, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(plan && clips && reduced && features, "pca_plan_launch_graph: null argument");
+    msamd_pca_plan* P = plan;
+    if (!P->exec || P->g_clips != clips || P->g_reduced != reduced || P->g_features != features) {
+        plan_free_graph(P);
+        if (!P->capture) MSAMD_HIP(hipStreamCreateWithFlags(&P->capture, hipStreamNonBlocking));
+        MSAMD_HIP(hipStreamBeginCapture(P->capture, hipStreamCaptureModeThreadLocal));
+        const int rc = plan_enqueue(P, clips, reduced, features, P->capture);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(P->capture, &g);
+        if (rc != MSAMD_OK) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        MSAMD_HIP(ec);
+        P->graph = g;
+        MSAMD_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+        P->g_clips = clips;
+        P->g_reduced = reduced;
+        P->g_features = features;
+    }
+    MSAMD_HIP(hipGraphLaunch(P->exec, as_stream(stream)));
+    return MSAMD_OK;
+}
+
+extern "C" void msamd_pca_plan_tiles(const msamd_pca_plan* plan, int32_t* tiles) {
+    if (!plan || !tiles) return;
+    tiles[0] = plan->tiles.gram_chunk;
+    tiles[1] = plan->tiles.pe1;
+    tiles[2] = plan->tiles.pe2;
+}
+
+extern "C" void msamd_pca_plan_destroy(msamd_pca_plan* plan) {
+    if (!plan) return;
+    plan_free_graph(plan);
+    if (plan->capture) hipStreamDestroy(plan->capture);
+    delete plan;
+}
+
+// ------------------------------------------------------------- autotuner
+namespace {
+
+constexpr int32_t kTuneGram[] = {1280, 2560, 5120, 10240};
+constexpr int32_t kTunePe1[] = {32, 64, 128, 256};
+constexpr int32_t kTunePe2[] = {16, 32, 64, 128};
+
+template <typename F>
+void for_each_candidate(const Tiles& base, F&& f) {
+    for (int32_t v : kTuneGram) { Tiles t = base; t.gram_chunk = v; f(0, t); }
+    for (int32_t v : kTunePe1) { Tiles t = base; t.pe1 = v; f(1, t); }
+    for (int32_t v : kTunePe2) { Tiles t = base; t.pe2 = v; f(2, t); }
+}
+
+}  // namespace
+
+extern "C" int64_t msamd_pca_autotune_workspace_size(int32_t G, const int64_t* num_events_host,
+                                                     const int32_t* widths_host, int32_t T,
+                                                     int32_t num_features_per_channel,
+                                                     int32_t num_features,
+                                                     int64_t max_pca_samples) {
+    int64_t best = -1;
+    // every candidate of every coordinate, on top of any base the search can reach
+    for (int32_t gc : kTuneGram)
+        for (int32_t e1 : kTunePe1)
+            for (int32_t e2 : kTunePe2) {
+                const int32_t t[3] = {gc, e1, e2};
+                const int64_t b = msamd_pca_plan_workspace_size(
+                    G, num_events_host, widths_host, T, num_features_per_channel, num_features,
+                    max_pca_samples, t);
+                if (b < 0) continue;  // does not fit this shape: the search skips it too
+                best = b > best ? b : best;
+            }
+    return best;
+}
+
+extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_host,
+                                  const int32_t* widths_host, int32_t T,
+                                  int32_t num_features_per_channel, int32_t num_features,
+                                  int64_t max_pca_samples, float* reduced, float* features,
+                                  void* workspace, int64_t workspace_bytes, void* stream,
+                                  int32_t* tiles_out, float* best_ms_out) {
+    clear_error();
+    MSAMD_REQUIRE(tiles_out, "pca_autotune: null tiles_out");
+    hipStream_t st = as_stream(stream);
+    hipEvent_t e0, e1;
+    MSAMD_HIP(hipEventCreate(&e0));
+    MSAMD_HIP(hipEventCreate(&e1));
+    Tiles best{};
+    float best_ms = -1.f;
+    int rc = MSAMD_OK;
+    // coordinate search: each tile dimension in turn, the others at their best so far
+    for (int dim = 0; dim < 3 && rc == MSAMD_OK; ++dim) {
+        Tiles base = best;
+        Tiles dim_best = best;
+        float dim_ms = -1.f;
+        for_each_candidate(base, [&](int d, const Tiles& t) {
+            if (d != dim || rc != MSAMD_OK) return;
+            msamd_pca_plan P;
+            int32_t maxW = 0;
+            for (int32_t g = 0; g < G; ++g) maxW = widths_host[g] > maxW ? widths_host[g] : maxW;
+            if (check_tiles(t, num_features_per_channel, maxW, num_features) != MSAMD_OK) {
+                clear_error();  // candidate does not fit this shape: skipped
+                return;
+            }
+            rc = plan_build(&P, G, num_events_host, widths_host, T, num_features_per_channel,
+                            num_features, max_pca_samples, t);
+            if (rc == MSAMD_OK) rc = plan_upload(&P, workspace, workspace_bytes, st);
+            float ms = 0.f;
+            for (int rep = 0; rep < 2 && rc == MSAMD_OK; ++rep) {  // second run timed
+                if (hipEventRecord(e0, st) != hipSuccess) { rc = MSAMD_ERR_HIP; break; }
+                rc = plan_enqueue(&P, clips, reduced, features, st);
+                if (rc != MSAMD_OK) break;
+                if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    set_error("pca_autotune: event timing failed");
+                    rc = MSAMD_ERR_HIP;
+                }
+            }
+            if (rc == MSAMD_OK && (dim_ms < 0.f || ms < dim_ms)) {
+                dim_ms = ms;
+                dim_best = t;
+            }
+        });
+        best = dim_best;
+        best_ms = dim_ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != MSAMD_OK) return rc;
+    tiles_out[0] = best.gram_chunk;
+    tiles_out[1] = best.pe1;
+    tiles_out[2] = best.pe2;
+    if (best_ms_out) *best_ms_out = best_ms;
     return MSAMD_OK;
 }

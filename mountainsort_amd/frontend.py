@@ -220,6 +220,120 @@ def pca_features(clips_flat, clip_offsets, event_offsets, nbhd_offsets, clip_siz
     return reduced, red_off, features, feat_off
 
 
+def _pca_shapes(num_events, widths):
+    import numpy as np
+    L = np.ascontiguousarray(np.asarray(num_events, dtype=np.int64).ravel())
+    W = np.ascontiguousarray(np.asarray(widths, dtype=np.int32).ravel())
+    if L.shape != W.shape or L.size == 0:
+        raise ValueError("num_events and widths must be non-empty and of one length")
+    return L, W
+
+
+class PcaPlan:
+    """A reusable batched two-stage PCA over fixed neighborhood shapes
+    (msamd_pca_plan_*): the host tables are built and uploaded once, and
+    run() only enqueues the batched kernels -- or, with graph=True, replays
+    them as one captured HIP graph.
+
+    num_events[g], widths[g]: events and channels of neighborhood g; clips
+    are packed as extract_clips_batched returns them.  tiles: (Gram columns
+    per item, stage-1 events per item, stage-2 events per item) or None for
+    the defaults (see pca_autotune).  Outputs live in the plan (reduced,
+    features) and are overwritten by every run.
+    """
+
+    def __init__(self, num_events, widths, clip_size, num_features_per_channel=10,
+                 num_features=10, max_pca_samples=10000, tiles=None, device=None):
+        import numpy as np
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        L, W = _pca_shapes(num_events, widths)
+        self.G = int(L.size)
+        self.T, self.nf1, self.nf2 = int(clip_size), int(num_features_per_channel), int(num_features)
+        self.max_s = int(max_pca_samples)
+        self._tiles_arg = None
+        if tiles is not None:
+            self._tiles_arg = np.ascontiguousarray(np.asarray(tiles, dtype=np.int32).ravel())
+            if self._tiles_arg.size != 3:
+                raise ValueError("tiles must hold three integers")
+        pt = self._tiles_arg.ctypes.data_as(ctypes.c_void_p) if self._tiles_arg is not None else None
+        pL, pW = L.ctypes.data_as(ctypes.c_void_p), W.ctypes.data_as(ctypes.c_void_p)
+        lib = _capi.lib()
+        nb = lib.msamd_pca_plan_workspace_size(self.G, pL, pW, self.T, self.nf1, self.nf2,
+                                               self.max_s, pt)
+        if nb < 0:
+            raise _capi.MsamdError("msamd_pca_plan_workspace_size", 1,
+                                   "unsupported neighborhood shapes or tiles")
+        self._ws = _workspace(nb, self.device)
+        self.clip_elems = int((W.astype(np.int64) * self.T * L).sum())
+        self.red_off = np.concatenate([[0], np.cumsum(W.astype(np.int64) * self.nf1 * L)])
+        self.feat_off = np.concatenate([[0], np.cumsum(self.nf2 * L)])
+        self.reduced = torch.empty(max(int(self.red_off[-1]), 1), dtype=torch.float32,
+                                   device=self.device)
+        self.features = torch.empty(max(int(self.feat_off[-1]), 1), dtype=torch.float32,
+                                    device=self.device)
+        self._plan = ctypes.c_void_p()
+        _capi.call("msamd_pca_plan_create", self.G, pL, pW, self.T, self.nf1, self.nf2,
+                   self.max_s, pt, _ptr(self._ws), self._ws.numel(), _stream(self.device),
+                   ctypes.addressof(self._plan))
+
+    @property
+    def tiles(self):
+        import numpy as np
+        t = np.zeros(3, dtype=np.int32)
+        _capi.lib().msamd_pca_plan_tiles(self._plan, t.ctypes.data_as(ctypes.c_void_p))
+        return tuple(int(x) for x in t)
+
+    def run(self, clips_flat, graph=False):
+        """Enqueue the PCA of clips_flat; returns (reduced, red_off, features, feat_off)."""
+        if self._plan is None or not self._plan.value:
+            raise RuntimeError("PcaPlan is closed")
+        if (clips_flat.dtype != torch.float32 or clips_flat.device != self.device
+                or not clips_flat.is_contiguous() or clips_flat.numel() < self.clip_elems):
+            raise ValueError("clips must be a contiguous float32 tensor of the plan's shapes "
+                             "on the plan's device")
+        fn = "msamd_pca_plan_launch_graph" if graph else "msamd_pca_plan_run"
+        _capi.call(fn, self._plan, _ptr(clips_flat), _ptr(self.reduced), _ptr(self.features),
+                   _stream(self.device))
+        return self.reduced, self.red_off, self.features, self.feat_off
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _capi.lib().msamd_pca_plan_destroy(self._plan)
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pca_autotune(clips_flat, num_events, widths, clip_size, num_features_per_channel=10,
+                 num_features=10, max_pca_samples=10000):
+    """Choose the PCA plan's tiles for this workload by timing candidates on
+    the device (msamd_pca_autotune).  Returns (tiles, milliseconds)."""
+    import numpy as np
+    dev = clips_flat.device
+    L, W = _pca_shapes(num_events, widths)
+    G, T = int(L.size), int(clip_size)
+    nf1, nf2 = int(num_features_per_channel), int(num_features)
+    pL, pW = L.ctypes.data_as(ctypes.c_void_p), W.ctypes.data_as(ctypes.c_void_p)
+    lib = _capi.lib()
+    nb = lib.msamd_pca_autotune_workspace_size(G, pL, pW, T, nf1, nf2, int(max_pca_samples))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_pca_autotune_workspace_size", 1, "unsupported shapes")
+    ws = _workspace(nb, dev)
+    red = torch.empty(max(int((W.astype(np.int64) * nf1 * L).sum()), 1), dtype=torch.float32,
+                      device=dev)
+    feat = torch.empty(max(int((nf2 * L).sum()), 1), dtype=torch.float32, device=dev)
+    tiles = np.zeros(3, dtype=np.int32)
+    ms = ctypes.c_float()
+    _capi.call("msamd_pca_autotune", _ptr(clips_flat), G, pL, pW, T, nf1, nf2,
+               int(max_pca_samples), _ptr(red), _ptr(feat), _ptr(ws), ws.numel(),
+               _stream(dev), tiles.ctypes.data_as(ctypes.c_void_p), ctypes.addressof(ms))
+    return tuple(int(x) for x in tiles), float(ms.value)
+
+
 # --------------------------------------------------------------------------
 # whitening (a4-a6)
 # --------------------------------------------------------------------------

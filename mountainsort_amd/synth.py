@@ -170,3 +170,48 @@ def generate(cfg, device="cpu", block=1 << 20, return_truth=False):
     if return_truth:
         return X, {"spike_times": spikes, "waveforms": wave, "gains": gains}
     return X
+
+
+# ---------------------------------------------------------------------------
+# Clip batches for per-neighborhood PCA
+# ---------------------------------------------------------------------------
+def pca_batch_shapes(G=384, seed=3, w_lo=5, w_hi=12, l_lo=100, l_hi=200_000):
+    """Neighborhood shapes of the batched PCA workload: widths uniform in
+    [w_lo, w_hi] channels and event counts log-uniform in [l_lo, l_hi].
+    Returns (num_events int64 (G,), widths int32 (G,))."""
+    rng = np.random.default_rng(seed)
+    W = rng.integers(w_lo, w_hi + 1, size=G).astype(np.int32)
+    L = np.round(np.exp(rng.uniform(np.log(l_lo), np.log(l_hi), size=G))).astype(np.int64)
+    return L, W
+
+
+def pca_batch_clips(num_events, widths, T=50, seed=3, units=4, noise_sigma=1.0, device="cpu"):
+    """Packed clips for the batched PCA (the extract_clips_batched layout:
+    neighborhood g holds L_g x T x W_g floats, element (i, t, w) at
+    i*T*W + t*W + w).  Each neighborhood has `units` biphasic waveforms
+    (the recording model above, amplitudes 3-12 noise sigmas) with spatial
+    gains U(0.1, 1); a clip is one of them at amplitude U(0.7, 1.3) plus
+    white noise.  Templates and labels come from numpy (seeded); the bulk
+    noise is drawn on `device` with torch's generator seeded from `seed`.
+    Returns (clips_flat float32, clip_offsets int64 (G+1,))."""
+    rng = np.random.default_rng(seed + 1000)
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    L = np.asarray(num_events, dtype=np.int64)
+    W = np.asarray(widths, dtype=np.int64)
+    sizes = L * T * W
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    out = torch.empty(max(int(off[-1]), 1), dtype=torch.float32, device=dev)
+    for g in range(len(L)):
+        if L[g] == 0:
+            continue
+        wave = _waveforms(rng, units, T, 3.0 * noise_sigma, 12.0 * noise_sigma)  # (K, T)
+        gain = rng.uniform(0.1, 1.0, size=(units, W[g]))
+        temps = torch.from_numpy((wave[:, :, None] * gain[:, None, :]).astype(np.float32)).to(dev)
+        lab = torch.from_numpy(rng.integers(0, units, L[g])).to(dev)
+        amp = torch.from_numpy(rng.uniform(0.7, 1.3, L[g]).astype(np.float32)).to(dev)
+        seg = out[off[g]:off[g + 1]].view(int(L[g]), T, int(W[g]))
+        torch.randn(seg.shape, generator=gen, device=dev, out=seg)
+        seg.mul_(noise_sigma).add_(temps[lab] * amp[:, None, None])
+    return out, off

@@ -58,3 +58,71 @@ def test_pca_features_match_oracle(gpu):
         worst1, worst2 = max(worst1, e1a), max(worst2, e2)
         assert e1a < 1e-3
     assert worst1 < 1e-3
+
+
+def test_pca_plan_graph_and_per_neighborhood_match(gpu):
+    """The reusable plan, its HIP-graph replay, one plan per neighborhood and
+    the one-call entry point give bit-identical features (every neighborhood's
+    work items are the same in all four), and the graph stays correct over
+    replays with other device work in between."""
+    from mountainsort_amd import frontend as fe
+    from mountainsort_amd import synth
+    L, W = synth.pca_batch_shapes(G=24, seed=11, l_hi=30_000)
+    L[3] = 0
+    say(f"pca plan: G=24, events {int(L.sum())}, widths {W.min()}-{W.max()}")
+    clips, co = synth.pca_batch_clips(L, W, seed=11, device=gpu)
+    eo = np.concatenate([[0], np.cumsum(L)])
+    no = np.concatenate([[0], np.cumsum(W)])
+    red1, ro, feat1, fo = fe.pca_features(clips, co, eo, no, 50)
+    red1, feat1 = red1.clone(), feat1.clone()
+    plan = fe.PcaPlan(L, W, 50, device=gpu)
+    assert plan.tiles == (2560, 64, 64)
+    red2, ro2, feat2, fo2 = plan.run(clips)
+    assert np.array_equal(ro, ro2) and np.array_equal(fo, fo2)
+    assert torch.equal(red1, red2) and torch.equal(feat1, feat2)
+    for rep in range(3):
+        plan.reduced.fill_(float("nan"))
+        plan.features.fill_(float("nan"))
+        torch.cuda.synchronize()
+        _, _, feat3, _ = plan.run(clips, graph=True)
+        torch.cuda.synchronize()
+        say(f"  graph replay {rep}")
+        assert torch.equal(plan.reduced, red1) and torch.equal(feat3, feat1)
+        junk = torch.randn(1 << 20, device=gpu).sum()  # eager work between replays
+        assert torch.isfinite(junk)
+    plan.close()
+    for g in range(len(L)):
+        p1 = fe.PcaPlan(L[g:g + 1], W[g:g + 1], 50, device=gpu)
+        seg = clips[co[g]:co[g + 1]].contiguous() if L[g] else torch.zeros(1, device=gpu)
+        r, _, f, _ = p1.run(seg)
+        assert torch.equal(r[:ro[g + 1] - ro[g]], red1[ro[g]:ro[g + 1]])
+        assert torch.equal(f[:fo[g + 1] - fo[g]], feat1[fo[g]:fo[g + 1]])
+        p1.close()
+
+
+def test_pca_autotune_tiles_keep_parity(gpu):
+    """The autotuner returns tiles from its candidate sets, and a plan with
+    them still matches the CPU oracle on the first components."""
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(5)
+    shapes = [(20000, 7), (700, 12), (3000, 5)]
+    say(f"pca autotune on (L, W) = {shapes}")
+    host = [_clips(rng, L, W) for L, W in shapes]
+    flat = torch.from_numpy(np.concatenate([c.ravel() for c in host])).to(gpu)
+    L = np.array([s[0] for s in shapes])
+    W = np.array([s[1] for s in shapes])
+    tiles, ms = fe.pca_autotune(flat, L, W, 50)
+    say(f"  tiles {tiles}, {ms:.3f} ms")
+    assert tiles[0] in (1280, 2560, 5120, 10240)
+    assert tiles[1] in (32, 64, 128, 256) and tiles[2] in (16, 32, 64, 128)
+    assert ms > 0
+    plan = fe.PcaPlan(L, W, 50, tiles=tiles, device=gpu)
+    red, ro, feat, fo = plan.run(flat)
+    red = red.cpu().numpy()
+    for g, ((Lg, Wg), c) in enumerate(zip(shapes, host)):
+        r_o = oc.dimension_reduce_clips(c, 10, 10000)
+        r_g = red[ro[g]:ro[g + 1]].reshape(Lg, 10, Wg)
+        e1a = np.max(np.abs(r_g[:, :3] - r_o[:, :3])) / np.max(np.abs(r_o[:, :3]))
+        say(f"  g={g}: stage1 first-3 max rel {e1a:.2e}")
+        assert e1a < 1e-3
+    plan.close()
