@@ -396,6 +396,8 @@ def main():
     ap.add_argument("--seconds", type=float, default=300.0, help="recording length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-pca-batching", action="store_true",
+                    help="leave out the batched-PCA leg (it runs after the timed steps)")
     ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS),
                     help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
                          "elementwise negation (counter calibration) and the named stage, "
@@ -498,7 +500,8 @@ def main():
     }
     if not args.no_learned:
         line["learned"] = learned_leg(dev, pipe, cfg)
-    line["pca_batching"] = pca_batching_leg(dev)
+    if not args.no_pca_batching:
+        line["pca_batching"] = pca_batching_leg(dev)
     if not args.no_cpu_baseline:
         widths = pipe.widths
         line["cpu_baseline"] = cpu_baseline(cfg.M, N, pipe.stats.get("events", 0), widths)

@@ -130,13 +130,28 @@ def main_traffic(argv):
 
 def main_all(argv):
     """python -m mountainsort_amd.profiling pmc-all FETCH.db WRITE.db OUT.md TITLE
-    -> per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch, raw counter
-    values) and durations of every msamd kernel in the runs."""
-    fetch_db, write_db, out, title = argv
+    ["fetch command" "write command"] -> per-kernel average FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch, raw counter values) and durations of every msamd kernel
+    in the runs, with the calibration factors of the negation kernel."""
+    fetch_db, write_db, out, title = argv[:4]
+    cmds = argv[4:6]
     fa, fdur = pmc_counters(fetch_db)
     wa, _ = pmc_counters(write_db)
-    lines = [f"# {title}", "", "| kernel | dispatches | avg us | FETCH_SIZE KiB | WRITE_SIZE KiB |",
-             "|---|---:|---:|---:|---:|"]
+    lines = [f"# {title}", ""]
+    if cmds:
+        lines += ["Commands:", "", "```", *cmds, "```", ""]
+    calib = {}
+    for agg, counter in ((fa, "FETCH_SIZE"), (wa, "WRITE_SIZE")):
+        vals = [v for k, c in agg.items() if "neg" in k.lower() and "msamd" not in k
+                for v in c.get(counter, [])]
+        calib[counter] = sum(vals) / len(vals) if vals else float("nan")
+    lines += [f"Calibration: a 1 GiB elementwise negation (reads and writes 1 GiB) reads "
+              f"FETCH_SIZE {calib['FETCH_SIZE']:.6g} KiB and WRITE_SIZE "
+              f"{calib['WRITE_SIZE']:.6g} KiB, so read bytes = FETCH_SIZE x "
+              f"{1048576 / calib['FETCH_SIZE']:.4g} KiB and write bytes = WRITE_SIZE x "
+              f"{1048576 / calib['WRITE_SIZE']:.4g} KiB.", ""]
+    lines += ["| kernel | dispatches | avg us | FETCH_SIZE KiB | WRITE_SIZE KiB |",
+              "|---|---:|---:|---:|---:|"]
     names = sorted((k for k in fdur if "msamd" in k), key=lambda k: -sum(fdur[k]))
     for k in names:
         f = fa.get(k, {}).get("FETCH_SIZE", [])
