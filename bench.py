@@ -229,6 +229,39 @@ def learned_leg(dev, pipe, cfg, reps=3, train_seconds=60.0, steps=4000):
     return out
 
 
+def exact_whitening_leg(pipe, reps=3):
+    """The exact whitening mode on the benchmark's covariance: W = C^{-1/2}
+    (FP64 Newton-Schulz on the GPU) against the reference-method W of the
+    timed steps.  Reports the deviation, both matrices' whitening error
+    |W C W - I|_F / sqrt(M), iterations and time."""
+    import numpy as np
+    import torch
+    from mountainsort_amd import frontend as fe
+    C = pipe.C
+    ms = []
+    for _ in range(reps + 1):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        We, (iters, resid) = fe.whitening_matrix(C, exact=True, return_info=True)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    Cn = C.cpu().numpy()
+    Wr = pipe.W.cpu().numpy()
+    We = We.cpu().numpy()
+    M = Cn.shape[0]
+    I = np.eye(M)
+    out = {
+        "ms": round(float(np.median(ms[1:])), 3), "iterations": iters, "residual": resid,
+        "deviation_from_reference_method": float(np.linalg.norm(We - Wr) / np.linalg.norm(Wr)),
+        "whitening_error_exact": float(np.linalg.norm(We @ Cn @ We - I) / np.sqrt(M)),
+        "whitening_error_reference_method": float(np.linalg.norm(Wr @ Cn @ Wr - I) / np.sqrt(M)),
+    }
+    log("[bench] exact whitening: " + json.dumps(out))
+    return out
+
+
 def pca_batching_leg(dev, reps=3):
     """Batched PCA workload: 384 neighborhoods of 5-12 channels with
     log-uniform 100-200000 events (synth.pca_batch_shapes seed 3), 50-sample
@@ -500,6 +533,7 @@ def main():
     }
     if not args.no_learned:
         line["learned"] = learned_leg(dev, pipe, cfg)
+    line["whitening_exact"] = exact_whitening_leg(pipe)
     if not args.no_pca_batching:
         line["pca_batching"] = pca_batching_leg(dev)
     if not args.no_cpu_baseline:

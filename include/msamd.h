@@ -243,6 +243,22 @@ int64_t msamd_whitening_matrix_workspace_size(int64_t M);
 int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* workspace,
                            int64_t workspace_bytes, void* stream);
 
+/* Exact mode of the whitening matrix (no reference counterpart; the
+ * reference's W in whitening_matrix_from_XXt, packages/mountainsortalg/
+ * pca.cpp:634-650, approximates it with unconverged power steps):
+ * W = C^{-1/2}, the symmetric inverse square root of XXt, by the coupled
+ * Newton-Schulz iteration in FP64 on the GPU (tiled FP64 GEMMs).  Channels
+ * with a zero diagonal get zero rows and columns.  Stops when
+ * ||(3I - ZY)/2 - I||_F < 1e-12 sqrt(M) or at rounding level; returns
+ * MSAMD_ERR_UNSUPPORTED if XXt is not positive definite on its nonzero
+ * channels or does not converge in 100 iterations.  iterations_out and
+ * residual_out (nullable) receive the iteration count and final residual.
+ * Synchronizes the stream every six iterations.  M <= 4096. */
+int64_t msamd_whitening_matrix_exact_workspace_size(int64_t M);
+int msamd_whitening_matrix_exact(const double* XXt, int64_t M, double* W, void* workspace,
+                                 int64_t workspace_bytes, void* stream, int32_t* iterations_out,
+                                 double* residual_out);
+
 /* Workspace bytes for msamd_whiten_apply. */
 int64_t msamd_whiten_apply_workspace_size(int64_t M);
 

@@ -47,6 +47,8 @@ SIGNATURES = {
     "msamd_whiten_covariance": (cint, [vp, i64, i64, vp, vp, i64, vp]),
     "msamd_whitening_matrix_workspace_size": (i64, [i64]),
     "msamd_whitening_matrix": (cint, [vp, i64, vp, vp, i64, vp]),
+    "msamd_whitening_matrix_exact_workspace_size": (i64, [i64]),
+    "msamd_whitening_matrix_exact": (cint, [vp, i64, vp, vp, i64, vp, vp, vp]),
     "msamd_whiten_apply_workspace_size": (i64, [i64]),
     "msamd_whiten_apply": (cint, [vp, i64, i64, vp, vp, f64, vp, vp, i64, vp]),
     "msamd_mask_workspace_size": (i64, [i64, i64, i64]),

@@ -350,14 +350,27 @@ def whiten_covariance(X):
     return C
 
 
-def whitening_matrix(C):
-    """W from the covariance by the reference's deflated power method (pca.cpp:634-650)."""
+def whitening_matrix(C, exact=False, return_info=False):
+    """W from the covariance.
+
+    exact=False: the reference's deflated power method (pca.cpp:634-650).
+    exact=True: W = C^{-1/2}, the symmetric inverse square root the power
+    method approximates (coupled Newton-Schulz in FP64; zero rows and columns
+    for channels with a zero diagonal).  With return_info the exact mode also
+    returns (iterations, residual)."""
     M = C.shape[0]
     dev = C.device
     Ccm = C.to(torch.float64).t().contiguous()
     L = _capi.lib()
-    ws = _workspace(L.msamd_whitening_matrix_workspace_size(M), dev)
     Wcm = torch.empty((M, M), dtype=torch.float64, device=dev)
+    if exact:
+        ws = _workspace(L.msamd_whitening_matrix_exact_workspace_size(M), dev)
+        iters, resid = ctypes.c_int32(), ctypes.c_double()
+        _capi.call("msamd_whitening_matrix_exact", _ptr(Ccm), M, _ptr(Wcm), _ptr(ws),
+                   ws.numel(), _stream(dev), ctypes.addressof(iters), ctypes.addressof(resid))
+        W = Wcm.t()
+        return (W, (int(iters.value), float(resid.value))) if return_info else W
+    ws = _workspace(L.msamd_whitening_matrix_workspace_size(M), dev)
     _capi.call("msamd_whitening_matrix", _ptr(Ccm), M, _ptr(Wcm), _ptr(ws), ws.numel(),
                _stream(dev))
     return Wcm.t()  # [row, col] view of the column-major result

@@ -192,3 +192,51 @@ def test_mask_bit_exact(gpu, M, N, isz):
     Yg, ug = fe.mask_out_artifacts(torch.from_numpy(X).to(gpu), 6.0, isz, return_use_it=True)
     np.testing.assert_array_equal(ug.cpu().numpy(), uo)
     np.testing.assert_array_equal(Yg.cpu().numpy(), Yo)
+
+
+def _spd(rng, M, cond):
+    """Random SPD matrix with eigenvalues log-spaced over [1, cond]."""
+    Q, _ = np.linalg.qr(rng.normal(size=(M, M)))
+    w = np.exp(rng.uniform(0, np.log(cond), M))
+    return (Q * w) @ Q.T
+
+
+@pytest.mark.parametrize("M,cond", [(4, 10.0), (64, 1e3), (130, 1e4), (384, 1e4)])
+def test_whitening_matrix_exact_is_inverse_sqrt(gpu, M, cond):
+    """Exact mode: W = C^{-1/2} (closed form from numpy's eigh) and W C W = I."""
+    from mountainsort_amd import frontend as fe
+    say(f"exact whitening matrix M={M} cond={cond:g}")
+    rng = np.random.default_rng(M)
+    C = _spd(rng, M, cond) * 37.0
+    w, V = np.linalg.eigh(C)
+    W_ref = (V / np.sqrt(w)) @ V.T
+    Wg, (iters, resid) = fe.whitening_matrix(torch.from_numpy(C).to(gpu), exact=True,
+                                             return_info=True)
+    Wg = Wg.cpu().numpy()
+    err = np.linalg.norm(Wg - W_ref) / np.linalg.norm(W_ref)
+    say(f"  {iters} iterations, residual {resid:.2e}, |W - C^-1/2| / |C^-1/2| = {err:.2e}")
+    assert err < 1e-10
+    assert np.allclose(Wg, Wg.T, rtol=0, atol=1e-14 * np.abs(Wg).max())
+    assert np.max(np.abs(Wg @ C @ Wg - np.eye(M))) < 1e-9
+
+
+def test_whitening_matrix_exact_zero_channels_and_singular(gpu):
+    """Channels with a zero diagonal get zero rows/columns; a singular
+    covariance on the nonzero channels is an error, not a wrong W."""
+    from mountainsort_amd import _capi
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(9)
+    M = 12
+    C = np.zeros((M, M))
+    act = np.array([0, 1, 2, 4, 5, 7, 8, 9, 11])
+    C[np.ix_(act, act)] = _spd(rng, len(act), 100.0)
+    Wg = fe.whitening_matrix(torch.from_numpy(C).to(gpu), exact=True).cpu().numpy()
+    w, V = np.linalg.eigh(C[np.ix_(act, act)])
+    Wa = (V / np.sqrt(w)) @ V.T
+    assert np.linalg.norm(Wg[np.ix_(act, act)] - Wa) / np.linalg.norm(Wa) < 1e-10
+    inactive = np.setdiff1d(np.arange(M), act)
+    assert np.all(Wg[inactive, :] == 0) and np.all(Wg[:, inactive] == 0)
+    x = rng.normal(size=(M, 3))
+    S = x @ x.T  # rank 3, every diagonal entry nonzero
+    with pytest.raises(_capi.MsamdError):
+        fe.whitening_matrix(torch.from_numpy(S).to(gpu), exact=True)
