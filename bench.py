@@ -6,7 +6,8 @@ artifact bursts on 1% of samples (mountainsort_amd.synth.config2, seed 2),
 generated on the GPU and resident in HBM before timing.  One step is one
 pass of the whole hot path over the recording:
   bandpass 300-6000 Hz (reference windows, 8-thread chunk plan)
-  -> mask_out_artifacts (threshold 6, interval 2000)
+  -> mask_out_artifacts (threshold 6, interval 2000; the decisions, with the
+     masked copy fused into the covariance and apply reads)
   -> whiten (covariance, whitening matrix, apply)
   -> detect (threshold 3, sign -1, interval 10, 8-thread chunk plan)
   -> clips (T = 50, neighborhoods of radius 40 um)
@@ -62,7 +63,6 @@ class Pipeline:
         self.widths = [len(c) for c in nb]
         # preallocated stage outputs
         self.filt = torch.empty((N, M), dtype=torch.float32, device=dev)
-        self.masked = torch.empty((N, M), dtype=torch.float32, device=dev)
         self.white = torch.empty((N, M), dtype=torch.float32, device=dev)
         self.stats = {}
 
@@ -79,14 +79,17 @@ class Pipeline:
         mark("start")
         fe.bandpass_filter(raw, 30000, 300, 6000, 1000, num_threads=self.num_threads, out=self.filt)
         mark("bandpass")
-        fe.mask_out_artifacts(self.filt, 6.0, 2000, out=self.masked)
+        # mask decisions only: the covariance and the apply read the filtered
+        # data with masked rows as zero, so the masked copy is never written
+        rows, _ = fe.mask_rows(self.filt, 6.0, 2000)
+        self.rows = rows
         mark("mask")
-        C = fe.whiten_covariance(self.masked)
+        C = fe.whiten_covariance(self.filt, row_keep=rows)
         mark("covariance")
         W = fe.whitening_matrix(C)
         mark("whitening_matrix")
         self.C, self.W = C, W
-        fe.whiten_apply(self.masked, W, out=self.white)
+        fe.whiten_apply(self.filt, W, out=self.white, row_keep=rows)
         mark("apply")
         offsets, times = fe.detect_events(self.white, 3.0, 10, -1, num_threads=self.num_threads)
         mark("detect")
@@ -463,9 +466,9 @@ def main():
             torch.neg(src, out=dst)
         for _ in range(3):
             if args.pmc_pass == "apply":
-                fe.whiten_apply(pipe.masked, pipe.W, out=pipe.white)
+                fe.whiten_apply(pipe.filt, pipe.W, out=pipe.white, row_keep=pipe.rows)
             elif args.pmc_pass == "covariance":
-                fe.whiten_covariance(pipe.masked)
+                fe.whiten_covariance(pipe.filt, row_keep=pipe.rows)
             else:
                 fe.whitening_matrix(pipe.C)
         torch.cuda.synchronize()
@@ -493,8 +496,10 @@ def main():
     stage_ms = {k: float(np.mean(v)) for k, v in timers.items()}
     log("[bench] stage ms: " + json.dumps({k: round(v, 3) for k, v in stage_ms.items()}))
 
-    # algorithmic bytes per channel-sample of each stage (input once, output once)
-    alg_bytes = {"bandpass": 2 + 4, "mask": 4 + 4 + 4, "covariance": 4, "apply": 4 + 4,
+    # algorithmic bytes per channel-sample of each stage (input once, output once);
+    # the mask's copy is fused into the covariance and apply reads, so the mask
+    # stage reads its input once (interval norms) and writes no timeseries
+    alg_bytes = {"bandpass": 2 + 4, "mask": 4, "covariance": 4, "apply": 4 + 4,
                  "detect": 4 + 4}
     rl = {}
     for k, b in alg_bytes.items():

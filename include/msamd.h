@@ -233,6 +233,14 @@ int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N);
 int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt, void* workspace,
                             int64_t workspace_bytes, void* stream);
 
+/* msamd_whiten_covariance of the masked timeseries, with the mask given as
+ * msamd_mask_rows' row bitmap: rows whose bit is clear read as zero.  The
+ * result equals msamd_whiten_covariance of the msamd_mask_out_artifacts
+ * output bit for bit.  Same workspace as msamd_whiten_covariance. */
+int msamd_whiten_covariance_rows(const float* X, int64_t M, int64_t N, const uint32_t* row_keep,
+                                 double* XXt, void* workspace, int64_t workspace_bytes,
+                                 void* stream);
+
 /* Workspace bytes for msamd_whitening_matrix. */
 int64_t msamd_whitening_matrix_workspace_size(int64_t M);
 
@@ -273,6 +281,14 @@ int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, fl
                        double quantization_unit, int16_t* Yq, void* workspace,
                        int64_t workspace_bytes, void* stream);
 
+/* msamd_whiten_apply of the masked timeseries (row bitmap as for
+ * msamd_whiten_covariance_rows; masked rows give zero output rows), equal
+ * bit for bit to msamd_whiten_apply of the masked array.  Same workspace as
+ * msamd_whiten_apply. */
+int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, const double* W,
+                            const uint32_t* row_keep, float* Y, double quantization_unit,
+                            int16_t* Yq, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Artifact masking (a7) -- replaces p_mask_out_artifacts
  * (packages/ms3/p_mask_out_artifacts.cpp:24-104).
@@ -291,6 +307,17 @@ int64_t msamd_mask_workspace_size(int64_t M, int64_t N, int64_t interval_size);
 int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, double threshold,
                              int64_t interval_size, float* Y, int32_t* use_it, void* workspace,
                              int64_t workspace_bytes, void* stream);
+
+/* The mask's decisions without the copy: row_keep receives ceil(N/32) words,
+ * bit t%32 of word t/32 set when row t lies in a kept interval (so the tail
+ * N mod interval_size is clear), for the masked-input forms of the
+ * covariance and the apply below.  use_it as above (may be NULL).  Same
+ * workspace as msamd_mask_out_artifacts.  No reference counterpart: it
+ * splits p_mask_out_artifacts so that the masked timeseries need not be
+ * written when only the whitening consumes it. */
+int msamd_mask_rows(const float* X, int64_t M, int64_t N, double threshold,
+                    int64_t interval_size, uint32_t* row_keep, int32_t* use_it, void* workspace,
+                    int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Learned spike-presence detector.  No counterpart in the reference: a

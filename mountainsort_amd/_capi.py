@@ -45,14 +45,17 @@ SIGNATURES = {
                                   vp]),
     "msamd_whiten_covariance_workspace_size": (i64, [i64, i64]),
     "msamd_whiten_covariance": (cint, [vp, i64, i64, vp, vp, i64, vp]),
+    "msamd_whiten_covariance_rows": (cint, [vp, i64, i64, vp, vp, vp, i64, vp]),
     "msamd_whitening_matrix_workspace_size": (i64, [i64]),
     "msamd_whitening_matrix": (cint, [vp, i64, vp, vp, i64, vp]),
     "msamd_whitening_matrix_exact_workspace_size": (i64, [i64]),
     "msamd_whitening_matrix_exact": (cint, [vp, i64, vp, vp, i64, vp, vp, vp]),
     "msamd_whiten_apply_workspace_size": (i64, [i64]),
     "msamd_whiten_apply": (cint, [vp, i64, i64, vp, vp, f64, vp, vp, i64, vp]),
+    "msamd_whiten_apply_rows": (cint, [vp, i64, i64, vp, vp, vp, f64, vp, vp, i64, vp]),
     "msamd_mask_workspace_size": (i64, [i64, i64, i64]),
     "msamd_mask_out_artifacts": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
+    "msamd_mask_rows": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
     "msamd_tcn_num_params": (i32, []),
     "msamd_tcn_init": (cint, [vp, ctypes.c_uint64, vp]),
     "msamd_tcn_labels": (cint, [vp, vp, i32, i64, vp, vp, vp]),

@@ -15,6 +15,8 @@
 namespace msamd {
 namespace {
 
+__device__ __forceinline__ int64_t ceil_div_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
 struct MaskBuffers {
     double* norms;     // [nint * M], interval-major
     double* thr;       // [M]
@@ -145,6 +147,22 @@ __global__ void __launch_bounds__(256) mask_copy_kernel(const float* __restrict_
     }
 }
 
+// Row bitmap of the decisions: bit t%32 of word t/32 is set when row t lies in
+// a kept interval (the tail N mod interval_size is never kept).
+__global__ void __launch_bounds__(256) mask_rowbits_kernel(const int32_t* __restrict__ use_it,
+                                                           int64_t N, int64_t nint, int64_t isz,
+                                                           uint32_t* __restrict__ rows) {
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w >= ceil_div_dev(N, 32)) return;
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int64_t t = 32 * w + b;
+        const int64_t i = t / isz;
+        if (t < N && i < nint && use_it[i]) bits |= 1u << b;
+    }
+    rows[w] = bits;
+}
+
 }  // namespace
 }  // namespace msamd
 
@@ -157,10 +175,11 @@ extern "C" int64_t msamd_mask_workspace_size(int64_t M, int64_t N, int64_t inter
     return bytes;
 }
 
-extern "C" int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, double threshold,
-                                        int64_t interval_size, float* Y, int32_t* use_it,
-                                        void* workspace, int64_t workspace_bytes, void* stream) {
-    clear_error();
+namespace {
+
+int mask_impl(const float* X, int64_t M, int64_t N, double threshold, int64_t interval_size,
+              float* Y, uint32_t* row_keep, int32_t* use_it, void* workspace,
+              int64_t workspace_bytes, void* stream) {
     MSAMD_REQUIRE(threshold != 0 && interval_size != 0,
                   "mask_out_artifacts: threshold or interval_size is zero");
     MSAMD_REQUIRE(interval_size > 0 && M >= 1 && N >= 0, "mask_out_artifacts: bad shape");
@@ -187,7 +206,12 @@ extern "C" int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, do
     mask_useit_kernel<<<static_cast<unsigned>(ceil_div(nint + 1, 256)), 256, 0, st>>>(
         b.bad, nint, b.use_it);
     MSAMD_LAUNCH_CHECK();
-    if (M * N > 0) {
+    if (row_keep && N > 0) {
+        mask_rowbits_kernel<<<static_cast<unsigned>(ceil_div(ceil_div(N, 32), 256)), 256, 0, st>>>(
+            b.use_it, N, nint, interval_size, row_keep);
+        MSAMD_LAUNCH_CHECK();
+    }
+    if (Y && M * N > 0) {
         int64_t blocks = ceil_div(M * N / 4 + 1, 256);
         if (blocks > 16384) blocks = 16384;
         mask_copy_kernel<<<static_cast<unsigned>(blocks), 256, 0, st>>>(X, M, N, nint,
@@ -199,4 +223,24 @@ extern "C" int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, do
                                  hipMemcpyDeviceToDevice, st));
     }
     return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_mask_out_artifacts(const float* X, int64_t M, int64_t N, double threshold,
+                                        int64_t interval_size, float* Y, int32_t* use_it,
+                                        void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(Y, "mask_out_artifacts: null output");
+    return mask_impl(X, M, N, threshold, interval_size, Y, nullptr, use_it, workspace,
+                     workspace_bytes, stream);
+}
+
+extern "C" int msamd_mask_rows(const float* X, int64_t M, int64_t N, double threshold,
+                               int64_t interval_size, uint32_t* row_keep, int32_t* use_it,
+                               void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(row_keep, "mask_rows: null row_keep");
+    return mask_impl(X, M, N, threshold, interval_size, nullptr, row_keep, use_it, workspace,
+                     workspace_bytes, stream);
 }

@@ -62,7 +62,14 @@ struct CovPlan {
     int64_t M, N, span, nsplit;
     int32_t Mp, nB, nTiles, G;
     int32_t all;  // single-pass kernel
+    const uint32_t* rowkeep;  // optional: bit t%32 of word t/32 = row t is kept
 };
+
+// Rows whose keep bit is clear read as zero (the artifact mask applied on the
+// fly); no bitmap keeps every row.
+__device__ __forceinline__ bool row_kept(const uint32_t* rk, int64_t t) {
+    return !rk || ((rk[t >> 5] >> (t & 31)) & 1u);
+}
 
 inline CovPlan cov_plan(int64_t M, int64_t N) {
     CovPlan p{};
@@ -140,6 +147,8 @@ __global__ void __launch_bounds__(256) cov_kernel(const float* __restrict__ X, C
 
     const int nitems = 4 * p.Mp;
     for (int64_t t0 = tb; t0 < te; t0 += 32) {
+        // t0 is a multiple of 32: one keep word covers the step's rows
+        const uint32_t kw = p.rowkeep ? p.rowkeep[t0 >> 5] : ~0u;
         for (int q = threadIdx.x; q < nitems; q += blockDim.x) {
             const int m = q % p.Mp;
             const int b = q / p.Mp;
@@ -147,7 +156,9 @@ __global__ void __launch_bounds__(256) cov_kernel(const float* __restrict__ X, C
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int64_t t = t0 + 8 * b + e;
-                v[e] = (m < p.M && t < te) ? __builtin_nontemporal_load(X + t * p.M + m) : 0.0f;
+                v[e] = (m < p.M && t < te && ((kw >> (8 * b + e)) & 1u))
+                           ? __builtin_nontemporal_load(X + t * p.M + m)
+                           : 0.0f;
             }
             bf16x8 h, l;
             split8(v, h, l);
@@ -209,6 +220,8 @@ __global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float*
     const int plane = p.nB * 64;
     float v[kCovAllItems][8];
     auto load = [&](int64_t t0) {
+        // t0 is a multiple of 32: one keep word covers the step's rows
+        const uint32_t kw = p.rowkeep ? p.rowkeep[t0 >> 5] : ~0u;
 #pragma unroll
         for (int it = 0; it < kCovAllItems; ++it) {
             const int q = threadIdx.x + it * kCovAllWaves * 64;
@@ -217,7 +230,7 @@ __global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float*
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int64_t t = t0 + 8 * b + e;
-                v[it][e] = (q < nitems && m < p.M && t < te)
+                v[it][e] = (q < nitems && m < p.M && t < te && ((kw >> (8 * b + e)) & 1u))
                                ? __builtin_nontemporal_load(X + t * p.M + m)
                                : 0.0f;
             }
@@ -653,6 +666,7 @@ struct ApplyPlan {
     int64_t M, N, nTB, nWG;
     int32_t Mp, nK, nColTiles, nCB, CT;
     double qunit;
+    const uint32_t* rowkeep;  // optional row mask, as in CovPlan
 };
 
 inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
@@ -767,6 +781,12 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     for (int pass = 0; pass < kApplyPasses; ++pass) {
         const int64_t row0 = tblock + (static_cast<int64_t>(pass) * kApplyWaves + wid) * (kApplyRowTiles * 16);
         if (row0 >= p.N) break;
+        bool keep[kApplyRowTiles];  // this lane's rows: in range and not masked
+#pragma unroll
+        for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+            const int64_t t = row0 + rt * 16 + li;
+            keep[rt] = t < p.N && row_kept(p.rowkeep, t);
+        }
         f32x4 acc[kApplyRowTiles][CT];
 #pragma unroll
         for (int rt = 0; rt < kApplyRowTiles; ++rt)
@@ -780,7 +800,7 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                 const int64_t j0 = static_cast<int64_t>(kk) * 32 + 8 * lb;
                 float v[8];
                 if (kVec) {
-                    if (t < p.N && j0 < p.M) {
+                    if (keep[rt] && j0 < p.M) {
                         const vf4* src = reinterpret_cast<const vf4*>(X + t * p.M + j0);
                         const vf4 a = src[0];
                         const vf4 b = src[1];
@@ -793,7 +813,7 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                 } else {
 #pragma unroll
                     for (int e = 0; e < 8; ++e)
-                        v[e] = (t < p.N && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
+                        v[e] = (keep[rt] && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
                 }
                 split3_8(v, ah[rt], am[rt], al[rt]);
             }
@@ -846,9 +866,10 @@ extern "C" int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N) 
     return c.bytes();
 }
 
-extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt,
-                                       void* workspace, int64_t workspace_bytes, void* stream) {
-    clear_error();
+namespace {
+
+int covariance_impl(const float* X, int64_t M, int64_t N, const uint32_t* rowkeep, double* XXt,
+                    void* workspace, int64_t workspace_bytes, void* stream) {
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_covariance: M=%lld outside 1..1024", (long long)M);
     MSAMD_REQUIRE(N >= 0, "whiten_covariance: negative N");
     hipStream_t st = as_stream(stream);
@@ -856,7 +877,8 @@ extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, dou
         MSAMD_HIP(hipMemsetAsync(XXt, 0, sizeof(double) * M * M, st));
         return MSAMD_OK;
     }
-    const CovPlan p = cov_plan(M, N);
+    CovPlan p = cov_plan(M, N);
+    p.rowkeep = rowkeep;
     Carver c(workspace);
     float* part = c.take<float>(p.nsplit * p.nTiles * 256);
     if (!workspace || workspace_bytes < c.bytes()) {
@@ -893,6 +915,23 @@ extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, dou
     cov_reduce_kernel<<<static_cast<unsigned>(ceil_div(nred, 256)), 256, 0, st>>>(part, p, XXt);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt,
+                                       void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    return covariance_impl(X, M, N, nullptr, XXt, workspace, workspace_bytes, stream);
+}
+
+extern "C" int msamd_whiten_covariance_rows(const float* X, int64_t M, int64_t N,
+                                            const uint32_t* row_keep, double* XXt,
+                                            void* workspace, int64_t workspace_bytes,
+                                            void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(row_keep, "whiten_covariance_rows: null row_keep");
+    return covariance_impl(X, M, N, row_keep, XXt, workspace, workspace_bytes, stream);
 }
 
 extern "C" int64_t msamd_whitening_matrix_workspace_size(int64_t M) {
@@ -992,15 +1031,17 @@ extern "C" int64_t msamd_whiten_apply_workspace_size(int64_t M) {
     return c.bytes();
 }
 
-extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,
-                                  double quantization_unit, int16_t* Yq, void* workspace,
-                                  int64_t workspace_bytes, void* stream) {
-    clear_error();
+namespace {
+
+int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const uint32_t* rowkeep,
+               float* Y, double quantization_unit, int16_t* Yq, void* workspace,
+               int64_t workspace_bytes, void* stream) {
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_apply: M=%lld outside 1..1024", (long long)M);
     MSAMD_REQUIRE(N >= 0, "whiten_apply: negative N");
     MSAMD_REQUIRE(Y || Yq, "whiten_apply: no output");
     MSAMD_REQUIRE(!Yq || quantization_unit > 0, "whiten_apply: int16 output needs quantization_unit > 0");
-    const ApplyPlan p = apply_plan(M, N, quantization_unit);
+    ApplyPlan p = apply_plan(M, N, quantization_unit);
+    p.rowkeep = rowkeep;
     Carver c(workspace);
     const int64_t nfr = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
     bf16x8* fr = c.take<bf16x8>(3 * nfr);
@@ -1033,4 +1074,24 @@ extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const do
 #undef MSAMD_APPLY_LAUNCH
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,
+                                  double quantization_unit, int16_t* Yq, void* workspace,
+                                  int64_t workspace_bytes, void* stream) {
+    clear_error();
+    return apply_impl(X, M, N, W, nullptr, Y, quantization_unit, Yq, workspace,
+                      workspace_bytes, stream);
+}
+
+extern "C" int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, const double* W,
+                                       const uint32_t* row_keep, float* Y,
+                                       double quantization_unit, int16_t* Yq, void* workspace,
+                                       int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(row_keep, "whiten_apply_rows: null row_keep");
+    return apply_impl(X, M, N, W, row_keep, Y, quantization_unit, Yq, workspace,
+                      workspace_bytes, stream);
 }

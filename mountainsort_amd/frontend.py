@@ -337,16 +337,33 @@ def pca_autotune(clips_flat, num_events, widths, clip_size, num_features_per_cha
 # --------------------------------------------------------------------------
 # whitening (a4-a6)
 # --------------------------------------------------------------------------
-def whiten_covariance(X):
-    """X X^T / (N-1) as an (M, M) float64 tensor (p_whiten.cpp:64-118)."""
+def _row_keep(row_keep, N, dev):
+    if row_keep is None:
+        return None
+    if (row_keep.dtype != torch.int32 or row_keep.device != dev or not row_keep.is_contiguous()
+            or row_keep.numel() < (N + 31) // 32):
+        raise ValueError("row_keep must be mask_rows' int32 bitmap on the timeseries' device")
+    return row_keep
+
+
+def whiten_covariance(X, row_keep=None):
+    """X X^T / (N-1) as an (M, M) float64 tensor (p_whiten.cpp:64-118).
+
+    row_keep: mask_rows' bitmap; rows whose bit is clear are read as zero
+    (the covariance of the masked timeseries without writing it)."""
     _check_ts(X)
     N, M = X.shape
     dev = X.device
     L = _capi.lib()
     ws = _workspace(L.msamd_whiten_covariance_workspace_size(M, N), dev)
     C = torch.empty((M, M), dtype=torch.float64, device=dev)
-    _capi.call("msamd_whiten_covariance", _ptr(X), M, N, _ptr(C), _ptr(ws), ws.numel(),
-               _stream(dev))
+    rk = _row_keep(row_keep, N, dev)
+    if rk is None:
+        _capi.call("msamd_whiten_covariance", _ptr(X), M, N, _ptr(C), _ptr(ws), ws.numel(),
+                   _stream(dev))
+    else:
+        _capi.call("msamd_whiten_covariance_rows", _ptr(X), M, N, _ptr(rk), _ptr(C), _ptr(ws),
+                   ws.numel(), _stream(dev))
     return C
 
 
@@ -376,11 +393,12 @@ def whitening_matrix(C, exact=False, return_info=False):
     return Wcm.t()  # [row, col] view of the column-major result
 
 
-def whiten_apply(X, W, quantization_unit=0.0, out=None):
+def whiten_apply(X, W, quantization_unit=0.0, out=None, row_keep=None):
     """quantize(W applied to every sample, 1e-4) (p_whiten.cpp:153-206).
 
     Returns float32 (N, M); with quantization_unit > 0 returns int16 values
-    (int)(y/q + 0.5) (p_whiten.cpp:445-451)."""
+    (int)(y/q + 0.5) (p_whiten.cpp:445-451).  row_keep: mask_rows' bitmap;
+    masked rows are read as zero."""
     _check_ts(X)
     N, M = X.shape
     dev = X.device
@@ -388,14 +406,22 @@ def whiten_apply(X, W, quantization_unit=0.0, out=None):
     L = _capi.lib()
     ws = _workspace(L.msamd_whiten_apply_workspace_size(M), dev)
     q = float(quantization_unit)
+    rk = _row_keep(row_keep, N, dev)
+
+    def call(Yp, qq, Yqp):
+        if rk is None:
+            _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), Yp, qq, Yqp, _ptr(ws),
+                       ws.numel(), _stream(dev))
+        else:
+            _capi.call("msamd_whiten_apply_rows", _ptr(X), M, N, _ptr(Wcm), _ptr(rk), Yp, qq,
+                       Yqp, _ptr(ws), ws.numel(), _stream(dev))
+
     if q > 0:
         Yq = torch.empty((N, M), dtype=torch.int16, device=dev)
-        _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), None, q, _ptr(Yq), _ptr(ws),
-                   ws.numel(), _stream(dev))
+        call(None, q, _ptr(Yq))
         return Yq
     Y = out if out is not None else torch.empty((N, M), dtype=torch.float32, device=dev)
-    _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), _ptr(Y), 0.0, None, _ptr(ws),
-               ws.numel(), _stream(dev))
+    call(_ptr(Y), 0.0, None)
     return Y
 
 
@@ -424,3 +450,23 @@ def mask_out_artifacts(X, threshold=6.0, interval_size=2000, return_use_it=False
     _capi.call("msamd_mask_out_artifacts", _ptr(X), M, N, float(threshold), isz, _ptr(Y),
                _ptr(use_it), _ptr(ws), ws.numel(), _stream(dev))
     return (Y, use_it) if return_use_it else Y
+
+
+def mask_rows(X, threshold=6.0, interval_size=2000):
+    """The decisions of mask_out_artifacts without the copy: returns
+    (row_keep, use_it), row_keep an int32 tensor of ceil(N/32) bitmap words
+    (bit t%32 of word t/32 set when row t is kept) for the row_keep argument
+    of whiten_covariance and whiten_apply."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    isz = int(interval_size)
+    if threshold == 0 or isz == 0:
+        raise ValueError("Problem with input parameters. Either threshold or interval_size is zero.")
+    L = _capi.lib()
+    ws = _workspace(L.msamd_mask_workspace_size(M, N, isz), dev)
+    rk = torch.zeros(max((N + 31) // 32, 1), dtype=torch.int32, device=dev)
+    use_it = torch.empty(N // isz + 1, dtype=torch.int32, device=dev)
+    _capi.call("msamd_mask_rows", _ptr(X), M, N, float(threshold), isz, _ptr(rk), _ptr(use_it),
+               _ptr(ws), ws.numel(), _stream(dev))
+    return rk, use_it

@@ -240,3 +240,35 @@ def test_whitening_matrix_exact_zero_channels_and_singular(gpu):
     S = x @ x.T  # rank 3, every diagonal entry nonzero
     with pytest.raises(_capi.MsamdError):
         fe.whitening_matrix(torch.from_numpy(S).to(gpu), exact=True)
+
+
+@pytest.mark.parametrize("M,N,isz", [(64, 2000 * 37 + 555, 2000), (384, 500 * 41, 500),
+                                     (7, 999, 2000)])
+def test_masked_whitening_without_the_masked_copy(gpu, M, N, isz):
+    """mask_rows + the row-bitmap covariance/apply equal mask_out_artifacts
+    followed by the plain covariance/apply, bit for bit."""
+    from mountainsort_amd import frontend as fe
+    say(f"masked whitening by row bitmap M={M} N={N} isz={isz}")
+    X = _recording(M, N, seed=300 + M)
+    rng = np.random.default_rng(M)
+    nint = N // isz
+    for i in rng.choice(max(nint, 1), size=min(3, max(nint, 1)), replace=False):
+        X[i * isz:(i + 1) * isz] *= 80.0   # artifacts
+    Xg = torch.from_numpy(X).to(gpu)
+    Xm, use_it = fe.mask_out_artifacts(Xg, 6.0, isz, return_use_it=True)
+    rk, use_it2 = fe.mask_rows(Xg, 6.0, isz)
+    assert torch.equal(use_it, use_it2)
+    bits = rk.cpu().numpy().view(np.uint32)
+    t = np.arange(N)
+    kept = (bits[t >> 5] >> (t & 31)) & 1
+    u = use_it.cpu().numpy()
+    expect = (t // isz < nint) & (u[np.minimum(t // isz, nint)] != 0)
+    assert np.array_equal(kept.astype(bool), expect)
+    C1 = fe.whiten_covariance(Xm)
+    C2 = fe.whiten_covariance(Xg, row_keep=rk)
+    assert torch.equal(C1, C2)
+    w, V = np.linalg.eigh(C1.cpu().numpy() + 1e-3 * np.eye(M))
+    W = torch.from_numpy((V / np.sqrt(w)) @ V.T).to(gpu)
+    Y1 = fe.whiten_apply(Xm, W)
+    Y2 = fe.whiten_apply(Xg, W, row_keep=rk)
+    assert torch.equal(Y1, Y2)
