@@ -289,6 +289,16 @@ int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, const double* 
                             const uint32_t* row_keep, float* Y, double quantization_unit,
                             int16_t* Yq, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ms3.whiten_clips (packages/ms3/p_whiten.cpp:295-345): every time sample of
+ * every clip times W, without the 1e-4 quantize of the timeseries form.
+ * clips, clips_out: M x T x L float32 (the clip layout); with
+ * quantization_unit > 0, clips_out_q receives the int16 file values
+ * (int)(y/q + 0.5) and clips_out may be NULL.  Same workspace as
+ * msamd_whiten_apply. */
+int msamd_whiten_clips(const float* clips, int64_t M, int64_t T, int64_t L, const double* W,
+                       float* clips_out, double quantization_unit, int16_t* clips_out_q,
+                       void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Artifact masking (a7) -- replaces p_mask_out_artifacts
  * (packages/ms3/p_mask_out_artifacts.cpp:24-104).

@@ -10,6 +10,8 @@
 // hi = bf16(x), lo = bf16(x - hi); x*y ~ hi*hi' + hi*lo' + lo*hi' (the lo*lo'
 // term is dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The
 // covariance sums FP32 block partials in FP64; the whitening matrix is FP64.
+#include <cstdint>
+
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -667,6 +669,7 @@ struct ApplyPlan {
     int32_t Mp, nK, nColTiles, nCB, CT;
     double qunit;
     const uint32_t* rowkeep;  // optional row mask, as in CovPlan
+    int32_t raw;              // 1: no 1e-4 quantize (the clip form)
 };
 
 inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
@@ -840,8 +843,11 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                 for (int r = 0; r < 4; ++r) {
                     const int64_t t = row0 + rt * 16 + 4 * lb + r;
                     if (t >= p.N) continue;
-                    const double xq = floor(div_unit(static_cast<double>(acc[rt][ct][r])) + 0.5) * 0.0001;
-                    const float yf = static_cast<float>(xq);
+                    const float yf =
+                        p.raw ? acc[rt][ct][r]
+                              : static_cast<float>(
+                                    floor(div_unit(static_cast<double>(acc[rt][ct][r])) + 0.5) *
+                                    0.0001);
                     if (Y) Y[t * p.M + i] = yf;
                     if (Yq) {
                         const double s = static_cast<double>(yf) / p.qunit + 0.5;
@@ -1035,13 +1041,14 @@ namespace {
 
 int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const uint32_t* rowkeep,
                float* Y, double quantization_unit, int16_t* Yq, void* workspace,
-               int64_t workspace_bytes, void* stream) {
+               int64_t workspace_bytes, void* stream, bool raw = false) {
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_apply: M=%lld outside 1..1024", (long long)M);
     MSAMD_REQUIRE(N >= 0, "whiten_apply: negative N");
     MSAMD_REQUIRE(Y || Yq, "whiten_apply: no output");
     MSAMD_REQUIRE(!Yq || quantization_unit > 0, "whiten_apply: int16 output needs quantization_unit > 0");
     ApplyPlan p = apply_plan(M, N, quantization_unit);
     p.rowkeep = rowkeep;
+    p.raw = raw ? 1 : 0;
     Carver c(workspace);
     const int64_t nfr = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
     bf16x8* fr = c.take<bf16x8>(3 * nfr);
@@ -1094,4 +1101,16 @@ extern "C" int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, con
     MSAMD_REQUIRE(row_keep, "whiten_apply_rows: null row_keep");
     return apply_impl(X, M, N, W, row_keep, Y, quantization_unit, Yq, workspace,
                       workspace_bytes, stream);
+}
+
+extern "C" int msamd_whiten_clips(const float* clips, int64_t M, int64_t T, int64_t L,
+                                  const double* W, float* clips_out, double quantization_unit,
+                                  int16_t* clips_out_q, void* workspace, int64_t workspace_bytes,
+                                  void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(T >= 0 && L >= 0 && (T == 0 || L <= INT64_MAX / T), "whiten_clips: bad shape");
+    if (T * L == 0) return MSAMD_OK;  // no clips: nothing to write
+    // M x T x L clips are an M x (T*L) array: the apply without the 1e-4 quantize
+    return apply_impl(clips, M, T * L, W, nullptr, clips_out, quantization_unit, clips_out_q,
+                      workspace, workspace_bytes, stream, true);
 }

@@ -425,6 +425,33 @@ def whiten_apply(X, W, quantization_unit=0.0, out=None, row_keep=None):
     return Y
 
 
+def whiten_clips(clips, W, quantization_unit=0.0):
+    """ms3.whiten_clips (p_whiten.cpp:295-345) on device clips of shape (L, T, M)
+    (the M x T x L .mda layout): W applied to every time sample, no 1e-4
+    quantize.  Returns float32 (L, T, M), or int16 (int)(y/q + 0.5) values
+    when quantization_unit > 0."""
+    if (not isinstance(clips, torch.Tensor) or not clips.is_cuda or clips.dim() != 3
+            or clips.dtype != torch.float32 or not clips.is_contiguous()):
+        raise ValueError("clips must be a contiguous float32 CUDA tensor of shape (L, T, M)")
+    L, T, M = clips.shape
+    dev = clips.device
+    q = float(quantization_unit)
+    if L * T == 0:
+        return torch.empty((L, T, M), dtype=torch.int16 if q > 0 else torch.float32, device=dev)
+    Wcm = W.to(device=dev, dtype=torch.float64).t().contiguous()
+    lib = _capi.lib()
+    ws = _workspace(lib.msamd_whiten_apply_workspace_size(M), dev)
+    if q > 0:
+        out = torch.empty((L, T, M), dtype=torch.int16, device=dev)
+        _capi.call("msamd_whiten_clips", _ptr(clips), M, T, L, _ptr(Wcm), None, q, _ptr(out),
+                   _ptr(ws), ws.numel(), _stream(dev))
+        return out
+    out = torch.empty((L, T, M), dtype=torch.float32, device=dev)
+    _capi.call("msamd_whiten_clips", _ptr(clips), M, T, L, _ptr(Wcm), _ptr(out), 0.0, None,
+               _ptr(ws), ws.numel(), _stream(dev))
+    return out
+
+
 def whiten(X, quantization_unit=0.0):
     """ms3.whiten: covariance, whitening matrix, apply (p_whiten.cpp:45-206)."""
     C = whiten_covariance(X)

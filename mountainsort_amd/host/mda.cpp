@@ -136,6 +136,31 @@ bool mda_read_f64(const std::string& path, std::vector<double>& out, MdaHeader& 
     return true;
 }
 
+bool mda_read_f32(const std::string& path, std::vector<float>& out, MdaHeader& h,
+                  std::string& err) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "unable to open " + path;
+        return false;
+    }
+    if (!mda_read_header(f, h, err)) {
+        fclose(f);
+        err = path + ": " + err;
+        return false;
+    }
+    const int64_t n = h.size();
+    std::vector<char> raw(static_cast<size_t>(n * h.bytes_per_entry));
+    const size_t got = n ? fread(raw.data(), h.bytes_per_entry, n, f) : 0;
+    fclose(f);
+    if (static_cast<int64_t>(got) != n) {
+        err = path + ": truncated data";
+        return false;
+    }
+    out.resize(n);
+    mda_convert_to_f32(h.type, raw.data(), out.data(), n);
+    return true;
+}
+
 MdaWriter::~MdaWriter() { abort(); }
 
 bool MdaWriter::open(const std::string& path, int32_t type, const std::vector<int64_t>& dims_in,

@@ -52,6 +52,8 @@ void mda_convert_to_f64(int32_t type, const void* src, double* dst, int64_t n);
 // Whole-array reads, converted (small arrays: event times, matrices, geometry).
 bool mda_read_f64(const std::string& path, std::vector<double>& out, MdaHeader& h,
                   std::string& err);
+bool mda_read_f32(const std::string& path, std::vector<float>& out, MdaHeader& h,
+                  std::string& err);
 
 // Streamed writer: writes path + ".tmp", renamed to path by close()
 // (packages/ms3/mda/diskwritemda.cpp:84,142-146).  The number of dimensions

@@ -300,6 +300,38 @@ bool run_apply_whitening_matrix(Ctx& c) {
                            c.num("quantization_unit"), s.get());
 }
 
+// ms3.whiten_clips (p_whiten.cpp:295-345): clips M x T x L, W M x M.
+bool run_whiten_clips(Ctx& c) {
+    std::vector<float> clips;
+    std::vector<double> w;
+    MdaHeader hc, hw;
+    if (!mda_read_f32(c.str("clips"), clips, hc, c.err) ||
+        !mda_read_f64(c.str("whitening_matrix"), w, hw, c.err))
+        return false;
+    const int64_t M = hc.dim(0), T = hc.dim(1), L = hc.dim(2);
+    if (hc.size() != M * T * L) return c.fail("clips must be three-dimensional (M x T x L)");
+    if (hw.dim(0) != M || hw.size() != M * M)
+        return c.fail("whitening matrix is not " + std::to_string(M) + " x " + std::to_string(M));
+    const double q = c.num("quantization_unit");
+    const int32_t out_type = q > 0 ? MDA_INT16 : MDA_FLOAT32;
+    Stream s;
+    DeviceBuffer X, W, Y, ws;
+    const int64_t n = std::max<int64_t>(1, M * T * L);
+    if (!s.create(c.err) || !X.alloc(n * 4, c.err) || !W.alloc(M * M * 8, c.err) ||
+        !Y.alloc(n * mda_bytes_per_entry(out_type), c.err) ||
+        !ws.alloc(msamd_whiten_apply_workspace_size(M), c.err) ||
+        !to_device(X.get(), clips.data(), M * T * L * 4, s.get(), c.err) ||
+        !to_device(W.get(), w.data(), M * M * 8, s.get(), c.err))
+        return false;
+    const int st = q > 0 ? msamd_whiten_clips(X.as<float>(), M, T, L, W.as<double>(), nullptr, q,
+                                              Y.as<int16_t>(), ws.get(), ws.bytes(), s.get())
+                         : msamd_whiten_clips(X.as<float>(), M, T, L, W.as<double>(),
+                                              Y.as<float>(), 0.0, nullptr, ws.get(), ws.bytes(),
+                                              s.get());
+    if (!lib_ok(c, st, "whiten_clips")) return false;
+    return write_device(c, c.str("clips_out"), out_type, {M, T, L}, Y.get(), s.get());
+}
+
 bool run_mask(Ctx& c) {
     const double thr = c.num("threshold");
     const int64_t isz = static_cast<int64_t>(c.num("interval_size"));
@@ -583,6 +615,15 @@ std::vector<ProcessorSpec> make_registry() {
         p.input("timeseries_list").output("whitening_matrix_out");
         p.optional("channels");
         p.run = run_compute_whitening_matrix;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.whiten_clips";
+        p.version = "0.1";
+        p.input("clips").input("whitening_matrix").output("clips_out");
+        p.optional("quantization_unit", "", 0LL);
+        p.run = run_whiten_clips;
         v.push_back(p);
     }
     {

@@ -41,6 +41,7 @@ def lib():
         L.orc_pca_from_XXt.argtypes = [vp, i64, i64, vp, vp]
         L.orc_whitening_matrix.argtypes = [vp, i64, vp]
         L.orc_whiten_apply.argtypes = [vp, i64, i64, vp, vp, f64, vp]
+        L.orc_whiten_clips.argtypes = [vp, i64, i64, vp, vp, f64, vp]
         L.orc_mask_out_artifacts.argtypes = [vp, i64, i64, f64, i64, vp, vp, vp]
         L.orc_mask_out_artifacts.restype = ctypes.c_int
         L.orc_pca.argtypes = [vp, i64, i64, i64, vp, vp, vp]
@@ -172,6 +173,21 @@ def whiten_apply(X, W, quantization_unit=0.0):
     Y = np.empty((N, M), np.float32)
     Yq = np.empty((N, M), np.int16) if quantization_unit > 0 else None
     lib().orc_whiten_apply(_p(X), M, N, _p(Wc), _p(Y), float(quantization_unit),
+                           _p(Yq) if Yq is not None else None)
+    return (Y, Yq) if Yq is not None else Y
+
+
+def whiten_clips(clips, W, quantization_unit=0.0):
+    """ms3.whiten_clips (p_whiten.cpp:295-345): every (clip, time sample) column
+    of the (L, T, M) clips times W with the float accumulation of pass 2 and no
+    1e-4 quantize.  Returns float32 (L, T, M); with quantization_unit > 0
+    returns (float32, int16 file values)."""
+    X = np.ascontiguousarray(clips, np.float32)
+    L, T, M = X.shape
+    Wc = np.ascontiguousarray(np.asarray(W, np.float64).T)
+    Y = np.empty_like(X)
+    Yq = np.empty(X.shape, np.int16) if quantization_unit > 0 else None
+    lib().orc_whiten_clips(_p(X), M, L * T, _p(Wc), _p(Y), float(quantization_unit),
                            _p(Yq) if Yq is not None else None)
     return (Y, Yq) if Yq is not None else Y
 

@@ -437,6 +437,35 @@ EXPORT void orc_whiten_apply(const float* X, int64_t M, int64_t N, const double*
 }
 
 /* ------------------------------------------------------------------------ */
+/* f1: p_whiten_clips, packages/ms3/p_whiten.cpp:295-345.  Per clip and time */
+/* sample the same float accumulation over m2 as pass 2 (:326-334), with no */
+/* 1e-4 quantize (it is commented out there, :347-352); with              */
+/* quantization_unit > 0, scale_for_quantization (:445-451) gives the int16 */
+/* file values.  Clips: M x T x L, i.e. an M x (T*L) array.                 */
+/* ------------------------------------------------------------------------ */
+EXPORT void orc_whiten_clips(const float* X, int64_t M, int64_t TL, const double* W,
+    float* out_f32, double quantization_unit, int16_t* out_i16)
+{
+    for (int64_t i = 0; i < TL; i++) {
+        const float* x = X + M * i;
+        float* y = out_f32 + M * i;
+        int64_t aa = 0;
+        for (int64_t m1 = 0; m1 < M; m1++) {
+            float acc = 0;
+            for (int64_t m2 = 0; m2 < M; m2++) {
+                acc = (float)((double)acc + (double)x[m2] * W[aa]);
+                aa++;
+            }
+            y[m1] = acc;
+            if (quantization_unit > 0 && out_i16) {
+                float s = (float)(int)((y[m1] / quantization_unit) + 0.5);
+                out_i16[M * i + m1] = (int16_t)s;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* a7: mask_out_artifacts, packages/ms3/p_mask_out_artifacts.cpp:24-104.     */
 /* norms_out (M x N/interval, column-major) and use_it_out (N/interval + 1) */
 /* may be NULL.  Returns 0, or -1 on the parameter check of :26-29.         */

@@ -272,3 +272,27 @@ def test_masked_whitening_without_the_masked_copy(gpu, M, N, isz):
     Y1 = fe.whiten_apply(Xm, W)
     Y2 = fe.whiten_apply(Xg, W, row_keep=rk)
     assert torch.equal(Y1, Y2)
+
+
+@pytest.mark.parametrize("L,T,M", [(1000, 50, 12), (37, 50, 64), (500, 20, 384), (0, 50, 8)])
+def test_whiten_clips_within_fp32_of_fp64(gpu, L, T, M):
+    """whiten_clips (no 1e-4 quantize) against the FP64 product and the
+    reference's float loop."""
+    from mountainsort_amd import frontend as fe
+    say(f"whiten_clips L={L} T={T} M={M}")
+    rng = np.random.default_rng(L + M)
+    clips = (rng.normal(size=(L, T, M)) * 30).astype(np.float32)
+    A = rng.normal(size=(M, M)) / np.sqrt(M)
+    W = A @ A.T + 0.1 * np.eye(M)
+    Y = fe.whiten_clips(torch.from_numpy(clips).to(gpu), torch.from_numpy(W).to(gpu))
+    Y = Y.cpu().numpy()
+    assert Y.shape == (L, T, M)
+    if L == 0:
+        return
+    Y64 = np.einsum("ltm,km->ltk", clips.astype(np.float64), W)
+    rms = np.sqrt(np.mean(Y64 ** 2))
+    e = np.max(np.abs(Y - Y64)) / rms
+    say(f"  max |y - y_fp64| / rms = {e:.2e}")
+    assert e < 1e-5
+    ref = oc.whiten_clips(clips, W)
+    assert np.max(np.abs(Y - ref)) / rms < 1e-5

@@ -112,6 +112,35 @@ def test_whiten_processors(gpu, exe, tmp_path):
     assert d <= 1
 
 
+def test_whiten_clips_processor(gpu, exe, tmp_path):
+    """ms3.whiten_clips: W on every time sample of every clip, no 1e-4 quantize;
+    float output within FP32 rounding of the reference's float loop, int16
+    output within one count."""
+    rng = np.random.default_rng(12)
+    L, T, M = 300, 50, 9
+    clips = (rng.normal(size=(L, T, M)) * 20).astype(np.float32)
+    W = rng.normal(size=(M, M)) * 0.1
+    W = W + W.T
+    mdaio.writemda(tmp_path / "c.mda", clips)              # M x T x L
+    mdaio.writemda(tmp_path / "W.mda", np.ascontiguousarray(W.T))
+    say(f"whiten_clips processor L={L} T={T} M={M}")
+    run("ms3.whiten_clips", f"--clips={tmp_path / 'c.mda'}",
+        f"--whitening_matrix={tmp_path / 'W.mda'}", f"--clips_out={tmp_path / 'o.mda'}")
+    out = mdaio.readmda(tmp_path / "o.mda")
+    ref = oc.whiten_clips(clips, W)
+    assert out.shape == ref.shape and out.dtype == np.float32
+    e = np.max(np.abs(out - ref)) / np.max(np.abs(ref))
+    say(f"  float output: max rel diff to the reference loop {e:.2e}")
+    assert e < 1e-5
+    run("ms3.whiten_clips", f"--clips={tmp_path / 'c.mda'}",
+        f"--whitening_matrix={tmp_path / 'W.mda'}", f"--clips_out={tmp_path / 'q.mda'}",
+        "--quantization_unit=0.1")
+    q = mdaio.readmda(tmp_path / "q.mda")
+    _, refq = oc.whiten_clips(clips, W, quantization_unit=0.1)
+    assert q.dtype == np.int16
+    assert np.max(np.abs(q.astype(np.int32) - refq.astype(np.int32))) <= 1
+
+
 def test_extract_clips_processor(gpu, exe, tmp_path):
     X = _recording(5, 50_000, 5).astype(np.float32)
     mdaio.writemda(tmp_path / "x.mda", X)

@@ -31,6 +31,7 @@ REFERENCE_SPECS = [
     ("ms3.whiten", "0.1", [], {"quantization_unit": 0}),
     ("ms3.compute_whitening_matrix", "0.11", [], {"channels": None}),
     ("ms3.apply_whitening_matrix", "0.1", [], {"quantization_unit": 0}),
+    ("ms3.whiten_clips", "0.1", [], {"quantization_unit": 0}),
     ("ms3.extract_clips", "0.11", ["clip_size"], {"channels": None}),
     ("ms3.mask_out_artifacts", "0.1", [], {"threshold": 6, "interval_size": 2000}),
 ]
