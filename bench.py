@@ -65,6 +65,15 @@ class Pipeline:
         self.filt = torch.empty((N, M), dtype=torch.float32, device=dev)
         self.white = torch.empty((N, M), dtype=torch.float32, device=dev)
         self.stats = {}
+        self.pca_tiles = None   # default tiles until tune_pca() has run
+
+    def tune_pca(self):
+        """Autotune the PCA tiles on the last run's clips (the same shapes every step)."""
+        import numpy as np
+        off = self.last_offsets.cpu().numpy().astype(np.int64)
+        L = off[1:] - off[:-1]
+        self.pca_tiles, ms = self.fe.pca_autotune(self.last_clips, L, np.array(self.widths), 50)
+        return self.pca_tiles, ms
 
     def run(self, raw, timers=None):
         fe, torch = self.fe, self.torch
@@ -98,8 +107,10 @@ class Pipeline:
         mark("clips")
         feats = None
         if self.with_pca:
-            feats = fe.pca_features(clips, clip_offsets, offsets, self.nb_offsets, 50)
+            feats = fe.pca_features(clips, clip_offsets, offsets, self.nb_offsets, 50,
+                                    tiles=self.pca_tiles)
             mark("pca")
+        self.last_clips, self.last_offsets = clips, offsets
         if timers is not None:
             torch.cuda.synchronize()
             names = list(ev.keys())
@@ -478,6 +489,9 @@ def main():
         pipe.run(raw)
         torch.cuda.synchronize()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+        if i == 0:
+            tiles, tms = pipe.tune_pca()
+            log(f"[bench] PCA tiles {tiles} ({tms:.2f} ms in the autotuner)")
     timers = {}
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -529,7 +543,7 @@ def main():
                                "0.195 uV/bit, 300 units, 1% artifacts (synth.config2 seed 2)",
                    "channels": cfg.M, "samples": N, "reference_threads": 8,
                    "adjacency_radius_um": 40, "clip_size": 50, "events": pipe.stats.get("events"),
-                   "pca": pipe.with_pca},
+                   "pca": pipe.with_pca, "pca_tiles": pipe.pca_tiles},
         "hbm_frac_pipeline": round(total_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_roofline": rl,

@@ -184,7 +184,8 @@ def extract_clips_batched(X, times, event_offsets, nbhd_channels, nbhd_offsets, 
 # PCA features (a12-a15)
 # --------------------------------------------------------------------------
 def pca_features(clips_flat, clip_offsets, event_offsets, nbhd_offsets, clip_size,
-                 num_features_per_channel=10, num_features=10, max_pca_samples=10000):
+                 num_features_per_channel=10, num_features=10, max_pca_samples=10000,
+                 tiles=None):
     """Two-stage PCA features of every neighborhood (neighborhoodsorter.cpp:236-276,
     sort_clips.cpp:58-70).
 
@@ -192,7 +193,8 @@ def pca_features(clips_flat, clip_offsets, event_offsets, nbhd_offsets, clip_siz
     and nbhd_offsets are the (G+1) CSR offsets of events and channels.  Returns
     (reduced_flat, reduced_offsets, features_flat, feature_offsets): per
     neighborhood g, reduced (L, nf1, W) at reduced_flat[reduced_offsets[g]:] and
-    features (L, nf2) at features_flat[feature_offsets[g]:].
+    features (L, nf2) at features_flat[feature_offsets[g]:].  tiles: the
+    plan's tile sizes (see pca_autotune); None uses the defaults.
     """
     import numpy as np
     dev = clips_flat.device
@@ -200,6 +202,12 @@ def pca_features(clips_flat, clip_offsets, event_offsets, nbhd_offsets, clip_siz
     no = torch.as_tensor(nbhd_offsets).cpu().numpy().astype(np.int64)
     L = np.ascontiguousarray(eo[1:] - eo[:-1], dtype=np.int64)
     W = np.ascontiguousarray(no[1:] - no[:-1], dtype=np.int32)
+    if tiles is not None:
+        plan = PcaPlan(L, W, clip_size, num_features_per_channel, num_features, max_pca_samples,
+                       tiles=tiles, device=dev)
+        out = plan.run(clips_flat)
+        plan.close()  # the outputs stay alive; the kernels are enqueued on the stream
+        return out
     G = len(L)
     T, nf1, nf2 = int(clip_size), int(num_features_per_channel), int(num_features)
     lib = _capi.lib()
