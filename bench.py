@@ -98,7 +98,7 @@ class Pipeline:
         W = fe.whitening_matrix(C)
         mark("whitening_matrix")
         self.C, self.W = C, W
-        fe.whiten_apply(self.filt, W, out=self.white, row_keep=rows)
+        fe.whiten_apply(self.filt, W, out=self.white, row_keep=rows, xxt=C)
         mark("apply")
         offsets, times = fe.detect_events(self.white, 3.0, 10, -1, num_threads=self.num_threads)
         mark("detect")
@@ -477,7 +477,7 @@ def main():
             torch.neg(src, out=dst)
         for _ in range(3):
             if args.pmc_pass == "apply":
-                fe.whiten_apply(pipe.filt, pipe.W, out=pipe.white, row_keep=pipe.rows)
+                fe.whiten_apply(pipe.filt, pipe.W, out=pipe.white, row_keep=pipe.rows, xxt=pipe.C)
             elif args.pmc_pass == "covariance":
                 fe.whiten_covariance(pipe.filt, row_keep=pipe.rows)
             else:

@@ -271,10 +271,13 @@ int msamd_whitening_matrix_exact(const double* XXt, int64_t M, double* W, void* 
 int64_t msamd_whiten_apply_workspace_size(int64_t M);
 
 /* Y = quantize(W X, 1e-4) with quantize(x,u) = floor(x/u + 0.5)*u
- * (p_whiten.cpp:31-34,153-206).  Split-BF16 MFMA with FP32 accumulation:
- * each operand is split into three bf16 pieces (exact for fp32) and the six
- * products of order >= 2^-16 are accumulated, so products carry FP32-level
- * error (the 1e-4 output quantum needs it; two pieces would not).
+ * (p_whiten.cpp:31-34,153-206).  Split-FP16 MFMA with FP32 accumulation:
+ * both operands are scaled by powers of two into fp16's range (W by its
+ * maximum, X by its maximum, found by a pass over X) and split into two
+ * fp16 pieces (22 significant bits); the three products of order >= 2^-11
+ * are accumulated, so products carry error O(2^-22), below the reference's
+ * own float accumulation (the 1e-4 output quantum needs this; bf16 hi+lo
+ * would not).
  * Y: M x N float32 (may be NULL when Yq is given).  When quantization_unit
  * > 0, Yq receives the int16 values (int)(y/q + 0.5) (p_whiten.cpp:445-451). */
 int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, float* Y,
@@ -288,6 +291,17 @@ int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const double* W, fl
 int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, const double* W,
                             const uint32_t* row_keep, float* Y, double quantization_unit,
                             int16_t* Yq, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* msamd_whiten_apply (or _rows, row_keep non-NULL) with the covariance XXt
+ * of this X (M x M float64, as msamd_whiten_covariance / _rows returned it
+ * for the same rows): its diagonal bounds |x| (x_tj^2 <= (N-1) C_jj), which
+ * sets the fp16 scaling of the samples without the pass over X that the
+ * other forms make.  No reference counterpart; results equal those forms'
+ * up to that scale choice. */
+int msamd_whiten_apply_cov(const float* X, int64_t M, int64_t N, const double* W,
+                           const double* XXt, const uint32_t* row_keep, float* Y,
+                           double quantization_unit, int16_t* Yq, void* workspace,
+                           int64_t workspace_bytes, void* stream);
 
 /* ms3.whiten_clips (packages/ms3/p_whiten.cpp:295-345): every time sample of
  * every clip times W, without the 1e-4 quantize of the timeseries form.

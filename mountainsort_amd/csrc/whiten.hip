@@ -645,24 +645,36 @@ __global__ void __launch_bounds__(256) whitening_assemble_kernel(
 // walks W column-major, p_whiten.cpp:174-182).  MFMA view: D(16 samples x
 // 16 outputs) += A(16 samples x 32 channels) B(32 channels x 16 outputs).
 //
-// Precision: the 1e-4 output quantum of the reference needs FP32-level
-// products, and bf16 hi+lo keeps only 16 significant bits (|x - hi - lo| <=
-// 2^-16 |x|), which with cancellation inside W x reaches two quanta.  Both
-// operands are therefore split into three bf16 pieces x = h + m + l (exact
-// for an fp32 value) and the six products of order >= 2^-16 are accumulated:
-// hh + hm + mh + hl + lh + mm; the dropped terms are O(2^-24) relative.
+// Precision: the 1e-4 output quantum of the reference needs products close
+// to FP32 accuracy (bf16 hi+lo keeps 16 bits, which with cancellation inside
+// W x reaches two quanta).  Both operands are split into two fp16 pieces,
+// v = h + l with h = fp16(v), l = fp16(v - h), which keep 22 significant
+// bits (|v - h - l| <= 2^-22 |v| while l is a normal number), and the three
+// products of order >= 2^-11 are accumulated in FP32: hh + hl + lh (the
+// dropped ll is O(2^-22)); fp16 x fp16 products are exact in FP32.  fp16 has
+// a narrow exponent range, so both operands are scaled by powers of two
+// first: W by 2^(14 - e_W) with 2^e_W <= max|W| < 2^(e_W+1), and X by
+// 2^(14 - e_X) with e_X from a bound on |x| -- the covariance diagonal when
+// the caller passes it (x_tj^2 <= sum_t x_tj^2 = (N-1) C_jj) or else the
+// maximum found by a pass over X.  Scaled values stay below 2^15; values
+// far below the bound lose relative precision in their low piece, but their
+// absolute error stays below 2^-24 of the scale, far under the quantum.  The
+// epilogue divides by the two scales exactly.
 //
-// Each workgroup keeps the three-piece fragments of CT column tiles (16*CT
+// Each workgroup keeps the two-piece fragments of CT column tiles (16*CT
 // outputs) of W in LDS and streams 8 waves x 64 samples per pass over its
 // 2048-sample time block.  Workgroups are numbered so that all column
 // blocks of one time block run back to back on the same XCD (blockIdx % 8
 // selects the XCD), and the time block is read once from HBM into that
 // XCD's L2.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
 constexpr int kApplyWaves = 8;
 constexpr int kApplyRowTiles = 4;    // 16-sample row tiles per wave per pass
 constexpr int kApplyPasses = 4;      // passes per workgroup
 constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 2048 samples
 constexpr int kXcd = 8;
+constexpr int kApplyPieces = 2;
 
 struct ApplyPlan {
     int64_t M, N, nTB, nWG;
@@ -679,8 +691,15 @@ inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
     p.Mp = static_cast<int32_t>(align_up(M, 64));
     p.nK = p.Mp / 32;
     p.nColTiles = p.Mp / 16;
-    p.CT = 4;
-    while (p.CT > 1 && 3 * p.CT * p.nK * 1024 > 150 * 1024) p.CT >>= 1;
+    // column tiles per workgroup: as many as the LDS holds (fewer column blocks
+    // means fewer re-splits of every sample)
+    const auto fits = [&](int ct) { return kApplyPieces * ct * p.nK * 1024 <= 150 * 1024; };
+    if (p.nColTiles % 6 == 0 && fits(6))
+        p.CT = 6;
+    else {
+        p.CT = 4;
+        while (p.CT > 1 && !fits(p.CT)) p.CT >>= 1;
+    }
     p.nCB = p.nColTiles / p.CT;
     p.qunit = qunit;
     p.nTB = ceil_div(N, kApplyTB);
@@ -699,39 +718,87 @@ __device__ __forceinline__ double div_unit(double x) {
     return __builtin_fma(e, r, q0);
 }
 
-__device__ __forceinline__ void split3_8(const float* v, bf16x8& h, bf16x8& m, bf16x8& l) {
+__device__ __forceinline__ void split2_8(const float* v, f16x8& h, f16x8& l) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const __bf16 hb = static_cast<__bf16>(v[e]);
-        const float r1 = v[e] - static_cast<float>(hb);   // exact
-        const __bf16 mb = static_cast<__bf16>(r1);
-        const float r2 = r1 - static_cast<float>(mb);     // exact
-        h[e] = hb;
-        m[e] = mb;
-        l[e] = static_cast<__bf16>(r2);
+        const _Float16 hh = static_cast<_Float16>(v[e]);
+        h[e] = hh;
+        l[e] = static_cast<_Float16>(v[e] - static_cast<float>(hh));  // the difference is exact
     }
 }
 
-__device__ __forceinline__ f32x4 mfma6(const bf16x8& ah, const bf16x8& am, const bf16x8& al,
-                                       const bf16x8& bh, const bf16x8& bm, const bf16x8& bl,
-                                       f32x4 acc) {
-    // smallest terms first
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-    return acc;
+// 2^(14 - e) for 2^e <= b < 2^(e+1); 1 for b == 0 or a non-finite bound
+__device__ __forceinline__ float pow2_scale(double b) {
+    if (!(b > 0.0) || !isfinite(b)) return 1.0f;
+    return ldexpf(1.0f, 14 - ilogb(b));
+}
+
+// Scale words (workspace): [0] W scale (float), [1] bound on |x| as float
+// bits, [2] X scale (float).
+// One workgroup: max|W| -> W scale; with the covariance, the |x| bound
+// sqrt((N-1) max_j C_jj); without it the bound word is cleared for the
+// X pass to raise.
+__global__ void __launch_bounds__(256) apply_scale_kernel(const double* __restrict__ Wcm,
+                                                          int64_t M, const double* __restrict__ C,
+                                                          int64_t N, uint32_t* __restrict__ sc) {
+    __shared__ double red[2][4];
+    double wm = 0.0, cm = 0.0;
+    for (int64_t e = threadIdx.x; e < M * M; e += blockDim.x) wm = fmax(wm, fabs(Wcm[e]));
+    if (C)
+        for (int64_t j = threadIdx.x; j < M; j += blockDim.x) cm = fmax(cm, C[j + M * j]);
+    for (int o = 32; o > 0; o >>= 1) {
+        wm = fmax(wm, __shfl_xor(wm, o, 64));
+        cm = fmax(cm, __shfl_xor(cm, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = wm;
+        red[1][threadIdx.x >> 6] = cm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            wm = fmax(wm, red[0][w]);
+            cm = fmax(cm, red[1][w]);
+        }
+        sc[0] = __float_as_uint(pow2_scale(wm));
+        // the bound rounded up to float so that it stays a bound
+        const double xb = C ? sqrt(cm * static_cast<double>(N > 1 ? N - 1 : 1)) * (1.0 + 1e-6) : 0.0;
+        sc[1] = __float_as_uint(static_cast<float>(xb));
+    }
+}
+
+// max|x| over the kept rows, raised into sc[1] (non-negative floats order as
+// their bit patterns).
+__global__ void __launch_bounds__(256) apply_xmax_kernel(const float* __restrict__ X, int64_t M,
+                                                         int64_t N, const uint32_t* __restrict__ rk,
+                                                         uint32_t* __restrict__ sc) {
+    float m = 0.0f;
+    const int64_t total = M * N;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < total;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t t = q / M;
+        if (rk && !((rk[t >> 5] >> (t & 31)) & 1u)) continue;
+        m = fmaxf(m, fabsf(X[q]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(sc + 1, __float_as_uint(m));
+}
+
+// X scale from the bound word.
+__global__ void apply_xscale_kernel(uint32_t* __restrict__ sc) {
+    if (threadIdx.x == 0) sc[2] = __float_as_uint(pow2_scale(__uint_as_float(sc[1])));
 }
 
 // Fragments of W: index ((ct * nK + kk) * 64 + lane) within each piece array;
-// lane (i + 16b) of column tile ct holds W(32kk + 8b + e, 16ct + i), e = 0..7.
+// lane (i + 16b) of column tile ct holds W(32kk + 8b + e, 16ct + i), e = 0..7,
+// scaled by the W scale.
 __global__ void __launch_bounds__(256) apply_prep_kernel(const double* __restrict__ Wcm,
-                                                         ApplyPlan p, bf16x8* __restrict__ fr) {
+                                                         ApplyPlan p, const uint32_t* __restrict__ sc,
+                                                         f16x8* __restrict__ fr) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
     if (q >= total) return;
+    const double sw = static_cast<double>(__uint_as_float(sc[0]));
     const int lane = static_cast<int>(q & 63);
     const int64_t tk = q >> 6;
     const int kk = static_cast<int>(tk % p.nK);
@@ -741,20 +808,19 @@ __global__ void __launch_bounds__(256) apply_prep_kernel(const double* __restric
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int64_t j = kk * 32 + 8 * (lane >> 4) + e;
-        v[e] = (i < p.M && j < p.M) ? static_cast<float>(Wcm[j + p.M * i]) : 0.0f;
+        v[e] = (i < p.M && j < p.M) ? static_cast<float>(Wcm[j + p.M * i] * sw) : 0.0f;
     }
-    bf16x8 h, m, l;
-    split3_8(v, h, m, l);
+    f16x8 h, l;
+    split2_8(v, h, l);
     fr[q] = h;
-    fr[total + q] = m;
-    fr[2 * total + q] = l;
+    fr[total + q] = l;
 }
 
 template <bool kVec, int CT>
 __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
-    const float* __restrict__ X, ApplyPlan p, const bf16x8* __restrict__ gfr,
-    float* __restrict__ Y, int16_t* __restrict__ Yq) {
-    extern __shared__ bf16x8 lds[];
+    const float* __restrict__ X, ApplyPlan p, const f16x8* __restrict__ gfr,
+    const uint32_t* __restrict__ sc, float* __restrict__ Y, int16_t* __restrict__ Yq) {
+    extern __shared__ f16x8 alds[];
     const int nfr = CT * p.nK * 64;  // fragments of this column block, per piece
     // XCD-aware numbering: all column blocks of a time block share blockIdx % 8
     const int64_t w = blockIdx.x;
@@ -763,18 +829,19 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     const int cb = static_cast<int>(jx % p.nCB);
     const int64_t tb = (jx / p.nCB) * kXcd + xcd;
     if (tb >= p.nTB) return;
-    bf16x8* fh = lds;
-    bf16x8* fm = lds + nfr;
-    bf16x8* fl = lds + 2 * nfr;
+    f16x8* fh = alds;
+    f16x8* fl = alds + nfr;
     {
         const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
         const int64_t base = static_cast<int64_t>(cb) * nfr;
         for (int q = threadIdx.x; q < nfr; q += blockDim.x) {
             fh[q] = gfr[base + q];
-            fm[q] = gfr[total + base + q];
-            fl[q] = gfr[2 * total + base + q];
+            fl[q] = gfr[total + base + q];
         }
     }
+    const float sx = __uint_as_float(sc[2]);
+    // 1 / (sx * sw): a power of two, so the unscaling is exact
+    const double unscale = 1.0 / (static_cast<double>(sx) * static_cast<double>(__uint_as_float(sc[0])));
     __syncthreads();
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -796,7 +863,7 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int kk = 0; kk < p.nK; ++kk) {
-            bf16x8 ah[kApplyRowTiles], am[kApplyRowTiles], al[kApplyRowTiles];
+            f16x8 ah[kApplyRowTiles], al[kApplyRowTiles];
 #pragma unroll
             for (int rt = 0; rt < kApplyRowTiles; ++rt) {
                 const int64_t t = row0 + rt * 16 + li;
@@ -818,21 +885,28 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                     for (int e = 0; e < 8; ++e)
                         v[e] = (keep[rt] && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
                 }
-                split3_8(v, ah[rt], am[rt], al[rt]);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] *= sx;  // exact: a power of two
+                split2_8(v, ah[rt], al[rt]);
             }
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
                 const int fi = (ct * p.nK + kk) * 64 + lane;
-                const bf16x8 bh = fh[fi];
-                const bf16x8 bm = fm[fi];
-                const bf16x8 bl = fl[fi];
+                const f16x8 bh = fh[fi];
+                const f16x8 bl = fl[fi];
 #pragma unroll
-                for (int rt = 0; rt < kApplyRowTiles; ++rt)
-                    acc[rt][ct] = mfma6(ah[rt], am[rt], al[rt], bh, bm, bl, acc[rt][ct]);
+                for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+                    // smallest terms first
+                    f32x4 a = acc[rt][ct];
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[rt], bh, a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[rt], bl, a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[rt], bh, a, 0, 0, 0);
+                    acc[rt][ct] = a;
+                }
             }
         }
-        // epilogue: quantize(x, 1e-4) = floor(x/1e-4 + 0.5)*1e-4 in double,
-        // stored as float; optional int16 (int)(y/q + 0.5).
+        // epilogue: undo the scales, quantize(x, 1e-4) = floor(x/1e-4 + 0.5)*1e-4
+        // in double, stored as float; optional int16 (int)(y/q + 0.5).
 #pragma unroll
         for (int rt = 0; rt < kApplyRowTiles; ++rt) {
 #pragma unroll
@@ -843,11 +917,9 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                 for (int r = 0; r < 4; ++r) {
                     const int64_t t = row0 + rt * 16 + 4 * lb + r;
                     if (t >= p.N) continue;
-                    const float yf =
-                        p.raw ? acc[rt][ct][r]
-                              : static_cast<float>(
-                                    floor(div_unit(static_cast<double>(acc[rt][ct][r])) + 0.5) *
-                                    0.0001);
+                    const double y = static_cast<double>(acc[rt][ct][r]) * unscale;
+                    const float yf = p.raw ? static_cast<float>(y)
+                                           : static_cast<float>(floor(div_unit(y) + 0.5) * 0.0001);
                     if (Y) Y[t * p.M + i] = yf;
                     if (Yq) {
                         const double s = static_cast<double>(yf) / p.qunit + 0.5;
@@ -1033,15 +1105,16 @@ extern "C" int64_t msamd_whiten_apply_workspace_size(int64_t M) {
     if (M < 1) return -1;
     const ApplyPlan p = apply_plan(M, 1, 0.0);
     Carver c(nullptr);
-    c.take<bf16x8>(3 * static_cast<int64_t>(p.nColTiles) * p.nK * 64);
+    c.take<f16x8>(kApplyPieces * static_cast<int64_t>(p.nColTiles) * p.nK * 64);
+    c.take<uint32_t>(4);
     return c.bytes();
 }
 
 namespace {
 
-int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const uint32_t* rowkeep,
-               float* Y, double quantization_unit, int16_t* Yq, void* workspace,
-               int64_t workspace_bytes, void* stream, bool raw = false) {
+int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const double* C,
+               const uint32_t* rowkeep, float* Y, double quantization_unit, int16_t* Yq,
+               void* workspace, int64_t workspace_bytes, void* stream, bool raw = false) {
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_apply: M=%lld outside 1..1024", (long long)M);
     MSAMD_REQUIRE(N >= 0, "whiten_apply: negative N");
     MSAMD_REQUIRE(Y || Yq, "whiten_apply: no output");
@@ -1051,27 +1124,40 @@ int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const uint
     p.raw = raw ? 1 : 0;
     Carver c(workspace);
     const int64_t nfr = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
-    bf16x8* fr = c.take<bf16x8>(3 * nfr);
+    f16x8* fr = c.take<f16x8>(kApplyPieces * nfr);
+    uint32_t* sc = c.take<uint32_t>(4);
     if (!workspace || workspace_bytes < c.bytes()) {
         set_error("whiten_apply: workspace too small (%lld < %lld)", (long long)workspace_bytes,
                   (long long)c.bytes());
         return MSAMD_ERR_WORKSPACE;
     }
     hipStream_t st = as_stream(stream);
-    apply_prep_kernel<<<static_cast<unsigned>(ceil_div(nfr, 256)), 256, 0, st>>>(W, p, fr);
+    apply_scale_kernel<<<1, 256, 0, st>>>(W, M, C, N, sc);
+    MSAMD_LAUNCH_CHECK();
+    if (!C && N > 0) {
+        int64_t blocks = ceil_div(M * N, 256 * 16);
+        if (blocks > 8192) blocks = 8192;
+        apply_xmax_kernel<<<static_cast<unsigned>(blocks), 256, 0, st>>>(X, M, N, rowkeep, sc);
+        MSAMD_LAUNCH_CHECK();
+    }
+    apply_xscale_kernel<<<1, 64, 0, st>>>(sc);
+    MSAMD_LAUNCH_CHECK();
+    apply_prep_kernel<<<static_cast<unsigned>(ceil_div(nfr, 256)), 256, 0, st>>>(W, p, sc, fr);
     MSAMD_LAUNCH_CHECK();
     if (N == 0) return MSAMD_OK;
-    const size_t lds = sizeof(bf16x8) * 3 * p.CT * p.nK * 64;
+    const size_t lds = sizeof(f16x8) * kApplyPieces * p.CT * p.nK * 64;
     const dim3 grid(static_cast<unsigned>(p.nWG));
     const bool vec = (M % 8 == 0);
-#define MSAMD_APPLY_LAUNCH(V, C)                                                              \
+#define MSAMD_APPLY_LAUNCH(V, CTv)                                                            \
     do {                                                                                      \
-        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<V, C>),      \
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<V, CTv>),    \
                                       hipFuncAttributeMaxDynamicSharedMemorySize,             \
                                       static_cast<int>(lds)));                                \
-        apply_kernel<V, C><<<grid, kApplyWaves * 64, lds, st>>>(X, p, fr, Y, Yq);             \
+        apply_kernel<V, CTv><<<grid, kApplyWaves * 64, lds, st>>>(X, p, fr, sc, Y, Yq);       \
     } while (0)
-    if (p.CT == 4) {
+    if (p.CT == 6) {
+        if (vec) MSAMD_APPLY_LAUNCH(true, 6); else MSAMD_APPLY_LAUNCH(false, 6);
+    } else if (p.CT == 4) {
         if (vec) MSAMD_APPLY_LAUNCH(true, 4); else MSAMD_APPLY_LAUNCH(false, 4);
     } else if (p.CT == 2) {
         if (vec) MSAMD_APPLY_LAUNCH(true, 2); else MSAMD_APPLY_LAUNCH(false, 2);
@@ -1089,7 +1175,7 @@ extern "C" int msamd_whiten_apply(const float* X, int64_t M, int64_t N, const do
                                   double quantization_unit, int16_t* Yq, void* workspace,
                                   int64_t workspace_bytes, void* stream) {
     clear_error();
-    return apply_impl(X, M, N, W, nullptr, Y, quantization_unit, Yq, workspace,
+    return apply_impl(X, M, N, W, nullptr, nullptr, Y, quantization_unit, Yq, workspace,
                       workspace_bytes, stream);
 }
 
@@ -1099,7 +1185,7 @@ extern "C" int msamd_whiten_apply_rows(const float* X, int64_t M, int64_t N, con
                                        int64_t workspace_bytes, void* stream) {
     clear_error();
     MSAMD_REQUIRE(row_keep, "whiten_apply_rows: null row_keep");
-    return apply_impl(X, M, N, W, row_keep, Y, quantization_unit, Yq, workspace,
+    return apply_impl(X, M, N, W, nullptr, row_keep, Y, quantization_unit, Yq, workspace,
                       workspace_bytes, stream);
 }
 
@@ -1111,6 +1197,16 @@ extern "C" int msamd_whiten_clips(const float* clips, int64_t M, int64_t T, int6
     MSAMD_REQUIRE(T >= 0 && L >= 0 && (T == 0 || L <= INT64_MAX / T), "whiten_clips: bad shape");
     if (T * L == 0) return MSAMD_OK;  // no clips: nothing to write
     // M x T x L clips are an M x (T*L) array: the apply without the 1e-4 quantize
-    return apply_impl(clips, M, T * L, W, nullptr, clips_out, quantization_unit, clips_out_q,
-                      workspace, workspace_bytes, stream, true);
+    return apply_impl(clips, M, T * L, W, nullptr, nullptr, clips_out, quantization_unit,
+                      clips_out_q, workspace, workspace_bytes, stream, true);
+}
+
+extern "C" int msamd_whiten_apply_cov(const float* X, int64_t M, int64_t N, const double* W,
+                                      const double* XXt, const uint32_t* row_keep, float* Y,
+                                      double quantization_unit, int16_t* Yq, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(XXt, "whiten_apply_cov: null covariance");
+    return apply_impl(X, M, N, W, XXt, row_keep, Y, quantization_unit, Yq, workspace,
+                      workspace_bytes, stream);
 }

@@ -401,12 +401,14 @@ def whitening_matrix(C, exact=False, return_info=False):
     return Wcm.t()  # [row, col] view of the column-major result
 
 
-def whiten_apply(X, W, quantization_unit=0.0, out=None, row_keep=None):
+def whiten_apply(X, W, quantization_unit=0.0, out=None, row_keep=None, xxt=None):
     """quantize(W applied to every sample, 1e-4) (p_whiten.cpp:153-206).
 
     Returns float32 (N, M); with quantization_unit > 0 returns int16 values
     (int)(y/q + 0.5) (p_whiten.cpp:445-451).  row_keep: mask_rows' bitmap;
-    masked rows are read as zero."""
+    masked rows are read as zero.  xxt: the covariance of these rows (as
+    whiten_covariance returned it); its diagonal bounds the samples, which
+    spares the library a pass over X to find their range."""
     _check_ts(X)
     N, M = X.shape
     dev = X.device
@@ -416,8 +418,18 @@ def whiten_apply(X, W, quantization_unit=0.0, out=None, row_keep=None):
     q = float(quantization_unit)
     rk = _row_keep(row_keep, N, dev)
 
+    C = None
+    if xxt is not None:
+        if xxt.shape != (M, M) or xxt.device != dev:
+            raise ValueError("xxt must be the (M, M) covariance on the timeseries' device")
+        C = xxt.to(torch.float64).contiguous()  # symmetric: row- and column-major agree
+
     def call(Yp, qq, Yqp):
-        if rk is None:
+        if C is not None:
+            _capi.call("msamd_whiten_apply_cov", _ptr(X), M, N, _ptr(Wcm), _ptr(C),
+                       _ptr(rk) if rk is not None else None, Yp, qq, Yqp, _ptr(ws), ws.numel(),
+                       _stream(dev))
+        elif rk is None:
             _capi.call("msamd_whiten_apply", _ptr(X), M, N, _ptr(Wcm), Yp, qq, Yqp, _ptr(ws),
                        ws.numel(), _stream(dev))
         else:

@@ -173,6 +173,13 @@ def test_whiten_apply_within_one_quantum(gpu, M, N):
     assert d.max() <= 1
     Yo = oc.whiten_apply(X, W)
     assert np.max(np.abs(np.round(Yo.astype(np.float64) / 1e-4) - k_gpu)) <= 1
+    # the covariance-scaled form (no range pass over X): the same bar
+    Cg = torch.from_numpy(C).to(gpu)
+    Yc = fe.whiten_apply(torch.from_numpy(X).to(gpu), torch.from_numpy(W).to(gpu),
+                         xxt=Cg).cpu().numpy()
+    dc = np.abs(np.round(Yc.astype(np.float64) / 1e-4) - k64)
+    say(f"  covariance-scaled: off-by-one fraction {np.mean(dc > 0):.4f}; max {dc.max():.0f}")
+    assert dc.max() <= 1
     Yq = fe.whiten_apply(torch.from_numpy(X).to(gpu), torch.from_numpy(W).to(gpu),
                          quantization_unit=0.01).cpu().numpy()
     np.testing.assert_array_equal(Yq, (Yg.astype(np.float64) / 0.01 + 0.5).astype(np.int32).astype(np.int16))
@@ -272,6 +279,9 @@ def test_masked_whitening_without_the_masked_copy(gpu, M, N, isz):
     Y1 = fe.whiten_apply(Xm, W)
     Y2 = fe.whiten_apply(Xg, W, row_keep=rk)
     assert torch.equal(Y1, Y2)
+    # with the covariance bound instead of the range pass: within one quantum
+    Y3 = fe.whiten_apply(Xg, W, row_keep=rk, xxt=C2)
+    assert torch.max(torch.abs(torch.round(Y3.double() / 1e-4) - torch.round(Y1.double() / 1e-4))) <= 1
 
 
 @pytest.mark.parametrize("L,T,M", [(1000, 50, 12), (37, 50, 64), (500, 20, 384), (0, 50, 8)])
@@ -296,3 +306,23 @@ def test_whiten_clips_within_fp32_of_fp64(gpu, L, T, M):
     assert e < 1e-5
     ref = oc.whiten_clips(clips, W)
     assert np.max(np.abs(Y - ref)) / rms < 1e-5
+
+
+@pytest.mark.parametrize("scale", [1e-5, 1e5, 3e7])
+def test_whiten_apply_scaling_extremes(gpu, scale):
+    """The fp16 split scales both operands by powers of two: data far from
+    unit size (W compensating) still whitens within one quantum of FP64."""
+    from mountainsort_amd import frontend as fe
+    M, N = 96, 20_000
+    say(f"whiten apply scale={scale:g}")
+    X = (_recording(M, N, seed=77) * scale).astype(np.float32)
+    C = oc.covariance_fp64(X)
+    w, V = np.linalg.eigh(C)
+    W = (V / np.sqrt(w)) @ V.T
+    k64 = np.floor(oc.whiten_apply_fp64(X, W) / 1e-4 + 0.5)
+    for xxt in (None, torch.from_numpy(C).to(gpu)):
+        Y = fe.whiten_apply(torch.from_numpy(X).to(gpu), torch.from_numpy(W).to(gpu),
+                            xxt=xxt).cpu().numpy()
+        d = np.abs(np.round(Y.astype(np.float64) / 1e-4) - k64)
+        say(f"  xxt={'yes' if xxt is not None else 'no'}: off-by-one {np.mean(d > 0):.4f}, max {d.max():.0f}")
+        assert d.max() <= 1
