@@ -202,8 +202,9 @@ bool run_bandpass(Ctx& c) {
 // Covariance and whitening matrix of a device timeseries (M x N float32) into
 // a device M x M float64 column-major buffer.
 bool whitening_matrix_of(Ctx& c, const float* X, int64_t M, int64_t N, DeviceBuffer& Wd,
-                         hipStream_t s) {
-    DeviceBuffer C, ws1, ws2;
+                         hipStream_t s, DeviceBuffer* Cout = nullptr) {
+    DeviceBuffer Cl, ws1, ws2;
+    DeviceBuffer& C = Cout ? *Cout : Cl;
     if (!C.alloc(M * M * 8, c.err) || !Wd.alloc(M * M * 8, c.err) ||
         !ws1.alloc(msamd_whiten_covariance_workspace_size(M, N), c.err) ||
         !ws2.alloc(msamd_whitening_matrix_workspace_size(M), c.err))
@@ -215,18 +216,20 @@ bool whitening_matrix_of(Ctx& c, const float* X, int64_t M, int64_t N, DeviceBuf
                   "whitening_matrix");
 }
 
-// Applies W (device, column-major M x M) and writes the result.
+// Applies W (device, column-major M x M) and writes the result.  C (nullable):
+// the covariance of X, which bounds its range for the apply's scaling.
 bool apply_and_write(Ctx& c, const float* X, int64_t M, int64_t N, const double* W, double q,
-                     hipStream_t s) {
+                     hipStream_t s, const double* C = nullptr) {
     DeviceBuffer Y, ws;
     const int32_t out_type = q > 0 ? MDA_INT16 : MDA_FLOAT32;
     if (!Y.alloc(M * N * mda_bytes_per_entry(out_type), c.err) ||
         !ws.alloc(msamd_whiten_apply_workspace_size(M), c.err))
         return false;
-    const int st = q > 0 ? msamd_whiten_apply(X, M, N, W, nullptr, q, Y.as<int16_t>(), ws.get(),
+    float* Yf = q > 0 ? nullptr : Y.as<float>();
+    int16_t* Yq = q > 0 ? Y.as<int16_t>() : nullptr;
+    const int st = C ? msamd_whiten_apply_cov(X, M, N, W, C, nullptr, Yf, q, Yq, ws.get(),
                                               ws.bytes(), s)
-                         : msamd_whiten_apply(X, M, N, W, Y.as<float>(), 0.0, nullptr, ws.get(),
-                                              ws.bytes(), s);
+                     : msamd_whiten_apply(X, M, N, W, Yf, q, Yq, ws.get(), ws.bytes(), s);
     if (!lib_ok(c, st, "whiten_apply")) return false;
     return write_device(c, c.str("timeseries_out"), out_type, {M, N}, Y.get(), s);
 }
@@ -241,9 +244,10 @@ bool run_whiten(Ctx& c) {
     DeviceBuffer X, W;
     if (!open_timeseries(c, "timeseries", src) || !s.create(c.err) || !load_f32(c, src, X, s.get()))
         return false;
-    return whitening_matrix_of(c, X.as<float>(), src.M, src.N, W, s.get()) &&
+    DeviceBuffer C;
+    return whitening_matrix_of(c, X.as<float>(), src.M, src.N, W, s.get(), &C) &&
            apply_and_write(c, X.as<float>(), src.M, src.N, W.as<double>(),
-                           c.num("quantization_unit"), s.get());
+                           c.num("quantization_unit"), s.get(), C.as<double>());
 }
 
 bool run_compute_whitening_matrix(Ctx& c) {
