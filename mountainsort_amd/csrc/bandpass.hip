@@ -483,6 +483,16 @@ constexpr int64_t kScratchBudget = static_cast<int64_t>(MSAMD_BP_SCRATCH_MB) << 
 }  // namespace
 }  // namespace msamd
 
+namespace msamd {
+// Windows of any other length (bandpass_bluestein.hip).
+int64_t bluestein_workspace_size(int64_t M, int64_t N, int64_t cs, int64_t ov);
+bool bluestein_supported(int64_t n);
+int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, double samplerate,
+                     double freq_min, double freq_max, double freq_wid, int64_t cs, int64_t ov,
+                     int k_sub, double qunit, void* Y, void* workspace, int64_t workspace_bytes,
+                     hipStream_t st);
+}  // namespace msamd
+
 using namespace msamd;
 
 extern "C" int64_t msamd_bandpass_chunk_size(int64_t M, int64_t num_threads) {
@@ -525,13 +535,20 @@ extern "C" int msamd_bandpass_kernel(int64_t n_fft, double samplerate, double fr
 
 extern "C" int msamd_bandpass_supported(int64_t chunk_size, int64_t overlap) {
     Shape s;
-    return pick_shape(chunk_size + 2 * overlap, &s) ? 1 : 0;
+    if (chunk_size < 1 || overlap < 0) return 0;
+    return (pick_shape(chunk_size + 2 * overlap, &s) || bluestein_supported(chunk_size + 2 * overlap))
+               ? 1
+               : 0;
 }
 
 extern "C" int64_t msamd_bandpass_workspace_size(int64_t M, int64_t N, int64_t chunk_size,
                                                  int64_t overlap) {
     Shape sh;
-    if (M < 1 || N < 0 || chunk_size < 1 || !pick_shape(chunk_size + 2 * overlap, &sh)) return -1;
+    if (M < 1 || N < 0 || chunk_size < 1 || overlap < 0) return -1;
+    if (!pick_shape(chunk_size + 2 * overlap, &sh))
+        return bluestein_supported(chunk_size + 2 * overlap)
+                   ? bluestein_workspace_size(M, N, chunk_size, overlap)
+                   : -1;
     const BpPlan p = make_plan(M, N > 0 ? N : 1, chunk_size, overlap, 1, 0.0, false, sh,
                                kScratchBudget);
     int64_t bytes = 0;
@@ -554,9 +571,15 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     const int k_sub = subsample_factor > 0 ? subsample_factor : 1;
     Shape sh;
     if (!pick_shape(chunk_size + 2 * overlap, &sh)) {
-        set_error("bandpass: FFT length %lld has no compiled factorization",
-                  (long long)(chunk_size + 2 * overlap));
-        return MSAMD_ERR_UNSUPPORTED;
+        // no four-step factorization: Bluestein's chirp-z form of the same window filter
+        if (!bluestein_supported(chunk_size + 2 * overlap)) {
+            set_error("bandpass: FFT length %lld is beyond the supported maximum (2^25)",
+                      (long long)(chunk_size + 2 * overlap));
+            return MSAMD_ERR_UNSUPPORTED;
+        }
+        return bluestein_filter(X, dtype == MSAMD_DTYPE_INT16, M, N, samplerate, freq_min,
+                                freq_max, freq_wid, chunk_size, overlap, k_sub,
+                                quantization_unit, Y, workspace, workspace_bytes, as_stream(stream));
     }
     const BpPlan p = make_plan(M, N, chunk_size, overlap, k_sub, quantization_unit,
                                dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget);

@@ -65,7 +65,11 @@ def test_bandpass_host_entry_points():
                                    H.ctypes.data_as(ctypes.c_void_p)) == 0
     np.testing.assert_array_equal(H, oc.define_kernel(240000, 30000, 300, 6000, 1000))
     assert L.msamd_bandpass_supported(120000, 60000) == 1
-    assert L.msamd_bandpass_supported(1278101, 60000) == 0
+    # any other window length goes through the chirp-z form, up to 2^25 points
+    assert L.msamd_bandpass_supported(1278101, 60000) == 1
+    assert L.msamd_bandpass_workspace_size(4, 1_800_000, 1278101, 60000) > 0
+    assert L.msamd_bandpass_supported((1 << 25) - 120000, 60000) == 1
+    assert L.msamd_bandpass_supported((1 << 25) - 120000 + 2, 60000) == 0
     assert L.msamd_bandpass_workspace_size(384, 9_000_000, 120000, 60000) > 0
 
 

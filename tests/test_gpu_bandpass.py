@@ -1,5 +1,6 @@
 """GPU parity: the windowed FFT bandpass against the FP64 oracle with the
-reference's windows (chunk 120000, overlap 60000, n = 240000)."""
+reference's windows (chunk 120000, overlap 60000, n = 240000 by the
+four-step FFT; other window lengths by the chirp-z form)."""
 import numpy as np
 import pytest
 import torch
@@ -73,9 +74,36 @@ def test_bandpass_subsample_and_quantize(gpu):
     assert d.max() <= 1 and np.mean(d > 0) < 1e-3
 
 
-def test_bandpass_unsupported_length_is_an_error(gpu):
+@pytest.mark.parametrize("M,N,dtype,T,cs,k,q", [
+    (4, 400_000, np.float32, 8, None, 1, 0.0),      # the 4-channel 8-thread plan: n = 1398101
+    (3, 250_000, np.int16, 8, 100_003, 1, 0.0),     # odd channels, several windows, n = 220003
+    (5, 260_000, np.float32, 8, 77_777, 3, 0.5),    # subsample 3, int16 output, n = 197777
+])
+def test_bandpass_any_window_length(gpu, M, N, dtype, T, cs, k, q):
+    """Window lengths without a four-step factorization go through the
+    chirp-z (Bluestein) form: same windows, within 1e-4 of the FP64 oracle."""
+    from mountainsort_amd import frontend as fe
+    n = (cs or fe.bandpass_chunk_size(M, T)) + 120000
+    say(f"bandpass (chirp-z) M={M} N={N} n={n} k={k} q={q}")
+    X = _signal(N, M, seed=M + 7, dtype=dtype)
+    Yg = fe.bandpass_filter(torch.from_numpy(X).to(gpu), 30000, 300, 6000, 1000, num_threads=T,
+                            chunk_size=cs, subsample_factor=k, quantization_unit=q).cpu().numpy()
+    Yo = oc.bandpass_filter(X, 30000, 300, 6000, 1000, num_threads=T, chunk_size=cs,
+                            subsample_factor=k, quantization_unit=q)
+    assert Yg.shape == Yo.shape
+    if q:
+        d = np.abs(Yg.astype(np.int32) - Yo.astype(np.int32))
+        say(f"  int16 outputs: max diff {d.max()}, fraction differing {np.mean(d > 0):.2e}")
+        assert d.max() <= 1 and np.mean(d > 0) < 1e-3
+    else:
+        err = _rel_err(Yg.astype(np.float64), Yo)
+        say(f"  max |y_gpu - y_fp64| / rms = {err:.3e}")
+        assert err < 1e-4
+
+
+def test_bandpass_too_long_window_is_an_error(gpu):
     from mountainsort_amd import frontend as fe
     from mountainsort_amd._capi import MsamdError
     X = torch.zeros((1000, 4), device=gpu)
     with pytest.raises(MsamdError):
-        fe.bandpass_filter(X, 30000, 300, 6000, 1000, chunk_size=1278101)
+        fe.bandpass_filter(X, 30000, 300, 6000, 1000, chunk_size=1 << 25)
