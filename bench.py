@@ -122,13 +122,19 @@ class Pipeline:
 
 
 # Single-launch kernels whose per-launch time the stage events measure (the
-# stage's ABI call launches this kernel plus at most a few-microsecond helper),
-# with their algorithmic work per channel-sample: flops for MFMA-bound kernels
-# (2M per sample for covariance and apply, the split-BF16 terms not counted)
-# and fp64 flops for the power method (11 matvecs of 2M^2 per component).
+# stage's ABI call launches this kernel plus at most a few-microsecond helper).
+# The bound is the roof the kernel's algorithmic work meets first: the apply
+# reads one float32 sample row and writes one (8 bytes per channel-sample)
+# against 2M flops per channel-sample, 96 flop/B at M = 384, below the
+# 312 flop/B ridge of dense fp16 MFMA over HBM, so it is HBM-bound; the
+# covariance reads 4 bytes against 2M flops (192 flop/B) and is priced
+# against MFMA; the power method against fp64 (11 matvecs of 2M^2 per
+# component).  split_terms is the MFMA products the kernel issues per
+# algorithmic product (split-FP16 / split-BF16 pieces).
 ROOFLINE_KERNELS = {
-    "apply": {"kernel": "apply_kernel", "bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",
-              "dtype_peak": "bf16 dense", "split_terms": 6},
+    "apply": {"kernel": "apply_kernel", "bound": "hbm", "peak": 8000.0, "unit": "GB/s",
+              "bytes_per_chs": 8, "mfma_peak": 2500.0, "dtype_peak": "fp16 dense",
+              "split_terms": 3},
     "covariance": {"kernel": "cov_all_kernel", "bound": "mfma", "peak": 2500.0,
                    "unit": "TFLOP/s", "dtype_peak": "bf16 dense", "split_terms": 3},
     "whitening_matrix": {"kernel": "whitening_power_kernel", "bound": "mfma", "peak": 78.6,
@@ -142,12 +148,22 @@ def roofline_of(stage, ms, M, N, traffic_file):
         flops = 11.0 * 2.0 * M * M * M
     else:
         flops = 2.0 * M * M * N
-    achieved = flops / (ms * 1e-3) / 1e12
-    out = {"bound": k["bound"], "kernel": k["kernel"], "achieved": round(achieved, 3),
-           "peak": k["peak"], "unit": k["unit"], "frac": round(achieved / k["peak"], 5),
-           "traffic": None, "algorithmic_flops_per_launch": flops, "launch_ms": round(ms, 4),
-           "peak_dtype": k["dtype_peak"], "mfma_terms_per_product": k["split_terms"],
-           "mfma_issue_frac": round(achieved * k["split_terms"] / k["peak"], 5)}
+    tflops = flops / (ms * 1e-3) / 1e12
+    mfma_peak = k.get("mfma_peak", k["peak"])
+    mfma = {"algorithmic_flops_per_launch": flops, "mfma_TFLOPs": round(tflops, 3),
+            "peak_dtype": k["dtype_peak"], "mfma_terms_per_product": k["split_terms"],
+            "mfma_issue_frac": round(tflops * k["split_terms"] / mfma_peak, 5)}
+    if k["bound"] == "hbm":
+        alg = float(k["bytes_per_chs"]) * M * N
+        achieved = alg / (ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "kernel": k["kernel"], "achieved": round(achieved, 1),
+               "peak": k["peak"], "unit": k["unit"], "frac": round(achieved / k["peak"], 5),
+               "traffic": None, "algorithmic_bytes_per_launch": alg,
+               "launch_ms": round(ms, 4), **mfma}
+    else:
+        out = {"bound": k["bound"], "kernel": k["kernel"], "achieved": round(tflops, 3),
+               "peak": k["peak"], "unit": k["unit"], "frac": round(tflops / k["peak"], 5),
+               "traffic": None, "launch_ms": round(ms, 4), **mfma}
     try:
         with open(traffic_file) as f:
             t = json.load(f)

@@ -569,6 +569,13 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     MSAMD_REQUIRE(chunk_size >= 1 && overlap >= 0, "bandpass: bad chunk plan");
     MSAMD_REQUIRE(samplerate > 0, "bandpass: samplerate must be positive");
     const int k_sub = subsample_factor > 0 ? subsample_factor : 1;
+    if (k_sub > 1 && chunk_size % k_sub != 0 && N / k_sub > 0) {
+        // Window w writes its cs/k_sub outputs from (w*cs)/k_sub on, which
+        // leaves a row between windows when k_sub does not divide cs; the
+        // reference's output file is zero there (DiskWriteMda pre-zeroes).
+        const int64_t bytes = (N / k_sub) * M * (quantization_unit != 0 ? 2 : 4);
+        MSAMD_HIP(hipMemsetAsync(Y, 0, static_cast<size_t>(bytes), as_stream(stream)));
+    }
     Shape sh;
     if (!pick_shape(chunk_size + 2 * overlap, &sh)) {
         // no four-step factorization: Bluestein's chirp-z form of the same window filter

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 measurements of this library's kernels.
 
-``python -m mountainsort_amd.profiling <kernel_stats.csv> <title> [command] [passes]``
+``python -m mountainsort_amd.profiling <kernel_stats.csv> <title> [command] [passes] [note]``
 prints a markdown table of the msamd kernels in a rocprofv3
 ``--stats --output-format csv`` kernel_stats.csv (other kernels, e.g. torch's
 synthetic-data generation outside the timed region, are summed into one
@@ -18,16 +18,61 @@ import re
 import sys
 
 
+def _demangle_nested(name):
+    """Readable form of an Itanium-mangled nested kernel name that the
+    profiler left mangled (it gives up on _Float16 vector parameters, DF16_):
+    the qualified name and its integral / bool template arguments, e.g.
+    _ZN5msamd12_GLOBAL__N_112apply_kernelILb1ELi6EEEv... -> msamd::apply_kernel<true, 6>.
+    Anything else is returned unchanged."""
+    if not name.startswith("_ZN"):
+        return name
+    i, parts = 3, []
+    try:
+        while i < len(name) and name[i] != "E":
+            if name[i].isdigit():
+                j = i
+                while name[j].isdigit():
+                    j += 1
+                n = int(name[i:j])
+                ident = name[j:j + n]
+                i = j + n
+                if ident != "_GLOBAL__N_1":
+                    parts.append(ident)
+            elif name[i] == "I":
+                i += 1
+                args = []
+                while name[i] != "E":
+                    if name[i] != "L":
+                        return name
+                    k = name.index("E", i)
+                    lit = name[i + 1:k]
+                    if lit[0] == "b":
+                        args.append("true" if lit[1:] == "1" else "false")
+                    else:
+                        v = lit[1:]
+                        args.append("-" + v[1:] if v.startswith("n") else v)
+                    i = k + 1
+                i += 1
+                parts[-1] += "<" + ", ".join(args) + ">"
+            else:
+                return name
+    except (IndexError, ValueError):
+        return name
+    return "::".join(parts) if parts else name
+
+
 def clean(name):
+    name = _demangle_nested(name)
     name = name.replace("(anonymous namespace)::", "")
     name = re.sub(r"^void ", "", name)
     return re.sub(r"\(.*", "", name)
 
 
-def summarize(path, title, command=None, per=1):
+def summarize(path, title, command=None, per=1, note=None):
     rows = list(csv.DictReader(open(path)))
-    ours = [r for r in rows if "msamd::" in r["Name"]]
-    other_ns = sum(float(r["TotalDurationNs"]) for r in rows if "msamd::" not in r["Name"])
+    ours = [r for r in rows if clean(r["Name"]).startswith("msamd::")]
+    other_ns = sum(float(r["TotalDurationNs"]) for r in rows
+                   if not clean(r["Name"]).startswith("msamd::"))
     total = sum(float(r["TotalDurationNs"]) for r in ours)
     out = [f"# {title}", ""]
     if command:
@@ -42,6 +87,8 @@ def summarize(path, title, command=None, per=1):
     out += ["", f"msamd kernels total: {total / 1e6:.3f} ms over the profiled run "
             f"({total / 1e6 / per:.3f} ms per pass when the run made {per} passes); "
             f"other kernels (input generation, torch): {other_ns / 1e6:.3f} ms."]
+    if note:
+        out += ["", note]
     return "\n".join(out) + "\n"
 
 
@@ -172,4 +219,5 @@ if __name__ == "__main__":
     else:
         cmd = sys.argv[3] if len(sys.argv) > 3 else None
         per = int(sys.argv[4]) if len(sys.argv) > 4 else 1
-        sys.stdout.write(summarize(sys.argv[1], sys.argv[2], cmd, per))
+        note = sys.argv[5] if len(sys.argv) > 5 else None
+        sys.stdout.write(summarize(sys.argv[1], sys.argv[2], cmd, per, note))
