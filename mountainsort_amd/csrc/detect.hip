@@ -136,8 +136,9 @@ __device__ __forceinline__ double detect_value(float x, double mean, int sign) {
 // One thread per (segment, central channel), lanes over consecutive channels
 // so every load is one coalesced row.  The thread streams its segment in
 // register batches of kB samples, loading the next batch while it tests the
-// current one.  Threshold crossings are rare; their +-D neighbours are read
-// again through the cache (the rows were just loaded by the wavefront).
+// current one.  Threshold crossings are rare; for D <= kB their +-D
+// neighbours are in the previous, current and next batches held in
+// registers, otherwise they are read again through the cache.
 __device__ __forceinline__ float load_x(const float* xp, int64_t t, int64_t M, int64_t N) {
     return (t >= 0 && t < N) ? xp[t * M] : 0.0f;
 }
@@ -161,8 +162,56 @@ __device__ __forceinline__ bool stencil_ok(const float* xp, int64_t t, double v,
     return true;
 }
 
+// detect_value with the sign fixed at compile time
+template <int kSign>
+__device__ __forceinline__ double detect_value_s(float x, double mean) {
+    const double v = static_cast<double>(x) - mean;
+    return kSign < 0 ? -v : (kSign == 0 ? fabs(v) : v);
+}
+
+// The +-D test of sample t for D <= kB, decided in float whenever float
+// decides it exactly: 1 keep, 0 reject, -1 undecided (left to the FP64 test).
+// The 2D neighbours are loaded together (one trip through the cache rather
+// than one per neighbour).
+//
+// With s = kSign = +-1, v = s * RN(x - mean) is a non-decreasing function of
+// the key s*x, strictly increasing except where the rounding of x - mean
+// merges two inputs, which needs the keys to differ by less than
+// 2^-51 * (|x| + |mean|).  So with bk / fk the largest key among the earlier
+// / later neighbours inside D and the padded window:
+//   bk >= kt                 some earlier v >= v_t: reject (exact)
+//   fk - kt > tau            some later v > v_t: reject (no merge possible)
+//   fk <= kt, kt - bk > tau  every earlier v < v_t, later v <= v_t: keep
+// with tau = 2^-40 * (|x_t| + |mean|) far above the merge distance; keys
+// within tau stay undecided.
+template <int kSign>
+__device__ __forceinline__ int stencil_float(const float* xp, int64_t t, float xt, float mabs,
+                                             int D, int64_t wa, int64_t wb, int64_t M,
+                                             int64_t N) {
+    const int nbk = static_cast<int>(t - wa < D ? t - wa : D);
+    const int nfw = static_cast<int>(wb - 1 - t < D ? wb - 1 - t : D);
+    float xb[kB], xf[kB];
+#pragma unroll
+    for (int d = 1; d <= kB; ++d) {
+        xb[d - 1] = d <= nbk ? load_x(xp, t - d, M, N) : 0.0f;
+        xf[d - 1] = d <= nfw ? load_x(xp, t + d, M, N) : 0.0f;
+    }
+    const float kt = kSign < 0 ? -xt : xt;
+    float bk = -INFINITY, fk = -INFINITY;
+#pragma unroll
+    for (int d = 1; d <= kB; ++d) {
+        if (d <= nbk) bk = fmaxf(bk, kSign < 0 ? -xb[d - 1] : xb[d - 1]);
+        if (d <= nfw) fk = fmaxf(fk, kSign < 0 ? -xf[d - 1] : xf[d - 1]);
+    }
+    const float tau = 0x1p-40f * (fabsf(xt) + mabs);
+    if (bk >= kt || fk - kt > tau) return 0;
+    if (fk <= kt && kt - bk > tau) return 1;
+    return -1;
+}
+
+template <int kSign>
 __global__ void __launch_bounds__(256) detect_scan_kernel(
-    const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels, int sign,
+    const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels,
     const double* __restrict__ mean_in, const double* __restrict__ thr_in,
     int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -187,6 +236,10 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
         uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
         const int64_t T0 = core0 + r0;
         const int64_t T1 = core0 + r1;
+        // one-sided sign and a window within kB: the float test first
+        const bool fast = kSign != 0 && p.D <= kB;
+        // |mean| rounded up to float
+        const float mabs = static_cast<float>(fabs(mean)) * (1.0f + 0x1p-20f);
         float cur[kB], nxt[kB];
 #pragma unroll
         for (int i = 0; i < kB; ++i) cur[i] = load_x(xp, T0 + i, p.M, p.N_file);
@@ -196,14 +249,18 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
             uint32_t over = 0;  // threshold crossings of this batch
 #pragma unroll
             for (int i = 0; i < kB; ++i)
-                if (tb + i < T1 && detect_value(cur[i], mean, sign) > thr) over |= 1u << i;
-            while (over) {  // in time order; kept events are > D apart, so cnt < cap
+                if (tb + i < T1 && detect_value_s<kSign>(cur[i], mean) > thr) over |= 1u << i;
+            // in time order; kept events are > D apart, so cnt < cap
+            while (over) {
                 const int i = __builtin_ctz(over);
                 over &= over - 1;
                 const int64_t t = tb + i;
-                const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
-                if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
-                    my_slots[cnt++] = static_cast<uint16_t>(t - T0);
+                const float xt = load_x(xp, t, p.M, p.N_file);
+                int keep = fast ? stencil_float<kSign>(xp, t, xt, mabs, p.D, wa, wb, p.M, p.N_file) : -1;
+                if (keep < 0)
+                    keep = stencil_ok(xp, t, detect_value(xt, mean, kSign), mean, kSign, p.D, wa, wb,
+                                      p.M, p.N_file);
+                if (keep && cnt < p.cap) my_slots[cnt++] = static_cast<uint16_t>(t - T0);
             }
 #pragma unroll
             for (int i = 0; i < kB; ++i) cur[i] = nxt[i];
@@ -431,8 +488,13 @@ extern "C" int msamd_detect_events(const float* X, int64_t M, int64_t N_file, in
     }
     {
         const int64_t n = p.nseg * p.Mc;
-        detect_scan_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
-            X, p, channels, detect_sign, b.mean, b.thr, b.counts, b.slots);
+        const dim3 grid(static_cast<unsigned>(ceil_div(n, 256)));
+        if (detect_sign < 0)
+            detect_scan_kernel<-1><<<grid, 256, 0, st>>>(X, p, channels, b.mean, b.thr, b.counts, b.slots);
+        else if (detect_sign == 0)
+            detect_scan_kernel<0><<<grid, 256, 0, st>>>(X, p, channels, b.mean, b.thr, b.counts, b.slots);
+        else
+            detect_scan_kernel<1><<<grid, 256, 0, st>>>(X, p, channels, b.mean, b.thr, b.counts, b.slots);
         MSAMD_LAUNCH_CHECK();
     }
     detect_seg_scan_kernel<<<dim3(p.Mc), 1024, 0, st>>>(b.counts, p.nseg, b.seg_off,

@@ -670,9 +670,9 @@ __global__ void __launch_bounds__(256) whitening_assemble_kernel(
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kApplyWaves = 8;
-constexpr int kApplyRowTiles = 4;    // 16-sample row tiles per wave per pass
-constexpr int kApplyPasses = 4;      // passes per workgroup
-constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 2048 samples
+constexpr int kApplyRowTiles = 3;    // 16-sample row tiles per wave per pass
+constexpr int kApplyPasses = 6;      // passes per workgroup
+constexpr int kApplyTB = kApplyWaves * kApplyRowTiles * 16 * kApplyPasses;  // 2304 samples
 constexpr int kXcd = 8;
 constexpr int kApplyPieces = 2;
 
@@ -705,17 +705,6 @@ inline ApplyPlan apply_plan(int64_t M, int64_t N, double qunit) {
     p.nTB = ceil_div(N, kApplyTB);
     p.nWG = align_up(p.nTB, kXcd) * p.nCB;
     return p;
-}
-
-// x / 1e-4 in double without the division sequence: q0 = x * RN(1/u), the
-// exact remainder e = x - q0*u by FMA, then one correction q0 + e * RN(1/u)
-// (Markstein); it matched IEEE division on 2e8 random floats.
-__device__ __forceinline__ double div_unit(double x) {
-    constexpr double u = 0.0001;
-    constexpr double r = 1.0 / u;
-    const double q0 = x * r;
-    const double e = __builtin_fma(-q0, u, x);
-    return __builtin_fma(e, r, q0);
 }
 
 __device__ __forceinline__ void split2_8(const float* v, f16x8& h, f16x8& l) {
@@ -842,6 +831,12 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     const float sx = __uint_as_float(sc[2]);
     // 1 / (sx * sw): a power of two, so the unscaling is exact
     const double unscale = 1.0 / (static_cast<double>(sx) * static_cast<double>(__uint_as_float(sc[0])));
+    // For a float y, y / 1e-4 in double equals y * 10000 exactly: y * 10000 has
+    // at most 24 + 10 significant bits, and 1e-4 in double is 1e-4 * (1 + 4.8e-17),
+    // a relative change below half an ulp, so the correctly rounded quotient is
+    // y * 10000 itself.  The quantizer's product acc * (unscale * 10000) is
+    // therefore the quotient of the FP64 formula, with no rounding.
+    const double qscale = unscale * 10000.0;
     __syncthreads();
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -851,44 +846,52 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     for (int pass = 0; pass < kApplyPasses; ++pass) {
         const int64_t row0 = tblock + (static_cast<int64_t>(pass) * kApplyWaves + wid) * (kApplyRowTiles * 16);
         if (row0 >= p.N) break;
-        bool keep[kApplyRowTiles];  // this lane's rows: in range and not masked
+        // Loads are unconditional: rows past the end read the last row and
+        // columns past the end read the row's last columns, which meet zero
+        // rows of W; masked rows are computed and replaced by zero on output.
+        const float* rowp[kApplyRowTiles];
 #pragma unroll
         for (int rt = 0; rt < kApplyRowTiles; ++rt) {
             const int64_t t = row0 + rt * 16 + li;
-            keep[rt] = t < p.N && row_kept(p.rowkeep, t);
+            rowp[rt] = X + (t < p.N ? t : p.N - 1) * p.M;
         }
+        typedef vf4 Samples[kApplyRowTiles][2];  // one step: 8 samples per row tile
+        const auto load_step = [&](int kk, Samples& nx) {
+            const int64_t j0 = static_cast<int64_t>(kk) * 32 + 8 * lb;
+#pragma unroll
+            for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+                if (kVec) {
+                    // M % 8 == 0: j0 < M implies j0 + 8 <= M
+                    const vf4* src = reinterpret_cast<const vf4*>(rowp[rt] + (j0 < p.M ? j0 : p.M - 8));
+                    nx[rt][0] = src[0];
+                    nx[rt][1] = src[1];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        nx[rt][e >> 2][e & 3] = rowp[rt][j0 + e < p.M ? j0 + e : p.M - 1];
+                }
+            }
+        };
+        // two FP16 pieces of x * sx (exact: a power of two); each piece is one
+        // fused multiply-add rounded once to FP16
+        const auto split_step = [&](const Samples& nx, f16x8* h, f16x8* l) {
+#pragma unroll
+            for (int rt = 0; rt < kApplyRowTiles; ++rt) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float x = nx[rt][e >> 2][e & 3];
+                    const _Float16 hh = static_cast<_Float16>(__builtin_fmaf(x, sx, 0.0f));
+                    h[rt][e] = hh;
+                    l[rt][e] = static_cast<_Float16>(__builtin_fmaf(x, sx, -static_cast<float>(hh)));
+                }
+            }
+        };
         f32x4 acc[kApplyRowTiles][CT];
 #pragma unroll
         for (int rt = 0; rt < kApplyRowTiles; ++rt)
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int kk = 0; kk < p.nK; ++kk) {
-            f16x8 ah[kApplyRowTiles], al[kApplyRowTiles];
-#pragma unroll
-            for (int rt = 0; rt < kApplyRowTiles; ++rt) {
-                const int64_t t = row0 + rt * 16 + li;
-                const int64_t j0 = static_cast<int64_t>(kk) * 32 + 8 * lb;
-                float v[8];
-                if (kVec) {
-                    if (keep[rt] && j0 < p.M) {
-                        const vf4* src = reinterpret_cast<const vf4*>(X + t * p.M + j0);
-                        const vf4 a = src[0];
-                        const vf4 b = src[1];
-                        v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
-                        v[4] = b[0]; v[5] = b[1]; v[6] = b[2]; v[7] = b[3];
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) v[e] = 0.0f;
-                    }
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e)
-                        v[e] = (keep[rt] && j0 + e < p.M) ? X[t * p.M + j0 + e] : 0.0f;
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[e] *= sx;  // exact: a power of two
-                split2_8(v, ah[rt], al[rt]);
-            }
+        const auto mfma_step = [&](int kk, const f16x8* ah, const f16x8* al) {
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
                 const int fi = (ct * p.nK + kk) * 64 + lane;
@@ -904,22 +907,50 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
                     acc[rt][ct] = a;
                 }
             }
+        };
+        // Two sample buffers in turn: the samples of step kk+2 are loaded
+        // before the MFMAs of step kk, and step kk+1 is split after them
+        // (steps past the end load the last step again and are not used).
+        // The scheduling barriers keep the loads ahead of the MFMAs and the
+        // splits after them; the scheduler would otherwise bring loads and
+        // uses together to shorten register lifetimes.
+        const int last = p.nK - 1;
+        f16x8 h[kApplyRowTiles], l[kApplyRowTiles];
+        Samples s0, s1;
+        load_step(0, s0);
+        load_step(last < 1 ? last : 1, s1);
+        split_step(s0, h, l);
+        for (int kk = 0; kk < p.nK; kk += 2) {
+            load_step(kk + 2 < last ? kk + 2 : last, s0);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_step(kk, h, l);
+            __builtin_amdgcn_sched_barrier(0);
+            split_step(s1, h, l);
+            if (kk + 1 > last) break;
+            load_step(kk + 3 < last ? kk + 3 : last, s1);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_step(kk + 1, h, l);
+            __builtin_amdgcn_sched_barrier(0);
+            split_step(s0, h, l);
         }
-        // epilogue: undo the scales, quantize(x, 1e-4) = floor(x/1e-4 + 0.5)*1e-4
-        // in double, stored as float; optional int16 (int)(y/q + 0.5).
+        // epilogue: undo the scales and quantize(y, 1e-4) =
+        // floor(y/1e-4 + 0.5)*1e-4 in double, stored as float (masked rows:
+        // zero); optional int16 (int)(y/q + 0.5).
 #pragma unroll
         for (int rt = 0; rt < kApplyRowTiles; ++rt) {
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const int64_t i = static_cast<int64_t>(cb) * (CT * 16) + ct * 16 + li;
-                if (i >= p.M) continue;
+            for (int r = 0; r < 4; ++r) {
+                const int64_t t = row0 + rt * 16 + 4 * lb + r;
+                if (t >= p.N) continue;
+                const bool kept = row_kept(p.rowkeep, t);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int64_t t = row0 + rt * 16 + 4 * lb + r;
-                    if (t >= p.N) continue;
-                    const double y = static_cast<double>(acc[rt][ct][r]) * unscale;
-                    const float yf = p.raw ? static_cast<float>(y)
-                                           : static_cast<float>(floor(div_unit(y) + 0.5) * 0.0001);
+                for (int ct = 0; ct < CT; ++ct) {
+                    const int64_t i = static_cast<int64_t>(cb) * (CT * 16) + ct * 16 + li;
+                    if (i >= p.M) continue;
+                    const double a = static_cast<double>(acc[rt][ct][r]);
+                    float yf = p.raw ? static_cast<float>(a * unscale)
+                                     : static_cast<float>(floor(a * qscale + 0.5) * 0.0001);
+                    if (!kept) yf = 0.0f;
                     if (Y) Y[t * p.M + i] = yf;
                     if (Yq) {
                         const double s = static_cast<double>(yf) / p.qunit + 0.5;

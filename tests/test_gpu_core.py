@@ -55,6 +55,28 @@ def test_detect_bit_exact(gpu, M, N, T, sign, D, t1, t2, cs):
     assert len(times) > 0
 
 
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_detect_rounding_merges(gpu, sign):
+    """Excursions to about 1e-3 on channels sitting at 1e8 (opposite to the
+    detection sign): x - mean rounds the distinct floats of an excursion to
+    the same double, so their order in float is not the order of their
+    detection values and only the FP64 test decides them."""
+    from mountainsort_amd import frontend as fe
+    say(f"detect rounding merges sign={sign}")
+    rng = np.random.default_rng(11)
+    M, N = 3, 45_000
+    Y = (-sign * 1e8 + rng.normal(size=(N, M)) * 100).astype(np.float32)
+    for j in range(M):
+        for t0 in rng.choice(np.arange(50, N - 50), 40, replace=False):
+            run = -sign * (1e-3 + 2.0 ** -30 * rng.integers(0, 4, size=7))
+            Y[t0:t0 + 7, j] = run.astype(np.float32)
+    counts, times = oc.ms3alg_detect(Y, 3.0, 10, sign, chunk_size=20000)
+    off, gt = fe.detect_events(torch.from_numpy(Y).to(gpu), 3.0, 10, sign, chunk_size=20000)
+    np.testing.assert_array_equal(off.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
+    np.testing.assert_array_equal(gt.cpu().numpy(), times)
+    assert len(times) > 0
+
+
 def test_detect_channel_subset_and_empty(gpu):
     from mountainsort_amd import frontend as fe
     say("detect channel subset / empty range")
