@@ -44,7 +44,10 @@ __global__ void __launch_bounds__(256) clips_kernel(
 // overlapping source rows of neighboring channels' clips meet in its L2.  For W <= 64 each lane owns one channel
 // (w = lane % W) and a row offset (lane / W), so the channel index is loaded
 // once and the W x T clip is covered by rows of 64 / W time samples; the
-// loads of several rows are issued before their stores.
+// loads of up to 16 rows per lane (a whole clip of T = 50 samples for
+// W <= 16) are issued before their stores.
+constexpr int kClipRows = 16;
+
 __global__ void __launch_bounds__(256) clips_batched_kernel(
     const float* __restrict__ X, int64_t M, int64_t N, const int64_t* __restrict__ times,
     const int64_t* __restrict__ eo, const int32_t* __restrict__ nc,
@@ -79,20 +82,20 @@ __global__ void __launch_bounds__(256) clips_batched_kernel(
             const int64_t t0 = times[e0 + i] - tmid;
             float* out = base + i * WT;
             if (!active) continue;
-            int t = tl;
-            for (; t + 3 * rpi < T; t += 4 * rpi) {
-                float v[4];
+            // up to kClipRows rows per lane in flight before their stores
+            for (int tb = tl; tb < T; tb += kClipRows * rpi) {
+                float v[kClipRows];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int64_t tt = t0 + t + k * rpi;
-                    v[k] = (tt >= 0 && tt < N) ? X[tt * M + chw] : 0.0f;
+                for (int k = 0; k < kClipRows; ++k) {
+                    const int t = tb + k * rpi;
+                    const int64_t tt = t0 + t;
+                    v[k] = (t < T && tt >= 0 && tt < N) ? X[tt * M + chw] : 0.0f;
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) out[(t + k * rpi) * W + w] = v[k];
-            }
-            for (; t < T; t += rpi) {
-                const int64_t tt = t0 + t;
-                out[t * W + w] = (tt >= 0 && tt < N) ? X[tt * M + chw] : 0.0f;
+                for (int k = 0; k < kClipRows; ++k) {
+                    const int t = tb + k * rpi;
+                    if (t < T) out[t * W + w] = v[k];
+                }
             }
         }
         return;
