@@ -920,19 +920,22 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
         load_step(0, s0);
         load_step(last < 1 ? last : 1, s1);
         split_step(s0, h, l);
-        for (int kk = 0; kk < p.nK; kk += 2) {
+        // nK = Mp / 32 is even (Mp is a multiple of 64), so the steps come in
+        // pairs and the loop body has no exit between a load and its use
+        const auto step_pair = [&](int kk) {
             load_step(kk + 2 < last ? kk + 2 : last, s0);
             __builtin_amdgcn_sched_barrier(0);
             mfma_step(kk, h, l);
             __builtin_amdgcn_sched_barrier(0);
             split_step(s1, h, l);
-            if (kk + 1 > last) break;
+            __builtin_amdgcn_sched_barrier(0);
             load_step(kk + 3 < last ? kk + 3 : last, s1);
             __builtin_amdgcn_sched_barrier(0);
             mfma_step(kk + 1, h, l);
             __builtin_amdgcn_sched_barrier(0);
             split_step(s0, h, l);
-        }
+        };
+        for (int kk = 0; kk < p.nK; kk += 2) step_pair(kk);
         // epilogue: undo the scales and quantize(y, 1e-4) =
         // floor(y/1e-4 + 0.5)*1e-4 in double, stored as float (masked rows:
         // zero); optional int16 (int)(y/q + 0.5).
