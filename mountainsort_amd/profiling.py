@@ -211,8 +211,42 @@ def main_all(argv):
     sys.stdout.write("\n".join(lines) + "\n")
 
 
+def main_counters(argv):
+    """python -m mountainsort_amd.profiling counters KERNEL OUT.md TITLE "command" DB [DB ...]
+    -> for each counter of each results database, its value for KERNEL summed
+    over the counter's hardware instances and averaged over the kernel's
+    dispatches, with the instance count and the mean dispatch time."""
+    import collections
+    import sqlite3
+    kernel, out, title, command = argv[:4]
+    lines = [f"# {title}", "", "Command(s):", "", "```", command, "```", "",
+             "| counter | per dispatch (sum over instances) | instances | mean per instance |",
+             "|---|---:|---:|---:|"]
+    durs = []
+    for db in argv[4:]:
+        con = sqlite3.connect(db)
+        vals = collections.defaultdict(list)
+        for name, cn, cv in con.execute("select name, counter_name, counter_value from pmc_events"):
+            if clean(name).split("<")[0].endswith(kernel):
+                vals[cn].append(float(cv))
+        mine = [d / 1000.0 for name, d in con.execute("select name, duration from kernels")
+                if clean(name).split("<")[0].endswith(kernel)]
+        durs += mine
+        nd = max(len(mine), 1)
+        for cn, v in sorted(vals.items()):
+            inst = len(v) // nd
+            lines.append(f"| {cn} | {sum(v) / nd:.4g} | {inst} | {sum(v) / max(len(v), 1):.4g} |")
+    if durs:
+        lines += ["", f"Mean dispatch time of {kernel}: {sum(durs) / len(durs):.1f} us "
+                  f"({len(durs)} dispatches)."]
+    open(out, "w").write("\n".join(lines) + "\n")
+    sys.stdout.write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "traffic":
+    if len(sys.argv) > 1 and sys.argv[1] == "counters":
+        main_counters(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "traffic":
         main_traffic(sys.argv[2:])
     elif len(sys.argv) > 1 and sys.argv[1] == "pmc-all":
         main_all(sys.argv[2:])
