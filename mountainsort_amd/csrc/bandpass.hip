@@ -229,6 +229,104 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     }
 }
 
+// Stage A for int16 input with M % 8 == 0 and kWideP pairs per unit: one
+// workgroup per (unit, column n2) over all the unit's pairs.  The N1 input
+// rows of the column are staged through LDS as whole 128-byte lines (64
+// channels of int16), eight lanes per row, where the kPB-pair form reads a
+// 32-byte piece of every row per workgroup and its loads wait on the L1's
+// pending misses.  The same LDS then holds the radix exchange.
+constexpr int kWideP = 32;
+
+template <int R1, int R2>
+constexpr size_t wide_a_lds_bytes() {
+    constexpr size_t stage = sizeof(uint32_t) * R1 * R2 * kWideP;
+    constexpr size_t exch = sizeof(cf) * R2 * lds_row(R1) * kWideP;
+    return stage > exch ? stage : exch;
+}
+
+template <int R1, int R2>
+__global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
+    const int16_t* __restrict__ X, BpPlan p, int64_t unit0, int64_t nu, const cf* __restrict__ twN1,
+    const cf* __restrict__ twHi, const cf* __restrict__ twLo, cf* __restrict__ S) {
+    constexpr int N1 = R1 * R2;
+    constexpr int RA = lds_row(R1);
+    extern __shared__ cf wbuf[];  // staging [N1][kWideP] uint32, then exchange [R2][RA][kWideP]
+    __shared__ cf tw1[N1];
+    __shared__ cf stp[R2];  // W_n^{n2*R1*r}, r < R2
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
+    const int n2 = static_cast<int>(blockIdx.x % p.N2);
+    int64_t u = blockIdx.x / p.N2;
+    if (u >= nu) return;  // whole workgroup
+    u += unit0;
+    if (kTwLds && tid < R2) {
+        const int e = n2 * R1 * tid;
+        stp[tid] = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+    }
+    const int g = static_cast<int>(u / p.nWin);
+    const int64_t w = u % p.nWin;
+    const int64_t tbase = w * p.cs - p.ov;  // time of window sample 0
+    const int64_t ch0 = static_cast<int64_t>(g) * 2 * kWideP;
+    uint32_t* stg = reinterpret_cast<uint32_t*>(wbuf);
+    // 16-byte chunk c of row n1 holds pairs 4c..4c+3; with M % 8 == 0 a chunk
+    // is wholly inside or wholly past the last channel
+    for (int q = tid; q < N1 * 8; q += blockDim.x) {
+        const int n1 = q >> 3;
+        const int c = q & 7;
+        const int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (t >= 0 && t < p.N && ch0 + 8 * c < p.M)
+            v = *reinterpret_cast<const uint4*>(X + t * p.M + ch0 + 8 * c);
+        *reinterpret_cast<uint4*>(stg + n1 * kWideP + 4 * c) = v;
+    }
+    __syncthreads();
+    const int pl = tid % kWideP;
+    const int j = tid / kWideP;
+    cf v[R1];
+    if (j < R2) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const uint32_t w2 = stg[(j + r * R2) * kWideP + pl];
+            v[r] = {static_cast<float>(static_cast<int16_t>(w2 & 0xffffu)),
+                    static_cast<float>(static_cast<int16_t>(w2 >> 16))};
+        }
+    }
+    __syncthreads();  // the staging is read before the exchange overwrites it
+    if (j < R2) {
+        fft::Dft<R1, -1>::run(v);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) wbuf[(j * RA + r) * kWideP + pl] = v[r];
+    }
+    __syncthreads();
+    if (j < R1) {
+        cf y[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const cf x = wbuf[(r * RA + j) * kWideP + pl];
+            y[r] = (r == 0 || j == 0) ? x : cmul(x, tw1[r * j]);
+        }
+        fft::Dft<R2, -1>::run(y);
+        cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+        cf wj = {1.f, 0.f};
+        if (kTwLds) {
+            const int e = n2 * j;
+            wj = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+        }
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const int k1 = j + r * R1;
+            cf w4;
+            if (kTwLds) {
+                w4 = cmul(wj, stp[r]);
+            } else {
+                const int e = n2 * k1;
+                w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
+            }
+            Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pl] = cmul(y[r], w4);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ stage B
 // Workgroup: one k1, kPB pairs of one unit.  Forward N2 = Q1*Q2 transform
 // (pass radix Q1 then Q2), gain, inverse (radix Q2 on the same registers,
@@ -629,6 +727,13 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         MSAMD_HIP(hipMemcpyAsync(b.twLo, tl.data(), sizeof(cf) * kTwLo, hipMemcpyHostToDevice, st));
         MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
     }
+    // whole-line staging of the int16 input (stage A) when a unit holds 32
+    // pairs and rows are 16-byte aligned
+    const bool wide_a = p.in_i16 && p.M % 8 == 0 && p.CG == kWideP && sh.R1 == 20 && sh.R2 == 20;
+    if (wide_a)
+        MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&bp_stage_a_wide<20, 20>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(wide_a_lds_bytes<20, 20>())));
     for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
         const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
         auto grid = [&](int pb_per_wg, int64_t cols_per_unit) {
@@ -640,7 +745,11 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         const unsigned gridC = grid(kPBC, p.N2);
         const unsigned gridB = grid(kPBB, p.N1);
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
-            if (p.in_i16)
+            if (wide_a) {
+                bp_stage_a_wide<20, 20><<<static_cast<unsigned>(nu * p.N2), kWideP * 20,
+                                          wide_a_lds_bytes<20, 20>(), st>>>(
+                    static_cast<const int16_t*>(X), p, u0, nu, b.twN1, b.twHi, b.twLo, b.S);
+            } else if (p.in_i16)
                 bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, nu, b.twN1,
                                                                              b.twHi, b.twLo, b.S);
             else

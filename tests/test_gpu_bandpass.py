@@ -39,6 +39,7 @@ def _rel_err(Yg, Yo):
     (64, 400_000, np.float32, 8),      # natural plan: chunk 120000
     (7, 250_000, np.int16, 32),        # odd channel count, int16 input, partial last window
     (384, 130_000, np.int16, 1),       # wide probe, two windows
+    (72, 150_000, np.int16, 8),        # whole-line int16 staging; the second unit holds 4 of 32 pairs
 ])
 def test_bandpass_fp32_within_1e4_of_fp64(gpu, M, N, dtype, T):
     from mountainsort_amd import frontend as fe
