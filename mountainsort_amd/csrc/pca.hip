@@ -362,12 +362,16 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
 
 // ------------------------------------------------------------ projections
 // Stage 1 item: up to pe1 events of one neighborhood.  The components
-// (W x T x nf1) sit in LDS with rows padded to 16 floats; thread (event,
-// channel) reads its clip samples straight from global memory (consecutive
-// threads -> consecutive channels -> contiguous addresses) and forms nf1
-// dot products over T.
-constexpr int kCPad = 16;
+// (W x T x nf1) sit in LDS with rows padded to 16 floats; thread (event
+// pair, channel) reads its two clips' samples straight from global memory
+// (consecutive threads -> consecutive channels -> contiguous addresses) and
+// forms nf1 dot products over T for each, so every component read from LDS
+// serves two events.
+// Component rows are padded to kCPad floats (a multiple of 4): 12 when
+// nf1 <= 12 (the default 10 features), else 16.
+inline int cpad_for(int nf1) { return nf1 <= 12 ? 12 : 16; }
 
+template <int kCPad>
 __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restrict__ clips,
                                                            PcaPlan p, PcaTables tb,
                                                            float* __restrict__ reduced) {
@@ -388,42 +392,62 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
     }
     __syncthreads();
     const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T;
-    for (int o = threadIdx.x; o < ne * W; o += blockDim.x) {
-        const int ev = o / W;
-        const int wc = o - ev * W;
-        float acc[kCPad];
+    // thread (event pair, channel): the two events share every component read
+    const int np = (ne + 1) / 2;
+    for (int o = threadIdx.x; o < np * W; o += blockDim.x) {
+        const int ev = 2 * (o / W);
+        const int wc = o % W;
+        const bool two = ev + 1 < ne;
+        float acc0[kCPad], acc1[kCPad];
 #pragma unroll
-        for (int a = 0; a < kCPad; ++a) acc[a] = 0.0f;
-        const float* x = xg + ev * W * T + wc;
+        for (int a = 0; a < kCPad; ++a) acc0[a] = acc1[a] = 0.0f;
+        const float* x0 = xg + ev * W * T + wc;
+        const float* x1 = two ? x0 + W * T : x0;
         const float* c = Cs + wc * T * kCPad;
         constexpr int U = 10;
         int t = 0;
         for (; t + U <= T; t += U) {
-            float xv[U];
+            float xa[U], xb[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = x[(t + u) * W];
+            for (int u = 0; u < U; ++u) {
+                xa[u] = x0[(t + u) * W];
+                xb[u] = x1[(t + u) * W];
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const float4* c4 = reinterpret_cast<const float4*>(c + (t + u) * kCPad);
 #pragma unroll
                 for (int q = 0; q < kCPad / 4; ++q) {
                     const float4 cc = c4[q];
-                    acc[4 * q + 0] = __builtin_fmaf(cc.x, xv[u], acc[4 * q + 0]);
-                    acc[4 * q + 1] = __builtin_fmaf(cc.y, xv[u], acc[4 * q + 1]);
-                    acc[4 * q + 2] = __builtin_fmaf(cc.z, xv[u], acc[4 * q + 2]);
-                    acc[4 * q + 3] = __builtin_fmaf(cc.w, xv[u], acc[4 * q + 3]);
+                    acc0[4 * q + 0] = __builtin_fmaf(cc.x, xa[u], acc0[4 * q + 0]);
+                    acc0[4 * q + 1] = __builtin_fmaf(cc.y, xa[u], acc0[4 * q + 1]);
+                    acc0[4 * q + 2] = __builtin_fmaf(cc.z, xa[u], acc0[4 * q + 2]);
+                    acc0[4 * q + 3] = __builtin_fmaf(cc.w, xa[u], acc0[4 * q + 3]);
+                    acc1[4 * q + 0] = __builtin_fmaf(cc.x, xb[u], acc1[4 * q + 0]);
+                    acc1[4 * q + 1] = __builtin_fmaf(cc.y, xb[u], acc1[4 * q + 1]);
+                    acc1[4 * q + 2] = __builtin_fmaf(cc.z, xb[u], acc1[4 * q + 2]);
+                    acc1[4 * q + 3] = __builtin_fmaf(cc.w, xb[u], acc1[4 * q + 3]);
                 }
             }
         }
         for (; t < T; ++t) {
-            const float xv = x[t * W];
+            const float xa = x0[t * W];
+            const float xb = x1[t * W];
 #pragma unroll
-            for (int a = 0; a < kCPad; ++a) acc[a] = __builtin_fmaf(c[t * kCPad + a], xv, acc[a]);
+            for (int a = 0; a < kCPad; ++a) {
+                acc0[a] = __builtin_fmaf(c[t * kCPad + a], xa, acc0[a]);
+                acc1[a] = __builtin_fmaf(c[t * kCPad + a], xb, acc1[a]);
+            }
         }
         float* out = reduced + tb.red_off[g] + static_cast<int64_t>(e0 + ev) * W * nf + wc;
 #pragma unroll
         for (int a = 0; a < kCPad; ++a)
-            if (a < nf) out[W * a] = acc[a];
+            if (a < nf) out[W * a] = acc0[a];
+        if (two) {
+#pragma unroll
+            for (int a = 0; a < kCPad; ++a)
+                if (a < nf) out[W * nf + W * a] = acc1[a];
+        }
     }
 }
 
@@ -690,7 +714,7 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     const int n2 = nf1 * P->h.maxW;
     P->lds_gram1 = sizeof(__bf16) * 2 * P->h.maxW * (T | 1) * kRowStride;
     P->lds_power1 = sizeof(double) * (T * T + 2 * T + 16);
-    P->lds_proj1 = sizeof(float) * P->h.maxW * T * kCPad;
+    P->lds_proj1 = sizeof(float) * P->h.maxW * T * cpad_for(nf1);
     P->lds_gram2 = sizeof(__bf16) * 2 * (n2 | 1) * kRowStride;
     P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * n2 : 0) + 2 * n2 + 16);
     P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, tl.pe2));
@@ -740,7 +764,10 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
                                    static_cast<int>(bytes));
     };
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<1>), P->lds_gram1));
-    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_project1_kernel), P->lds_proj1));
+    MSAMD_HIP(lds_attr(cpad_for(P->p.nf1) == 12
+                           ? reinterpret_cast<const void*>(&pca_project1_kernel<12>)
+                           : reinterpret_cast<const void*>(&pca_project1_kernel<16>),
+                       P->lds_proj1));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<2>), P->lds_gram2));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_power_kernel<2>), P->lds_power2));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_project2_kernel), P->lds_proj2));
@@ -770,8 +797,11 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, P->lds_power1, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     if (!h.j1.empty()) {  // stage 1 projections
-        pca_project1_kernel<<<static_cast<unsigned>(h.j1.size() / 2), 256, P->lds_proj1, st>>>(
-            clips, p, l.tb, reduced);
+        const unsigned g1 = static_cast<unsigned>(h.j1.size() / 2);
+        if (cpad_for(nf1) == 12)
+            pca_project1_kernel<12><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
+        else
+            pca_project1_kernel<16><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
         MSAMD_LAUNCH_CHECK();
     }
     // stage 2 Gram
