@@ -43,7 +43,7 @@ struct PcaPlan {
     int32_t G, T, nf1, nf2;
     int64_t max_s;
     int32_t maxW;
-    int64_t idx_stride;  // max_s + 2
+    int64_t idx_stride;  // idx_stride_for(max_s)
     // tiles (chosen per workload, see msamd_pca_autotune)
     int32_t gram_chunk;  // subsampled columns per Gram work item (multiple of kKC)
     int32_t pe1;         // events per stage-1 projection item
@@ -90,13 +90,32 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
         return;
     }
     const double increment = L * 1.0 / p.max_s;
-    int32_t* out = tb.idx + g * p.idx_stride;
-    int32_t c = 0;
-    for (double i = 0; i < L; i += increment) {
-        if (c < p.idx_stride) out[c] = static_cast<int32_t>(static_cast<int64_t>(i));
-        ++c;
+    int32_t* out = tb.idx + g * p.idx_stride;  // 16-byte aligned: idx_stride % 4 == 0
+    int64_t c = 0;
+    double i = 0;
+    // four indices per store: one lane per neighborhood writes its own list,
+    // so single-index stores would touch 64 cache lines per instruction
+    for (;;) {
+        int32_t v[4];
+        int n = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (n == q && i < L) {
+                v[q] = static_cast<int32_t>(static_cast<int64_t>(i));
+                i += increment;
+                n = q + 1;
+            }
+        }
+        if (n == 4 && c + 4 <= p.idx_stride) {
+            *reinterpret_cast<int4*>(out + c) = make_int4(v[0], v[1], v[2], v[3]);
+        } else {
+            for (int q = 0; q < n; ++q)
+                if (c + q < p.idx_stride) out[c + q] = v[q];
+        }
+        c += n;
+        if (n < 4) break;
     }
-    tb.ns[g] = c < p.idx_stride ? c : static_cast<int32_t>(p.idx_stride);
+    tb.ns[g] = static_cast<int32_t>(c < p.idx_stride ? c : p.idx_stride);
 }
 
 // ------------------------------------------------------------------ Gram
@@ -591,6 +610,10 @@ struct PcaLayout {
     float* part = nullptr;
 };
 
+// Room for the subsample list of one neighborhood (at most max_s + 1
+// indices), rounded up to a multiple of 4 for 16-byte stores.
+inline int64_t idx_stride_for(int64_t max_s) { return align_up(max_s + 2, 4); }
+
 PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride) {
     PcaLayout l;
     Carver c(ws);
@@ -709,8 +732,9 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     if (rc != MSAMD_OK) return rc;
     P->h = host_plan(G, L, W, T, nf1, nf2, max_s, tl);
     P->tiles = tl;
-    P->p = PcaPlan{G, T, nf1, nf2, max_s, P->h.maxW, max_s + 2, tl.gram_chunk, tl.pe1, tl.pe2};
-    P->l = pca_layout(nullptr, P->h, G, max_s + 2);
+    P->p = PcaPlan{G, T, nf1, nf2, max_s, P->h.maxW, idx_stride_for(max_s), tl.gram_chunk, tl.pe1,
+                   tl.pe2};
+    P->l = pca_layout(nullptr, P->h, G, idx_stride_for(max_s));
     const int n2 = nf1 * P->h.maxW;
     P->lds_gram1 = sizeof(__bf16) * 2 * P->h.maxW * (T | 1) * kRowStride;
     P->lds_power1 = sizeof(double) * (T * T + 2 * T + 16);
@@ -826,8 +850,8 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
 }
 
 void plan_free_graph(msamd_pca_plan* P) {
-    if (P->exec) hipGraphExecDestroy(P->exec);
-    if (P->graph) hipGraphDestroy(P->graph);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);  // teardown: nothing to report
+    if (P->graph) (void)hipGraphDestroy(P->graph);
     P->exec = nullptr;
     P->graph = nullptr;
 }
@@ -912,7 +936,7 @@ extern "C" int msamd_pca_plan_launch_graph(msamd_pca_plan* plan, const float* cl
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(P->capture, &g);
         if (rc != MSAMD_OK) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         MSAMD_HIP(ec);
@@ -936,7 +960,7 @@ extern "C" void msamd_pca_plan_tiles(const msamd_pca_plan* plan, int32_t* tiles)
 extern "C" void msamd_pca_plan_destroy(msamd_pca_plan* plan) {
     if (!plan) return;
     plan_free_graph(plan);
-    if (plan->capture) hipStreamDestroy(plan->capture);
+    if (plan->capture) (void)hipStreamDestroy(plan->capture);
     delete plan;
 }
 
@@ -1027,8 +1051,8 @@ extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* 
         best = dim_best;
         best_ms = dim_ms;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc != MSAMD_OK) return rc;
     tiles_out[0] = best.gram_chunk;
     tiles_out[1] = best.pe1;
