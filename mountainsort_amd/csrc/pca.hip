@@ -287,6 +287,8 @@ __global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTabl
 // One workgroup per Gram matrix (n x n, n <= 128 held in LDS as FP64).
 // Components are written as float [row][component].
 constexpr int kPowMaxN = 128;
+// Row stride of a Gram matrix held in LDS: n rounded up to odd.
+__host__ __device__ inline int pow_ld(int n) { return n | 1; }
 
 __device__ double pow_block_sum(double x, double* red) {
     const int lane = threadIdx.x & 63;
@@ -305,7 +307,9 @@ __device__ double pow_block_sum(double x, double* red) {
     return s;
 }
 
-__device__ void power_method(double* Gs, int n, int K, const double* sin_tab, float* comp,
+// Gs is n x n with row stride ld (odd in LDS: thread r reads row r, and an
+// odd stride puts 32 consecutive rows' doubles in distinct bank pairs).
+__device__ void power_method(double* Gs, int n, int ld, int K, const double* sin_tab, float* comp,
                              double* v, double* w, double* red) {
     for (int k = 0; k < K; ++k) {
         // v = normalize(sin(1..n))
@@ -319,7 +323,7 @@ __device__ void power_method(double* Gs, int n, int K, const double* sin_tab, fl
         __syncthreads();
         for (int it = 0; it < 11; ++it) {
             for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                const double* row = Gs + r * n;
+                const double* row = Gs + r * ld;
                 double a = 0.0;
                 for (int c = 0; c < n; ++c) a += row[c] * v[c];
                 w[r] = a;
@@ -340,7 +344,8 @@ __device__ void power_method(double* Gs, int n, int K, const double* sin_tab, fl
                 for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
                     const int i = q / n;
                     const int j = q - i * n;
-                    Gs[q] = (Gs[q] - (v[i] * w[j] + w[i] * v[j])) + alpha * (v[i] * v[j]);
+                    double& gq = Gs[i * ld + j];
+                    gq = (gq - (v[i] * w[j] + w[i] * v[j])) + alpha * (v[i] * v[j]);
                 }
                 __syncthreads();
             }
@@ -371,12 +376,16 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
     // the FP64 copy of G lives in LDS when it fits, otherwise in the workspace
     const bool in_lds = kStage == 1 || p.nf1 * p.maxW <= kPowMaxN;  // decided per launch
     double* Gs = in_lds ? psh : tb.work2 + tb.w2_off[g];
-    double* v = in_lds ? Gs + n * n : psh;
+    const int ld = in_lds ? pow_ld(n) : n;
+    double* v = in_lds ? Gs + n * ld : psh;
     double* w = v + n;
     double* red = w + n;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Gs[e] = static_cast<double>(Gsrc[e]);
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n;
+        Gs[i * ld + (e - i * n)] = static_cast<double>(Gsrc[e]);
+    }
     __syncthreads();
-    power_method(Gs, n, K, tb.sin_tab, comp, v, w, red);
+    power_method(Gs, n, ld, K, tb.sin_tab, comp, v, w, red);
 }
 
 // ------------------------------------------------------------ projections
@@ -737,10 +746,10 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     P->l = pca_layout(nullptr, P->h, G, idx_stride_for(max_s));
     const int n2 = nf1 * P->h.maxW;
     P->lds_gram1 = sizeof(__bf16) * 2 * P->h.maxW * (T | 1) * kRowStride;
-    P->lds_power1 = sizeof(double) * (T * T + 2 * T + 16);
+    P->lds_power1 = sizeof(double) * (T * pow_ld(T) + 2 * T + 16);
     P->lds_proj1 = sizeof(float) * P->h.maxW * T * cpad_for(nf1);
     P->lds_gram2 = sizeof(__bf16) * 2 * (n2 | 1) * kRowStride;
-    P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * n2 : 0) + 2 * n2 + 16);
+    P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * pow_ld(n2) : 0) + 2 * n2 + 16);
     P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, tl.pe2));
     return MSAMD_OK;
 }
