@@ -196,16 +196,11 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
             colbase[threadIdx.x] = c < cend ? static_cast<int64_t>(sub ? idx[c] : c) * S : -1;
         }
         __syncthreads();
-        // item (kg, o): element o of columns 8kg..8kg+7
-        for (int it = threadIdx.x; it < 4 * S; it += blockDim.x) {
+        // item (kg, o): element o of columns 8kg..8kg+7; two items per pass so
+        // that sixteen loads are in flight before the first is used
+        const auto stage_item = [&](int it, const float (&v)[8]) {
             const int kg = it / S;
             const int o = it - kg * S;
-            float v[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t cb = colbase[8 * kg + e];
-                v[e] = cb >= 0 ? base[cb + o] : 0.0f;
-            }
             bf16x8 h8, l8;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -224,6 +219,23 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
             const int li = ((q * ra + r) * kRowStride + 8 * kg) / 8;
             reinterpret_cast<bf16x8*>(hi)[li] = h8;
             reinterpret_cast<bf16x8*>(lo)[li] = l8;
+        };
+        const auto load_item = [&](int it, float (&v)[8]) {
+            const int kg = it / S;
+            const int o = it - kg * S;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t cb = colbase[8 * kg + e];
+                v[e] = cb >= 0 ? base[cb + o] : 0.0f;
+            }
+        };
+        for (int it = threadIdx.x; it < 4 * S; it += 2 * blockDim.x) {
+            const int it2 = it + blockDim.x;
+            float va[8], vb[8];
+            load_item(it, va);
+            if (it2 < 4 * S) load_item(it2, vb);
+            stage_item(it, va);
+            if (it2 < 4 * S) stage_item(it2, vb);
         }
         __syncthreads();
 #pragma unroll
