@@ -36,8 +36,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kKC = 32;        // subsampled columns per MFMA K step
 constexpr int kMaxW = 16;      // channels per neighborhood supported
 constexpr int kMaxT = 64;      // clip length supported
-constexpr int kMaxTiles = 20;  // Gram tiles per wavefront (8 waves per workgroup)
-constexpr int kGramThreads = 512;
+constexpr int kMaxTiles = 10;  // Gram tiles per wavefront (16 waves per workgroup)
+constexpr int kGramThreads = 1024;
+// every Gram tile has a wave slot: stage 1 up to 10 tiles (T <= 64) per
+// channel, stage 2 up to 136 tiles (nf1 * W <= 256)
+static_assert((kGramThreads / 64) * kMaxTiles >= 10 * kMaxW && (kGramThreads / 64) * kMaxTiles >= 136,
+              "Gram tiles exceed the wave slots");
 
 struct PcaPlan {
     int32_t G, T, nf1, nf2;
@@ -196,8 +200,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
             colbase[threadIdx.x] = c < cend ? static_cast<int64_t>(sub ? idx[c] : c) * S : -1;
         }
         __syncthreads();
-        // item (kg, o): element o of columns 8kg..8kg+7; two items per pass so
-        // that sixteen loads are in flight before the first is used
+        // item (kg, o): element o of columns 8kg..8kg+7
         const auto stage_item = [&](int it, const float (&v)[8]) {
             const int kg = it / S;
             const int o = it - kg * S;
@@ -229,13 +232,10 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
                 v[e] = cb >= 0 ? base[cb + o] : 0.0f;
             }
         };
-        for (int it = threadIdx.x; it < 4 * S; it += 2 * blockDim.x) {
-            const int it2 = it + blockDim.x;
-            float va[8], vb[8];
+        for (int it = threadIdx.x; it < 4 * S; it += blockDim.x) {
+            float va[8];
             load_item(it, va);
-            if (it2 < 4 * S) load_item(it2, vb);
             stage_item(it, va);
-            if (it2 < 4 * S) stage_item(it2, vb);
         }
         __syncthreads();
 #pragma unroll
