@@ -229,13 +229,17 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     }
 }
 
-// Stage A for int16 input with M % 8 == 0 and kWideP pairs per unit: one
-// workgroup per (unit, column n2) over all the unit's pairs.  The N1 input
-// rows of the column are staged through LDS as whole 128-byte lines (64
-// channels of int16), eight lanes per row, where the kPB-pair form reads a
-// 32-byte piece of every row per workgroup and its loads wait on the L1's
-// pending misses.  The same LDS then holds the radix exchange.
-constexpr int kWideP = 32;
+// Stage A for int16 input with M % 8 == 0: one workgroup per (unit, pair
+// block of kWideP pairs, column n2).  The N1 input rows of the column are
+// staged through LDS in 16-byte pieces (4 pairs), kWideP/4 lanes per row
+// (64 or 128 contiguous bytes), where the kPB-pair form reads a 32-byte piece
+// of every row per workgroup and its loads wait on the L1's pending misses.
+// The same LDS then holds the radix exchange.
+#ifndef MSAMD_BP_WIDEP
+#define MSAMD_BP_WIDEP 32
+#endif
+constexpr int kWideP = MSAMD_BP_WIDEP;
+static_assert(kWideP % 4 == 0 && kCGMax % kWideP == 0, "wide stage A: pairs per workgroup");
 
 template <int R1, int R2>
 constexpr size_t wide_a_lds_bytes() {
@@ -255,9 +259,11 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
     __shared__ cf stp[R2];  // W_n^{n2*R1*r}, r < R2
     const int tid = threadIdx.x;
     for (int q = tid; q < N1; q += blockDim.x) tw1[q] = twN1[q];
-    const int n2 = static_cast<int>(blockIdx.x % p.N2);
-    int64_t u = blockIdx.x / p.N2;
-    if (u >= nu) return;  // whole workgroup
+    const int npb = p.CG / kWideP;
+    int n2, pb;
+    int64_t u;
+    block_unit_pb_col(blockIdx.x, npb, p.N2, &u, &pb, &n2);
+    if (u >= nu) return;  // whole workgroup: past the last column of the batch
     u += unit0;
     if (kTwLds && tid < R2) {
         const int e = n2 * R1 * tid;
@@ -266,13 +272,14 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
     const int g = static_cast<int>(u / p.nWin);
     const int64_t w = u % p.nWin;
     const int64_t tbase = w * p.cs - p.ov;  // time of window sample 0
-    const int64_t ch0 = static_cast<int64_t>(g) * 2 * kWideP;
+    const int64_t ch0 = 2 * (static_cast<int64_t>(g) * p.CG + pb * kWideP);
     uint32_t* stg = reinterpret_cast<uint32_t*>(wbuf);
     // 16-byte chunk c of row n1 holds pairs 4c..4c+3; with M % 8 == 0 a chunk
     // is wholly inside or wholly past the last channel
-    for (int q = tid; q < N1 * 8; q += blockDim.x) {
-        const int n1 = q >> 3;
-        const int c = q & 7;
+    constexpr int kCh = kWideP / 4;  // chunks per row
+    for (int q = tid; q < N1 * kCh; q += blockDim.x) {
+        const int n1 = q / kCh;
+        const int c = q % kCh;
         const int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
         uint4 v = {0u, 0u, 0u, 0u};
         if (t >= 0 && t < p.N && ch0 + 8 * c < p.M)
@@ -322,7 +329,7 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
                 const int e = n2 * k1;
                 w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
             }
-            Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pl] = cmul(y[r], w4);
+            Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kWideP + pl] = cmul(y[r], w4);
         }
     }
 }
@@ -729,7 +736,7 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     }
     // whole-line staging of the int16 input (stage A) when a unit holds 32
     // pairs and rows are 16-byte aligned
-    const bool wide_a = p.in_i16 && p.M % 8 == 0 && p.CG == kWideP && sh.R1 == 20 && sh.R2 == 20;
+    const bool wide_a = p.in_i16 && p.M % 8 == 0 && p.CG % kWideP == 0 && sh.R1 == 20 && sh.R2 == 20;
     if (wide_a)
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&bp_stage_a_wide<20, 20>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -746,7 +753,7 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         const unsigned gridB = grid(kPBB, p.N1);
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
             if (wide_a) {
-                bp_stage_a_wide<20, 20><<<static_cast<unsigned>(nu * p.N2), kWideP * 20,
+                bp_stage_a_wide<20, 20><<<grid(kWideP, p.N2), kWideP * 20,
                                           wide_a_lds_bytes<20, 20>(), st>>>(
                     static_cast<const int16_t*>(X), p, u0, nu, b.twN1, b.twHi, b.twLo, b.S);
             } else if (p.in_i16)
