@@ -243,6 +243,7 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
         float cur[kB], nxt[kB];
 #pragma unroll
         for (int i = 0; i < kB; ++i) cur[i] = load_x(xp, T0 + i, p.M, p.N_file);
+        float prev = load_x(xp, T0 - 1, p.M, p.N_file);  // the sample before the batch
         for (int64_t tb = T0; tb < T1; tb += kB) {
 #pragma unroll
             for (int i = 0; i < kB; ++i) nxt[i] = load_x(xp, tb + kB + i, p.M, p.N_file);
@@ -250,6 +251,28 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
 #pragma unroll
             for (int i = 0; i < kB; ++i)
                 if (tb + i < T1 && detect_value_s<kSign>(cur[i], mean) > thr) over |= 1u << i;
+            // A crossing that its immediate neighbour already beats (by the
+            // float rules of stencil_float) is rejected from the registers:
+            // only the local maxima of a run above threshold reach the
+            // neighbour loads.  lo / hi: the first / last batch index whose
+            // left / right neighbour lies inside the padded window.
+            if (fast && p.D >= 1 && over) {
+                const int64_t lo64 = wa - tb + 1, hi64 = wb - tb - 2;
+                const int lo = static_cast<int>(lo64 < 0 ? 0 : (lo64 > kB ? kB : lo64));
+                const int hi = static_cast<int>(hi64 < -1 ? -1 : (hi64 > kB ? kB : hi64));
+                uint32_t rej = 0;
+#pragma unroll
+                for (int i = 0; i < kB; ++i) {
+                    const float xm = i > 0 ? cur[i - 1] : prev;
+                    const float xq = i < kB - 1 ? cur[i + 1] : nxt[0];
+                    const float kc = kSign < 0 ? -cur[i] : cur[i];
+                    const float km = kSign < 0 ? -xm : xm;
+                    const float kq = kSign < 0 ? -xq : xq;
+                    const float tau = 0x1p-40f * (fabsf(cur[i]) + mabs);
+                    if ((i >= lo && km >= kc) || (i <= hi && kq - kc > tau)) rej |= 1u << i;
+                }
+                over &= ~rej;
+            }
             // in time order; kept events are > D apart, so cnt < cap
             while (over) {
                 const int i = __builtin_ctz(over);
@@ -262,6 +285,7 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
                                       p.M, p.N_file);
                 if (keep && cnt < p.cap) my_slots[cnt++] = static_cast<uint16_t>(t - T0);
             }
+            prev = cur[kB - 1];
 #pragma unroll
             for (int i = 0; i < kB; ++i) cur[i] = nxt[i];
         }
