@@ -137,7 +137,6 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
     const void* __restrict__ Xv, BpPlan p, int64_t unit0, int64_t nu, const cf* __restrict__ twN1,
     const cf* __restrict__ twHi, const cf* __restrict__ twLo, cf* __restrict__ S) {
     constexpr int N1 = R1 * R2;
-    constexpr int TPS = R1 > R2 ? R1 : R2;
     constexpr int RA = lds_row(R1);
     __shared__ cf buf[R2 * RA * kPB];
     __shared__ cf tw1[N1];

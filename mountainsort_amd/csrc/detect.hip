@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(1024) detect_seg_scan_kernel(
         const int64_t carry = carry_s;
         if (i < nseg) oj[i] = carry + warp_sums[wid] + x - v;
         __syncthreads();
-        if (tid == blockDim.x - 1) carry_s = carry + warp_sums[wid] + x;
+        if (tid == static_cast<int>(blockDim.x) - 1) carry_s = carry + warp_sums[wid] + x;
         __syncthreads();
     }
     if (tid == 0) chan_total[j] = carry_s;
