@@ -467,8 +467,9 @@ def main():
                          "three times each")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is "
-                                   "described in DESIGN.md and not run"}))
+        print(json.dumps({"error": "the pipeline runs on one GPU only; sharding channel "
+                                   "blocks and time spans over several GPUs is not "
+                                   "implemented, so --gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np
