@@ -49,16 +49,18 @@ __device__ __forceinline__ f32x4 mfma3(const bf16x8& ah, const bf16x8& al, const
 constexpr int kCovTPW = 16;  // output tiles per wavefront
 constexpr int kCovWaves = 4;
 
-// Streaming form for nB <= 24 (M <= 384): G workgroups of 8 waves share a
+// Streaming form for nB <= 24 (M <= 384): G workgroups of 12 waves share a
 // time span and hold the output tiles of the upper triangle between them
-// (up to 20 per wave); each streams the span once, staging the fragments of
-// all channel blocks per 32 samples in a double-buffered LDS area while the
-// next 32 samples load.  The G workgroups of one span run on the same XCD,
-// so the span is read from HBM about once.
-constexpr int kCovAllWaves = 8;
-constexpr int kCovAllTPW = 20;
+// (up to 13 per wave, 168 registers: three waves per SIMD, which hide the
+// sample loads and the per-step barrier better than two waves of 20 tiles);
+// each streams the span once, staging the fragments of all channel blocks
+// per 32 samples in a double-buffered LDS area while the next 32 samples
+// load.  The G workgroups of one span run on the same XCD, so the span is
+// read from HBM about once.
+constexpr int kCovAllWaves = 12;
+constexpr int kCovAllTPW = 13;
 constexpr int kCovAllKSteps = 1200;  // 32-sample steps per workgroup (FP32 partial length)
-constexpr int kCovAllItems = 3;      // (channel, 8-sample group) items per thread (4*Mp <= 1536)
+constexpr int kCovAllItems = 2;      // (channel, 8-sample group) items per thread (4*Mp <= 1536)
 
 struct CovPlan {
     int64_t M, N, span, nsplit;
