@@ -28,6 +28,11 @@ def test_oracle_reproduces_fixtures(golden, inp):
     for k, v in golden.items():
         if np.issubdtype(v.dtype, np.integer):
             assert np.array_equal(out[k], v), k
+        elif k == "bandpass_val":
+            # an FP64 DFT pair: its last bits depend on the FFT library's
+            # factorization order, not on the filter
+            rms = np.sqrt(np.mean(v ** 2))
+            assert np.max(np.abs(out[k] - v)) < 1e-12 * rms, k
         else:
             np.testing.assert_allclose(out[k], v, rtol=1e-12, atol=0, err_msg=k)
 
@@ -37,7 +42,7 @@ def test_gpu_matches_fixtures(gpu, golden, inp):
     import torch
 
     from mountainsort_amd import frontend as fe
-    from tests.conftest import say
+    from tests.conftest import assert_pca_parity, say
 
     say("golden: bandpass")
     bp = fe.bandpass_filter(torch.from_numpy(inp["raw"]).to(gpu), 30000, 300, 6000,
@@ -79,5 +84,4 @@ def test_gpu_matches_fixtures(gpu, golden, inp):
     red = red.cpu().numpy()[:ro[1]].reshape(L, 10, Wd)
     feat = feat.cpu().numpy()[:fo[1]].reshape(L, 10)
     r0, f0 = golden["pca_reduced"], golden["pca_features"]
-    assert np.max(np.abs(red[:, :3] - r0[:, :3])) / np.max(np.abs(r0[:, :3])) < 1e-3
-    assert np.max(np.abs(feat[:, :3] - f0[:, :3])) / np.max(np.abs(f0[:, :3])) < 2e-3
+    assert_pca_parity(red, r0, feat, f0, "golden")

@@ -121,7 +121,12 @@ def test_clips_bit_exact(gpu):
 
 
 # --------------------------------------------------------------- whitening
-@pytest.mark.parametrize("M,N", [(4, 100_000), (16, 50_000), (64, 40_000), (384, 20_000), (5, 1000)])
+# M covers every kernel instance: the streaming kernel with 5, 10 and 13
+# tiles per wave (one or two tile groups) up to M = 384, and the generic
+# kernel beyond (385..1024)
+@pytest.mark.parametrize("M,N", [(4, 100_000), (16, 50_000), (64, 40_000), (384, 20_000), (5, 1000),
+                                 (200, 12_000), (256, 10_000), (300, 9000), (512, 6000),
+                                 (1000, 4000)])
 def test_covariance_split_bf16_error(gpu, M, N):
     from mountainsort_amd import frontend as fe
     say(f"covariance M={M} N={N}")
@@ -274,7 +279,7 @@ def test_whitening_matrix_exact_zero_channels_and_singular(gpu):
 
 
 @pytest.mark.parametrize("M,N,isz", [(64, 2000 * 37 + 555, 2000), (384, 500 * 41, 500),
-                                     (7, 999, 2000)])
+                                     (7, 999, 2000), (256, 500 * 23 + 17, 500)])
 def test_masked_whitening_without_the_masked_copy(gpu, M, N, isz):
     """mask_rows + the row-bitmap covariance/apply equal mask_out_artifacts
     followed by the plain covariance/apply, bit for bit."""
