@@ -303,6 +303,19 @@ int msamd_whiten_apply_cov(const float* X, int64_t M, int64_t N, const double* W
                            double quantization_unit, int16_t* Yq, void* workspace,
                            int64_t workspace_bytes, void* stream);
 
+/* msamd_whiten_apply_cov without a workspace and without any allocation: the
+ * W fragments and scale words live in a device array owned by the library
+ * (about 1 MiB per device, held until the process ends), so M <= 512
+ * (MSAMD_ERR_CAPACITY beyond).  Results equal msamd_whiten_apply_cov's bit
+ * for bit (the same kernels).  Calls on one device share that array: calls
+ * that overlap in time on one device race; calls ordered on one stream are
+ * safe.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). */
+int msamd_whiten_apply_cov_noalloc(const float* X, int64_t M, int64_t N, const double* W,
+                                   const double* XXt, const uint32_t* row_keep, float* Y,
+                                   double quantization_unit, int16_t* Yq, void* stream);
+
 /* ms3.whiten_clips (packages/ms3/p_whiten.cpp:295-345): every time sample of
  * every clip times W, without the 1e-4 quantize of the timeseries form.
  * clips, clips_out: M x T x L float32 (the clip layout); with

@@ -55,6 +55,7 @@ SIGNATURES = {
     "msamd_whiten_apply_rows": (cint, [vp, i64, i64, vp, vp, vp, f64, vp, vp, i64, vp]),
     "msamd_whiten_clips": (cint, [vp, i64, i64, i64, vp, vp, f64, vp, vp, i64, vp]),
     "msamd_whiten_apply_cov": (cint, [vp, i64, i64, vp, vp, vp, vp, f64, vp, vp, i64, vp]),
+    "msamd_whiten_apply_cov_noalloc": (cint, [vp, i64, i64, vp, vp, vp, vp, f64, vp, vp]),
     "msamd_mask_workspace_size": (i64, [i64, i64, i64]),
     "msamd_mask_out_artifacts": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
     "msamd_mask_rows": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),

@@ -11,6 +11,7 @@
 // term is dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The
 // covariance sums FP32 block partials in FP64; the whitening matrix is FP64.
 #include <cstdint>
+#include <mutex>
 
 #include "common.h"
 
@@ -1245,4 +1246,78 @@ extern "C" int msamd_whiten_apply_cov(const float* X, int64_t M, int64_t N, cons
     MSAMD_REQUIRE(XXt, "whiten_apply_cov: null covariance");
     return apply_impl(X, M, N, W, XXt, row_keep, Y, quantization_unit, Yq, workspace,
                       workspace_bytes, stream);
+}
+
+// ------------------------------------------------- apply without allocation
+// msamd_whiten_apply_cov_noalloc keeps its scratch (the two-piece W
+// fragments and the scale words) in this library-owned device array instead
+// of a caller workspace.  Sized for M <= kNoAllocMaxM.  In host code the
+// symbol is only a stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is ever passed
+// to a kernel.  Calls on one device share the array, so calls that overlap
+// in time on one device (different streams, or threads) race; calls that are
+// ordered on one stream do not.  Nothing in it has to start at zero: every
+// call's scale kernel writes each word the later kernels read.
+namespace {
+
+constexpr int64_t kNoAllocMaxM = 512;
+constexpr int64_t kNoAllocBytes = (int64_t{1} << 20) + 4096;  // fragments of M = 512 + scale words
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+int scratch_address(void** out) {
+    int dev = -1;
+    MSAMD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("whiten_apply_cov_noalloc: device %d outside 0..%d", dev,
+                  kNoAllocMaxDevices - 1);
+        return MSAMD_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MSAMD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_whiten_apply_cov_noalloc(const float* X, int64_t M, int64_t N,
+                                              const double* W, const double* XXt,
+                                              const uint32_t* row_keep, float* Y,
+                                              double quantization_unit, int16_t* Yq,
+                                              void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(XXt, "whiten_apply_cov_noalloc: null covariance");
+    if (M < 1 || M > kNoAllocMaxM) {
+        set_error("whiten_apply_cov_noalloc: M=%lld outside 1..%lld (the library's scratch "
+                  "holds the W fragments of at most %lld channels; use msamd_whiten_apply_cov "
+                  "with a workspace)",
+                  (long long)M, (long long)kNoAllocMaxM, (long long)kNoAllocMaxM);
+        return MSAMD_ERR_CAPACITY;
+    }
+    const int64_t need = msamd_whiten_apply_workspace_size(M);
+    if (need < 0 || need > kNoAllocBytes) {
+        set_error("whiten_apply_cov_noalloc: scratch of %lld bytes needed, %lld held",
+                  (long long)need, (long long)kNoAllocBytes);
+        return MSAMD_ERR_CAPACITY;
+    }
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != MSAMD_OK) return rc;
+    return apply_impl(X, M, N, W, XXt, row_keep, Y, quantization_unit, Yq, ws, kNoAllocBytes,
+                      stream);
 }
