@@ -156,16 +156,18 @@ int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events_host,
                                  int64_t max_pca_samples);
 
 /* Features of G neighborhoods in one batched pass.  Neighborhood g has
- * num_events_host[g] clips of widths_host[g] channels (1..16) and T samples
- * (<= 64), packed one after another in clips (W x T x L each, the clip
- * layout above).  Stage 1: per channel, PCA of the T x L slice with
- * num_features_per_channel components; reduced receives W x nf1 x L per
- * neighborhood, packed.  Stage 2: PCA of reduced viewed as (nf1*W) x L with
- * num_features components; features receives nf2 x L per neighborhood,
- * packed (the isosplit5 input).  Both PCAs use the pca_subsampled column
- * rule with max_pca_samples, the 10-step power method with deflation from
- * normalize(sin(1..n)) (FP64), Gram matrices from split-BF16 3-term MFMA
- * products accumulated in FP32, and FP32 projections over all columns. */
+ * num_events_host[g] clips of widths_host[g] channels (1..128, with
+ * num_features_per_channel * width <= 2048) and T samples (<= 64), packed
+ * one after another in clips (W x T x L each, the clip layout above).
+ * Stage 1: per channel, PCA of the T x L slice with num_features_per_channel
+ * components; reduced receives W x nf1 x L per neighborhood, packed.
+ * Stage 2: PCA of reduced viewed as (nf1*W) x L with num_features
+ * components; features receives nf2 x L per neighborhood, packed (the
+ * isosplit5 input).  Both PCAs use the pca_subsampled column rule with
+ * max_pca_samples, the 10-step power method with deflation from
+ * normalize(sin(1..n)) (FP64), Gram matrices from exact three-piece BF16
+ * MFMA products (FP32 within 32-column steps, FP64 across them: within
+ * ~1e-8 of the FP64 Gram), and FP32 projections over all columns. */
 int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_host,
                        const int32_t* widths_host, int32_t T, int32_t num_features_per_channel,
                        int32_t num_features, int64_t max_pca_samples, float* reduced,

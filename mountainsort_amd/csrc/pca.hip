@@ -10,7 +10,8 @@
 // stage 2, per g: R = reduced clips viewed as (nf1*W) x L.  Each PCA:
 //   columns: all L, or the pca_subsampled indices (int64)i for
 //            i = 0, inc, 2inc, ... < L, inc = L/max (double accumulation)
-//   Gram:    G = A_s A_s^T  -- split-BF16 3-term MFMA, FP32 accumulation
+//   Gram:    G = A_s A_s^T  -- three-piece BF16 MFMA (6 product terms, exact
+//            products), FP32 within a 32-column step, FP64 across steps
 //   power:   per component k < K: v = normalize(sin(1..n)); 10 times
 //            v <- G v / |G v| (sigma = |G v|); deflate G <- (I-vv')G(I-vv')
 //            (FP64; the reference iterates X X^T v on the data, equal in
@@ -32,21 +33,39 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kKC = 32;        // subsampled columns per MFMA K step
-constexpr int kMaxW = 16;      // channels per neighborhood supported
+constexpr int kKC = 32;        // subsampled columns per staging step
+constexpr int kMaxW = 128;     // channels per neighborhood supported
 constexpr int kMaxT = 64;      // clip length supported
-constexpr int kMaxTiles = 10;  // Gram tiles per wavefront (16 waves per workgroup)
-constexpr int kGramThreads = 1024;
-// every Gram tile has a wave slot: stage 1 up to 10 tiles (T <= 64) per
-// channel, stage 2 up to 136 tiles (nf1 * W <= 256)
-static_assert((kGramThreads / 64) * kMaxTiles >= 10 * kMaxW && (kGramThreads / 64) * kMaxTiles >= 136,
+constexpr int kGroupW = 12;    // stage-1 channels per Gram work item (at most)
+constexpr int kProjW = 16;     // stage-1 channels per projection work item (at most)
+constexpr int kMaxTiles = 10;  // Gram tiles per wavefront (12 waves per workgroup)
+constexpr int kGramThreads = 768;
+constexpr int kMaxR = 2048;    // stage-2 dimension nf1 * W supported
+constexpr int kSinTab = kMaxR; // sin(i + 1), i < kSinTab
+// every staged element has a thread (stage 1 kGroupW * T <= 768 for
+// T <= 64; stage 2 at most 256 rows), and every Gram tile of a work item
+// has a wave slot: stage 1 up to 10 tiles (T <= 64) for each channel of a
+// group; stage 2 up to 120 tiles (one band of up to 240 rows) or 64 (a pair
+// of 128-row bands)
+static_assert((kGramThreads / 64) * kMaxTiles >= 10 * kGroupW &&
+                  (kGramThreads / 64) * kMaxTiles >= 120,
               "Gram tiles exceed the wave slots");
+static_assert(kGroupW * 64 <= kGramThreads && 256 <= kGramThreads,
+              "staged elements exceed the threads");
+
+// Stage-2 Gram rows are split into bands: one band when nf1 * W <= 240,
+// else bands of 128 rows; a work item covers one band pair (A <= B).
+__host__ __device__ inline int band_rows(int R) { return R <= 240 ? 240 : 128; }
+__host__ __device__ inline int num_bands(int R) { return R <= 240 ? 1 : (R + 127) / 128; }
 
 struct PcaPlan {
     int32_t G, T, nf1, nf2;
     int64_t max_s;
     int32_t maxW;
+    int32_t gw;          // stage-1 channels per Gram work item (<= kGroupW, sized to the LDS)
+    int32_t pw;          // stage-1 channels per projection work item (<= kProjW)
     int64_t idx_stride;  // idx_stride_for(max_s)
     // tiles (chosen per workload, see msamd_pca_autotune)
     int32_t gram_chunk;  // subsampled columns per Gram work item (multiple of kKC)
@@ -65,21 +84,22 @@ struct PcaTables {
     const int64_t* c1_off;     // [G] floats: W*T*nf1
     const int64_t* c2_off;     // [G] floats: nf1*W*nf2
     const int32_t* p1_items;   // [n1][2] (g, w)
-    const int32_t* j1_items;   // [nj1][2] (g, first event)
+    const int32_t* j1_items;   // [nj1][3] (g, first event, first channel of the group)
     const int32_t* j2_items;   // [nj2][2] (g, first event)
     int32_t* ns;               // [G] subsampled column count
     int32_t* idx;              // [G][idx_stride]
-    float* g1;                 // Gram stage 1
-    float* g2;                 // Gram stage 2
+    double* g1;                // Gram stage 1
+    double* g2;                // Gram stage 2
     float* c1;                 // components stage 1, [w][t][a]
     float* c2;                 // components stage 2, [r][a]
-    const double* sin_tab;     // sin(i+1), i < 1024
+    const double* sin_tab;     // sin(i+1), i < kSinTab
     double* work2;             // FP64 stage-2 Gram copies when n > kPowMaxN
     const int64_t* w2_off;     // [G] doubles
-    const int32_t* gram_items; // [ngi][2] (g, chunk)
-    const int32_t* gram_first; // [G+1] first item of each neighborhood
-    const int64_t* gp1_off;    // [ngi] floats: stage-1 partial of each item
-    const int64_t* gp2_off;    // [ngi] floats: stage-2 partial of each item
+    const int32_t* gram1_items;  // [n][3] (g, chunk, channel group)
+    const int32_t* gram2_items;  // [n][3] (g, chunk, band pair A << 16 | B)
+    const int32_t* gram_first;   // [G+1] first chunk of each neighborhood
+    const int64_t* gp1_off;      // [nchunks] floats: stage-1 partial of each chunk
+    const int64_t* gp2_off;      // [nchunks] floats: stage-2 partial of each chunk
 };
 
 // ----------------------------------------------------- subsample indices
@@ -123,49 +143,91 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 }
 
 // ------------------------------------------------------------------ Gram
-// Work item = (neighborhood, chunk of up to gram_chunk subsampled columns);
-// the chunk partial Grams are summed in FP64 by pca_gram_reduce_kernel in
-// chunk order.  Each K step stages 32 columns (S floats each: the W x T clip
-// for stage 1, the nf1*W reduced vector for stage 2) in LDS as bf16 hi/lo
-// pieces laid out [sub-matrix][row][k]: a thread loads element o of 8
-// consecutive columns (coalesced across threads) and writes them as one
-// 16-byte store, and an MFMA operand fragment (8 consecutive k of one row)
-// is one 16-byte load.  Rows have a 40-element stride and sub-matrices an odd
-// row count so the stores spread over the LDS banks.  Stage 1 has W
-// sub-matrices of T rows (row t of channel w is element t*W + w of a
-// column); stage 2 one sub-matrix of nf1*W rows.  Rows past the end read as
-// zero.  Waves own Gram tiles (q, I, J >= I).
+// Work item = (neighborhood, chunk of up to gram_chunk subsampled columns,
+// sub-block): a group of up to gw channels for stage 1, a band pair for
+// stage 2.  The items of one chunk write disjoint parts of the chunk's FP64
+// partial Gram, and pca_gram_reduce_kernel sums a neighborhood's chunk
+// partials in chunk order.
+//
+// Precision.  Every float is split exactly into three bf16 pieces,
+// x = h + m + l (h = bf16(x), m = bf16(x - h), l = x - h - m, each
+// difference exact in FP32), and the six products of weight >= 2^-16 are
+// accumulated: hh + hm + mh + hl + lh + mm (the dropped ml, lm, ll are
+// below 2^-23 of |x y| and of random sign).  bf16 products are exact in
+// FP32; the FP32 accumulator is cleared at every 32-column step and its
+// step sum added to FP64 accumulators, so rounding grows with one step's
+// sum, not with the whole chunk's.  The Gram comes out within ~1e-8 of
+// FP64.  (Two-piece split-BF16 or split-FP16 with FP32 accumulation over a
+// chunk leaves 1e-6 .. 1e-5, which the power method's later components
+// amplify past 1e-3 on some neighborhoods.)
+//
+// Each K step stages 32 columns -- the item's rows of each: T rows per
+// channel of the group for stage 1, the rows of bands A and B of the nf1*W
+// reduced vector for stage 2 -- in LDS as the three bf16 pieces laid out
+// [sub-matrix][row][k]: thread o loads element o of the 32 columns
+// (coalesced across threads; the column bases are uniform) and writes each
+// piece of 8 columns as one 16-byte store, and an MFMA operand fragment
+// (8 consecutive k of one row) is one 16-byte load.
+// Rows have a 40-element stride and each sub-matrix an odd row count, so
+// stores spread over the banks.  Operands of the last row block read up to
+// 15 rows past a sub-matrix's end (the next sub-matrix, or 16 rows of slack
+// after the last plane): such rows feed only output rows and columns that
+// are not written, so the loads need no lane mask.  Waves own Gram tiles
+// (q, I, J).
 constexpr int kRowStride = 40;
+constexpr int kPieces = 3;
 
-__device__ __forceinline__ int gram_rows_alloc(int nrows) { return nrows | 1; }
+// rows allocated per sub-matrix: odd
+__host__ __device__ inline int gram_rows_alloc(int nrows) { return nrows | 1; }
 
 template <int kStage>
 __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
-                                                       PcaTables tb, float* __restrict__ part) {
+                                                       PcaTables tb, double* __restrict__ part) {
     extern __shared__ __bf16 gl[];
-    __shared__ int64_t colbase[kKC];
-    const int g = tb.gram_items[2 * blockIdx.x];
-    const int chunk = tb.gram_items[2 * blockIdx.x + 1];
+    const int32_t* item = (kStage == 1 ? tb.gram1_items : tb.gram2_items) + 3 * blockIdx.x;
+    const int g = item[0];
+    const int chunk = item[1];
+    const int sub = item[2];
     const int L = tb.L[g];
     const int W = tb.W[g];
     const int ns = tb.ns[g];
-    const int nrows = kStage == 1 ? p.T : p.nf1 * W;
-    const int nsub = kStage == 1 ? W : 1;
-    const int S = kStage == 1 ? W * p.T : p.nf1 * W;
-    const int nb = (nrows + 15) / 16;
+    // rows of this item: stage 1 channels q0 .. q0 + nsub - 1 (T rows each);
+    // stage 2 band A (nA rows) then, when B != A, band B (nB rows)
+    int nsub, nrows, S, q0 = 0, A = 0, B = 0, bs = 0, nA = 0, nB = 0;
+    if (kStage == 1) {
+        q0 = sub * p.gw;
+        nsub = W - q0 < p.gw ? W - q0 : p.gw;
+        nrows = p.T;
+        S = nsub * p.T;
+    } else {
+        const int R = p.nf1 * W;
+        A = sub >> 16;
+        B = sub & 0xffff;
+        bs = band_rows(R);
+        nA = R - A * bs < bs ? R - A * bs : bs;
+        nB = A == B ? 0 : (R - B * bs < bs ? R - B * bs : bs);
+        nsub = 1;
+        nrows = nA + nB;
+        S = nrows;
+    }
+    const int Sfull = kStage == 1 ? W * p.T : p.nf1 * W;  // floats per source column
     const int ra = gram_rows_alloc(nrows);
     const float* base = src + (kStage == 1 ? tb.clip_off[g] : tb.red_off[g]);
     const int32_t* idx = tb.idx + g * p.idx_stride;
-    const bool sub = L > p.max_s;
-    const int tpq = nb * (nb + 1) / 2;  // tiles per sub-matrix
+    const bool sub_cols = L > p.max_s;
+    // tiles: stage 1 and stage 2 with A == B: J >= I over nb row blocks;
+    // stage 2 with A < B: every (I, J), I over band A, J over band B
+    const bool pair = kStage == 2 && A != B;
+    const int nbI = pair ? nA / 16 : (nrows + 15) / 16;
+    const int nbJ = pair ? (nB + 15) / 16 : nbI;
+    const int tpq = pair ? nbI * nbJ : nbI * (nbI + 1) / 2;
     const int ntiles = tpq * nsub;
     const int plane = nsub * ra * kRowStride;  // bf16 elements per piece
-    __bf16* hi = gl;
-    __bf16* lo = gl + plane;
-    const int wid = threadIdx.x >> 6;
+    // the wave index is uniform: scalar registers for the tile table
+    const int wid = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
-    // packed (q << 16) | (I << 8) | J
+    // packed (q << 16) | (I << 8) | J, J local to its band
     int tile[kMaxTiles];
 #pragma unroll
     for (int k = 0; k < kMaxTiles; ++k) {
@@ -174,101 +236,152 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
         if (t < ntiles) {
             q = t / tpq;
             int tt = t - q * tpq;
-            int rl = nb;
-            while (tt >= rl) {
-                tt -= rl;
-                ++I;
-                --rl;
+            if (pair) {
+                I = tt / nbJ;
+                J = tt - I * nbJ;
+            } else {
+                int rl = nbI;
+                while (tt >= rl) {
+                    tt -= rl;
+                    ++I;
+                    --rl;
+                }
+                J = I + tt;
             }
-            J = I + tt;
         }
         tile[k] = (q << 16) | (I << 8) | J;
     }
-    f32x4 acc[kMaxTiles];
+    f64x4 acc[kMaxTiles];
 #pragma unroll
-    for (int k = 0; k < kMaxTiles; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16x8* hv = reinterpret_cast<const bf16x8*>(hi);
-    const bf16x8* lv = reinterpret_cast<const bf16x8*>(lo);
-    const bf16x8 zero8 = {};
+    for (int k = 0; k < kMaxTiles; ++k) acc[k] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const int bOff = pair ? nA : 0;  // local row of band B's first row
+    const bf16x8* pv = reinterpret_cast<const bf16x8*>(gl);
+    const int lfrag = (lane & 15) * (kRowStride / 8) + (lane >> 4);
     const int cbeg = chunk * p.gram_chunk;
     int cend = cbeg + p.gram_chunk;
     if (cend > ns) cend = ns;
-    for (int c0 = cbeg; c0 < cend; c0 += kKC) {
-        __syncthreads();  // previous step's fragments consumed
+    // staging slots: element o of a run of ng groups of 8 columns, ng = 1, 2
+    // or 4 so that the 4 * S / ng slots fit the threads (S <= 768: stage 1
+    // gw * T, stage 2 at most 256 rows); a slot's source offset and LDS row
+    // are fixed for the whole item
+    const int ng = 4 * S <= static_cast<int>(blockDim.x) ? 1 : (2 * S <= static_cast<int>(blockDim.x) ? 2 : 4);
+    const int o = threadIdx.x % S;
+    const int kg0 = (threadIdx.x / S) * ng;
+    const bool stager = kg0 < 4;
+    int soff = 0, srow = 0;
+    if (kStage == 1) {
+        const int q = o % nsub;
+        const int r = o / nsub;
+        soff = r * W + q0 + q;
+        srow = q * ra + r;
+    } else {
+        soff = o < nA ? A * bs + o : B * bs + (o - nA);
+        srow = o;
+    }
+    bf16x8* stg = reinterpret_cast<bf16x8*>(gl) + srow * (kRowStride / 8);
+    // the next step's samples are loaded into registers before the current
+    // step's MFMAs, so the load latency overlaps them
+    float nxt[kKC];
+    // Column indices of the step being fetched sit in colbuf: threads 0..31
+    // load the indices two steps ahead into a register (issued with the
+    // prefetch, after the split, so no wait before a barrier covers it) and
+    // publish them a step later.  Indices are clamped into
+    // the chunk (the sample is then replaced by zero), so loads need no branch.
+    __shared__ int colbuf[kKC];
+    const auto colidx = [&](int c) {
+        c = c < cend ? c : cend - 1;
+        return sub_cols ? idx[c] : c;
+    };
+    const auto fetch = [&](int c0) {
+        const int cb = c0 + 8 * kg0;
+#pragma unroll
+        for (int j = 0; j < kKC; ++j) {
+            if (j >= 8 * ng) break;
+            const float x = base[static_cast<int64_t>(colbuf[8 * kg0 + j]) * Sfull + soff];
+            nxt[j] = cb + j < cend ? x : 0.0f;
+        }
+    };
+    int cnext = 0;
+    if (cbeg < cend) {
         if (threadIdx.x < kKC) {
-            const int c = c0 + threadIdx.x;
-            colbase[threadIdx.x] = c < cend ? static_cast<int64_t>(sub ? idx[c] : c) * S : -1;
+            colbuf[threadIdx.x] = colidx(cbeg + threadIdx.x);
+            cnext = colidx(cbeg + kKC + threadIdx.x);
         }
         __syncthreads();
-        // item (kg, o): element o of columns 8kg..8kg+7
-        const auto stage_item = [&](int it, const float (&v)[8]) {
-            const int kg = it / S;
-            const int o = it - kg * S;
-            bf16x8 h8, l8;
+        if (stager) fetch(cbeg);
+    }
+    for (int c0 = cbeg; c0 < cend; c0 += kKC) {
+        __syncthreads();  // previous step's fragments and index reads consumed
+        if (threadIdx.x < kKC) colbuf[threadIdx.x] = cnext;  // indices of step c0 + kKC
+        if (stager) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const __bf16 h = static_cast<__bf16>(v[e]);
-                h8[e] = h;
-                l8[e] = static_cast<__bf16>(v[e] - static_cast<float>(h));
-            }
-            int q, r;
-            if (kStage == 1) {
-                q = o % W;
-                r = o / W;
-            } else {
-                q = 0;
-                r = o;
-            }
-            const int li = ((q * ra + r) * kRowStride + 8 * kg) / 8;
-            reinterpret_cast<bf16x8*>(hi)[li] = h8;
-            reinterpret_cast<bf16x8*>(lo)[li] = l8;
-        };
-        const auto load_item = [&](int it, float (&v)[8]) {
-            const int kg = it / S;
-            const int o = it - kg * S;
+            for (int g8 = 0; g8 < 4; ++g8) {
+                if (g8 >= ng) break;
+                bf16x8 h8, m8, l8;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t cb = colbase[8 * kg + e];
-                v[e] = cb >= 0 ? base[cb + o] : 0.0f;
+                for (int e = 0; e < 8; ++e) {
+                    const float x = nxt[8 * g8 + e];
+                    const __bf16 h = static_cast<__bf16>(x);
+                    const float r1 = x - static_cast<float>(h);  // exact
+                    const __bf16 m = static_cast<__bf16>(r1);
+                    h8[e] = h;
+                    m8[e] = m;
+                    l8[e] = static_cast<__bf16>(r1 - static_cast<float>(m));  // exact
+                }
+                stg[kg0 + g8] = h8;
+                stg[plane / 8 + kg0 + g8] = m8;
+                stg[2 * (plane / 8) + kg0 + g8] = l8;
             }
-        };
-        for (int it = threadIdx.x; it < 4 * S; it += blockDim.x) {
-            float va[8];
-            load_item(it, va);
-            stage_item(it, va);
         }
         __syncthreads();
+        if (threadIdx.x < kKC) cnext = colidx(c0 + 2 * kKC + threadIdx.x);
+        if (stager && c0 + kKC < cend) fetch(c0 + kKC);
+        // keep the per-tile fragment addresses inside the loop: hoisted out of
+        // it they would exceed the registers and spill
+        int lf = lfrag;
+        asm volatile("" : "+v"(lf));
 #pragma unroll
         for (int k = 0; k < kMaxTiles; ++k) {
             if (wid + nw * k < ntiles) {
-                // fragment of row block X: lane (i + 16b) reads row 16X + i, k = 8b..8b+7
+                // fragment of row block X: lane (i + 16b) reads row 16X + i, k = 8b..8b+7;
+                // a per-lane part plus a scalar per-tile part (row stride 40 = 5 fragments)
                 const int q = tile[k] >> 16, I = (tile[k] >> 8) & 255, J = tile[k] & 255;
-                const int rA = I * 16 + (lane & 15), rB = J * 16 + (lane & 15);
-                const int fa = ((q * ra + rA) * kRowStride + 8 * (lane >> 4)) / 8;
-                const int fb = ((q * ra + rB) * kRowStride + 8 * (lane >> 4)) / 8;
-                const bool va = rA < nrows, vb = rB < nrows;
-                const bf16x8 ah = va ? hv[fa] : zero8, al = va ? lv[fa] : zero8;
-                const bf16x8 bh = vb ? hv[fb] : zero8, bl = vb ? lv[fb] : zero8;
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[k], 0, 0, 0);
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[k], 0, 0, 0);
+                const int fa = lf + (q * ra + I * 16) * (kRowStride / 8);
+                const int fb = lf + (q * ra + bOff + J * 16) * (kRowStride / 8);
+                const bf16x8 ah = pv[fa], am = pv[plane / 8 + fa], al = pv[2 * (plane / 8) + fa];
+                const bf16x8 bh = pv[fb], bm = pv[plane / 8 + fb], bl = pv[2 * (plane / 8) + fb];
+                f32x4 s = {0.f, 0.f, 0.f, 0.f};
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, s, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[k][r] += static_cast<double>(s[r]);
             }
         }
     }
-    // partial Gram of this chunk: [sub-matrix][row][row], upper triangle mirrored
-    float* out = part + (kStage == 1 ? tb.gp1_off : tb.gp2_off)[blockIdx.x];
+    // this item's part of the chunk's partial Gram ([channel][row][row] for
+    // stage 1, [row][row] for stage 2), each tile's transpose mirrored.
+    // Accumulator element r of lane l holds row 4 (l / 16) + r, column l % 16.
+    double* out = part + (kStage == 1 ? tb.gp1_off : tb.gp2_off)[tb.gram_first[g] + chunk];
+    const int ldo = kStage == 1 ? p.T : p.nf1 * W;
 #pragma unroll
     for (int k = 0; k < kMaxTiles; ++k) {
         if (wid + nw * k < ntiles) {
             const int q = tile[k] >> 16, I = (tile[k] >> 8) & 255, J = tile[k] & 255;
-            float* oq = out + static_cast<int64_t>(q) * nrows * nrows;
+            double* oq = out + static_cast<int64_t>(kStage == 1 ? q0 + q : 0) * ldo * ldo;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int ii = I * 16 + 4 * (lane >> 4) + r;
-                const int jj = J * 16 + (lane & 15);
-                if (ii < nrows && jj < nrows && !(I == J && ii > jj)) {
-                    oq[ii * nrows + jj] = acc[k][r];
-                    oq[jj * nrows + ii] = acc[k][r];
+                const int li = I * 16 + 4 * (lane >> 4) + r;  // local rows
+                const int lj = J * 16 + (lane & 15);
+                const bool ok = pair ? (lj < nB) : (li < nrows && lj < nrows && !(I == J && li > lj));
+                if (ok) {
+                    const int ii = kStage == 1 ? li : A * bs + li;
+                    const int jj = kStage == 1 ? lj : B * bs + lj;
+                    oq[static_cast<int64_t>(ii) * ldo + jj] = acc[k][r];
+                    oq[static_cast<int64_t>(jj) * ldo + ii] = acc[k][r];
                 }
             }
         }
@@ -278,7 +391,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
 // Sum of a neighborhood's chunk partials (FP64, chunk order) into its Gram.
 template <int kStage>
 __global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTables tb,
-                                                              const float* __restrict__ part) {
+                                                              const double* __restrict__ part) {
     const int g = blockIdx.y;
     const int W = tb.W[g];
     const int nrows = kStage == 1 ? p.T : p.nf1 * W;
@@ -290,9 +403,9 @@ __global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTabl
     const int nch = tb.gram_first[g + 1] - i0;
     const int64_t* offs = kStage == 1 ? tb.gp1_off : tb.gp2_off;
     double s = 0.0;
-    for (int c = 0; c < nch; ++c) s = s + static_cast<double>(part[offs[i0 + c] + e]);
-    float* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
-    out[e] = static_cast<float>(s);
+    for (int c = 0; c < nch; ++c) s = s + part[offs[i0 + c] + e];
+    double* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
+    out[e] = s;
 }
 
 // ----------------------------------------------------------- power method
@@ -381,8 +494,8 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
     const int W = tb.W[g];
     const int n = kStage == 1 ? p.T : p.nf1 * W;
     const int K = kStage == 1 ? p.nf1 : p.nf2;
-    const float* Gsrc = kStage == 1 ? tb.g1 + tb.g1_off[g] + static_cast<int64_t>(q) * n * n
-                                    : tb.g2 + tb.g2_off[g];
+    const double* Gsrc = kStage == 1 ? tb.g1 + tb.g1_off[g] + static_cast<int64_t>(q) * n * n
+                                     : tb.g2 + tb.g2_off[g];
     float* comp = kStage == 1 ? tb.c1 + tb.c1_off[g] + static_cast<int64_t>(q) * n * K
                               : tb.c2 + tb.c2_off[g];
     // the FP64 copy of G lives in LDS when it fits, otherwise in the workspace
@@ -394,19 +507,19 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
     double* red = w + n;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e / n;
-        Gs[i * ld + (e - i * n)] = static_cast<double>(Gsrc[e]);
+        Gs[i * ld + (e - i * n)] = Gsrc[e];
     }
     __syncthreads();
     power_method(Gs, n, ld, K, tb.sin_tab, comp, v, w, red);
 }
 
 // ------------------------------------------------------------ projections
-// Stage 1 item: up to pe1 events of one neighborhood.  The components
-// (W x T x nf1) sit in LDS with rows padded to 16 floats; thread (event
-// pair, channel) reads its two clips' samples straight from global memory
-// (consecutive threads -> consecutive channels -> contiguous addresses) and
-// forms nf1 dot products over T for each, so every component read from LDS
-// serves two events.
+// Stage 1 item: up to pe1 events of one neighborhood and a group of up to pw
+// of its channels.  The group's components (gw x T x nf1) sit in LDS with
+// rows padded to kCPad floats; thread (event pair, channel) reads its two
+// clips' samples straight from global memory (consecutive threads ->
+// consecutive channels -> contiguous addresses) and forms nf1 dot products
+// over T for each, so every component read from LDS serves two events.
 // Component rows are padded to kCPad floats (a multiple of 4): 12 when
 // nf1 <= 12 (the default 10 features), else 16.
 inline int cpad_for(int nf1) { return nf1 <= 12 ? 12 : 16; }
@@ -416,32 +529,34 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
                                                            PcaPlan p, PcaTables tb,
                                                            float* __restrict__ reduced) {
     extern __shared__ float sp[];
-    const int g = tb.j1_items[2 * blockIdx.x];
-    const int e0 = tb.j1_items[2 * blockIdx.x + 1];
+    const int g = tb.j1_items[3 * blockIdx.x];
+    const int e0 = tb.j1_items[3 * blockIdx.x + 1];
+    const int q0 = tb.j1_items[3 * blockIdx.x + 2];
     const int L = tb.L[g];
     const int W = tb.W[g];
+    const int Wg = W - q0 < p.pw ? W - q0 : p.pw;
     const int T = p.T;
     const int nf = p.nf1;
     const int ne = (L - e0) < p.pe1 ? (L - e0) : p.pe1;
-    float* Cs = sp;  // [W][T][kCPad]
-    const float* cg = tb.c1 + tb.c1_off[g];
-    for (int e = threadIdx.x; e < W * T * kCPad; e += blockDim.x) {
+    float* Cs = sp;  // [Wg][T][kCPad]
+    const float* cg = tb.c1 + tb.c1_off[g] + static_cast<int64_t>(q0) * T * nf;
+    for (int e = threadIdx.x; e < Wg * T * kCPad; e += blockDim.x) {
         const int a = e % kCPad;
         const int wt = e / kCPad;
         Cs[e] = a < nf ? cg[wt * nf + a] : 0.0f;
     }
     __syncthreads();
-    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T;
+    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T + q0;
     // thread (event pair, channel): the two events share every component read
     const int np = (ne + 1) / 2;
-    for (int o = threadIdx.x; o < np * W; o += blockDim.x) {
-        const int ev = 2 * (o / W);
-        const int wc = o % W;
+    for (int o = threadIdx.x; o < np * Wg; o += blockDim.x) {
+        const int ev = 2 * (o / Wg);
+        const int wc = o % Wg;
         const bool two = ev + 1 < ne;
         float acc0[kCPad], acc1[kCPad];
 #pragma unroll
         for (int a = 0; a < kCPad; ++a) acc0[a] = acc1[a] = 0.0f;
-        const float* x0 = xg + ev * W * T + wc;
+        const float* x0 = xg + static_cast<int64_t>(ev) * W * T + wc;
         const float* x1 = two ? x0 + W * T : x0;
         const float* c = Cs + wc * T * kCPad;
         constexpr int U = 10;
@@ -479,7 +594,7 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
                 acc1[a] = __builtin_fmaf(c[t * kCPad + a], xb, acc1[a]);
             }
         }
-        float* out = reduced + tb.red_off[g] + static_cast<int64_t>(e0 + ev) * W * nf + wc;
+        float* out = reduced + tb.red_off[g] + static_cast<int64_t>(e0 + ev) * W * nf + q0 + wc;
 #pragma unroll
         for (int a = 0; a < kCPad; ++a)
             if (a < nf) out[W * a] = acc0[a];
@@ -525,7 +640,7 @@ struct HostPlan {
     std::vector<int64_t> clip_off, red_off, feat_off, g1_off, g2_off, c1_off, c2_off;
     std::vector<int32_t> L, W, p1, j1, j2;
     std::vector<int64_t> w2_off, gp1_off, gp2_off;
-    std::vector<int32_t> gram_items, gram_first;
+    std::vector<int32_t> gram1_items, gram2_items, gram_first;
     int64_t g1_total = 0, g2_total = 0, c1_total = 0, c2_total = 0, w2_total = 0;
     int64_t gp1_total = 0, gp2_total = 0;
     int32_t maxW = 0;
@@ -535,13 +650,27 @@ struct Tiles {
     int32_t gram_chunk = 2560, pe1 = 64, pe2 = 64;
 };
 
+constexpr int64_t kMaxLds = 160 * 1024;
+
+// Stage-1 channels per work item: at most kGroupW, fewer when that many
+// channels' staged T-row sub-matrices would not fit the LDS.
+inline int32_t group_width(int32_t T, int32_t maxW) {
+    const int64_t per = static_cast<int64_t>(sizeof(__bf16)) * kPieces * gram_rows_alloc(T) * kRowStride;
+    int64_t gw = (kMaxLds - 2048) / per;
+    if (gw > kGroupW) gw = kGroupW;
+    if (gw > maxW) gw = maxW;
+    return static_cast<int32_t>(gw < 1 ? 1 : gw);
+}
+
 HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
-                   int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl) {
+                   int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl, int32_t gw,
+                   int32_t pw) {
     HostPlan h;
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = widths[g] > maxW ? widths[g] : maxW;
     const bool w2_global = static_cast<int64_t>(nf1) * maxW > kPowMaxN;
     int64_t co = 0, ro = 0, fo = 0;
+    int32_t nchunks = 0;
     for (int32_t g = 0; g < G; ++g) {
         const int64_t L = num_events[g];
         const int32_t W = widths[g];
@@ -561,17 +690,29 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
         h.g2_total += R * R;
         h.w2_off.push_back(h.w2_total);
         if (w2_global) h.w2_total += R * R;
-        // Gram work items: the subsampled column count is min(L, max_s + 1)
+        // Gram chunks: the subsampled column count is min(L, max_s + 1)
         const int64_t ns_bound = L < max_s ? L : max_s + 1;
         const int64_t nch = ns_bound > 0 ? ceil_div(ns_bound, tl.gram_chunk) : 1;
-        h.gram_first.push_back(static_cast<int32_t>(h.gram_items.size() / 2));
+        const int32_t ngroups = static_cast<int32_t>(ceil_div(W, gw));
+        const int32_t nb2 = num_bands(static_cast<int>(R));
+        h.gram_first.push_back(nchunks);
         for (int64_t c = 0; c < nch; ++c) {
-            h.gram_items.push_back(g);
-            h.gram_items.push_back(static_cast<int32_t>(c));
+            for (int32_t q = 0; q < ngroups; ++q) {
+                h.gram1_items.push_back(g);
+                h.gram1_items.push_back(static_cast<int32_t>(c));
+                h.gram1_items.push_back(q);
+            }
+            for (int32_t a = 0; a < nb2; ++a)
+                for (int32_t b = a; b < nb2; ++b) {
+                    h.gram2_items.push_back(g);
+                    h.gram2_items.push_back(static_cast<int32_t>(c));
+                    h.gram2_items.push_back((a << 16) | b);
+                }
             h.gp1_off.push_back(h.gp1_total);
             h.gp1_total += static_cast<int64_t>(W) * T * T;
             h.gp2_off.push_back(h.gp2_total);
             h.gp2_total += R * R;
+            ++nchunks;
         }
         h.c1_off.push_back(h.c1_total);
         h.c1_total += static_cast<int64_t>(W) * T * nf1;
@@ -581,16 +722,18 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
             h.p1.push_back(g);
             h.p1.push_back(w);
         }
-        for (int64_t e = 0; e < L; e += tl.pe1) {
-            h.j1.push_back(g);
-            h.j1.push_back(static_cast<int32_t>(e));
-        }
+        for (int64_t e = 0; e < L; e += tl.pe1)
+            for (int32_t q0 = 0; q0 < W; q0 += pw) {
+                h.j1.push_back(g);
+                h.j1.push_back(static_cast<int32_t>(e));
+                h.j1.push_back(q0);
+            }
         for (int64_t e = 0; e < L; e += tl.pe2) {
             h.j2.push_back(g);
             h.j2.push_back(static_cast<int32_t>(e));
         }
     }
-    h.gram_first.push_back(static_cast<int32_t>(h.gram_items.size() / 2));
+    h.gram_first.push_back(nchunks);
     return h;
 }
 
@@ -624,11 +767,12 @@ struct PcaLayout {
     int32_t* d_j2 = nullptr;
     double* d_sin = nullptr;
     int64_t* d_w2_off = nullptr;
-    int32_t* d_gram_items = nullptr;
+    int32_t* d_gram1_items = nullptr;
+    int32_t* d_gram2_items = nullptr;
     int32_t* d_gram_first = nullptr;
     int64_t* d_gp1_off = nullptr;
     int64_t* d_gp2_off = nullptr;
-    float* part = nullptr;
+    double* part = nullptr;
 };
 
 // Room for the subsample list of one neighborhood (at most max_s + 1
@@ -650,25 +794,26 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     l.d_p1 = take_copy(c, h.p1);
     l.d_j1 = take_copy(c, h.j1);
     l.d_j2 = take_copy(c, h.j2);
-    l.d_sin = c.take<double>(1024);
+    l.d_sin = c.take<double>(kSinTab);
     l.d_w2_off = take_copy(c, h.w2_off);
     double* work2 = c.take<double>(h.w2_total > 0 ? h.w2_total : 1);
-    l.d_gram_items = take_copy(c, h.gram_items);
+    l.d_gram1_items = take_copy(c, h.gram1_items);
+    l.d_gram2_items = take_copy(c, h.gram2_items);
     l.d_gram_first = take_copy(c, h.gram_first);
     l.d_gp1_off = take_copy(c, h.gp1_off);
     l.d_gp2_off = take_copy(c, h.gp2_off);
-    l.part = c.take<float>(std::max<int64_t>(std::max(h.gp1_total, h.gp2_total), 1));
+    l.part = c.take<double>(std::max<int64_t>(std::max(h.gp1_total, h.gp2_total), 1));
     int32_t* ns = c.take<int32_t>(G);
     int32_t* idx = c.take<int32_t>(static_cast<int64_t>(G) * idx_stride);
-    float* g1 = c.take<float>(h.g1_total > 0 ? h.g1_total : 1);
-    float* g2 = c.take<float>(h.g2_total > 0 ? h.g2_total : 1);
+    double* g1 = c.take<double>(h.g1_total > 0 ? h.g1_total : 1);
+    double* g2 = c.take<double>(h.g2_total > 0 ? h.g2_total : 1);
     float* c1 = c.take<float>(h.c1_total > 0 ? h.c1_total : 1);
     float* c2 = c.take<float>(h.c2_total > 0 ? h.c2_total : 1);
     l.bytes = c.bytes();
     l.tb = PcaTables{l.d_clip_off, l.d_red_off, l.d_feat_off, l.d_L, l.d_W, l.d_g1_off,
                      l.d_g2_off, l.d_c1_off, l.d_c2_off, l.d_p1, l.d_j1, l.d_j2, ns, idx,
-                     g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off, l.d_gram_items,
-                     l.d_gram_first, l.d_gp1_off, l.d_gp2_off};
+                     g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off, l.d_gram1_items,
+                     l.d_gram2_items, l.d_gram_first, l.d_gp1_off, l.d_gp2_off};
     return l;
 }
 
@@ -684,6 +829,9 @@ int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, 
         MSAMD_REQUIRE(widths[g] >= 1 && widths[g] <= kMaxW,
                       "pca_features: neighborhood %d has %d channels (1..%d supported)", g,
                       widths[g], kMaxW);
+        MSAMD_REQUIRE(static_cast<int64_t>(nf1) * widths[g] <= kMaxR,
+                      "pca_features: neighborhood %d: stage-2 dimension %d exceeds %d", g,
+                      nf1 * widths[g], kMaxR);
         MSAMD_REQUIRE(num_events[g] >= 0 && num_events[g] < (1LL << 31),
                       "pca_features: bad event count");
         MSAMD_REQUIRE(nf2 <= nf1 * widths[g], "pca_features: num_features > stage-2 dimension");
@@ -692,21 +840,25 @@ int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, 
     return MSAMD_OK;
 }
 
-constexpr int64_t kMaxLds = 160 * 1024;
-
 // Dynamic LDS of the stage-2 projection for a tile choice.
 int64_t project2_lds(int32_t nf1, int32_t maxW, int32_t nf2, int32_t pe2) {
     return static_cast<int64_t>(sizeof(float)) * (static_cast<int64_t>(nf1) * maxW * nf2 +
                                                   static_cast<int64_t>(pe2) * nf1 * maxW);
 }
 
-int check_tiles(const Tiles& t, int32_t nf1, int32_t maxW, int32_t nf2) {
+// The stage-2 events per item that fit the LDS next to the components
+// (wide neighborhoods take fewer than the requested pe2).
+int32_t fit_pe2(int32_t pe2, int32_t nf1, int32_t maxW, int32_t nf2) {
+    while (pe2 > 1 && project2_lds(nf1, maxW, nf2, pe2) > kMaxLds) pe2 = (pe2 + 1) / 2;
+    return pe2;
+}
+
+int check_tiles(const Tiles& t) {
     MSAMD_REQUIRE(t.gram_chunk >= kKC && t.gram_chunk <= (1 << 20) && t.gram_chunk % kKC == 0,
                   "pca tiles: gram_chunk %d must be a multiple of %d in [%d, 2^20]",
                   t.gram_chunk, kKC, kKC);
     MSAMD_REQUIRE(t.pe1 >= 1 && t.pe1 <= 4096, "pca tiles: stage-1 events per item %d", t.pe1);
-    MSAMD_REQUIRE(t.pe2 >= 1 && t.pe2 <= 4096 && project2_lds(nf1, maxW, nf2, t.pe2) <= kMaxLds,
-                  "pca tiles: stage-2 events per item %d exceed the LDS", t.pe2);
+    MSAMD_REQUIRE(t.pe2 >= 1 && t.pe2 <= 4096, "pca tiles: stage-2 events per item %d", t.pe2);
     return MSAMD_OK;
 }
 
@@ -749,20 +901,26 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     if (rc != MSAMD_OK) return rc;
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = W[g] > maxW ? W[g] : maxW;
-    rc = check_tiles(tl, nf1, maxW, nf2);
+    rc = check_tiles(tl);
     if (rc != MSAMD_OK) return rc;
-    P->h = host_plan(G, L, W, T, nf1, nf2, max_s, tl);
-    P->tiles = tl;
-    P->p = PcaPlan{G, T, nf1, nf2, max_s, P->h.maxW, idx_stride_for(max_s), tl.gram_chunk, tl.pe1,
-                   tl.pe2};
+    Tiles t = tl;
+    t.pe2 = fit_pe2(tl.pe2, nf1, maxW, nf2);
+    const int32_t gw = group_width(T, maxW);
+    const int32_t pw = maxW < kProjW ? maxW : kProjW;
+    P->h = host_plan(G, L, W, T, nf1, nf2, max_s, t, gw, pw);
+    P->tiles = t;
+    P->p = PcaPlan{G, T, nf1, nf2, max_s, P->h.maxW, gw, pw, idx_stride_for(max_s),
+                   t.gram_chunk, t.pe1, t.pe2};
     P->l = pca_layout(nullptr, P->h, G, idx_stride_for(max_s));
     const int n2 = nf1 * P->h.maxW;
-    P->lds_gram1 = sizeof(__bf16) * 2 * P->h.maxW * (T | 1) * kRowStride;
+    // stage-2 Gram items stage one band (up to 240 rows) or a pair of 128-row bands
+    const int rows2 = n2 <= 240 ? n2 : 256;
+    P->lds_gram1 = sizeof(__bf16) * (kPieces * gw * gram_rows_alloc(T) + 16) * kRowStride;
     P->lds_power1 = sizeof(double) * (T * pow_ld(T) + 2 * T + 16);
-    P->lds_proj1 = sizeof(float) * P->h.maxW * T * cpad_for(nf1);
-    P->lds_gram2 = sizeof(__bf16) * 2 * (n2 | 1) * kRowStride;
+    P->lds_proj1 = sizeof(float) * pw * T * cpad_for(nf1);
+    P->lds_gram2 = sizeof(__bf16) * (kPieces * gram_rows_alloc(rows2) + 16) * kRowStride;
     P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * pow_ld(n2) : 0) + 2 * n2 + 16);
-    P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, tl.pe2));
+    P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, t.pe2));
     return MSAMD_OK;
 }
 
@@ -778,8 +936,8 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
         return MSAMD_ERR_WORKSPACE;
     }
     static const std::vector<double> sins = [] {
-        std::vector<double> v(1024);
-        for (int i = 0; i < 1024; ++i) v[i] = std::sin(static_cast<double>(i + 1));
+        std::vector<double> v(kSinTab);
+        for (int i = 0; i < kSinTab; ++i) v[i] = std::sin(static_cast<double>(i + 1));
         return v;
     }();
     auto up = [&](void* dst, const void* src, size_t n) {
@@ -799,7 +957,8 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
     MSAMD_HIP(up(l.d_j2, h.j2.data(), h.j2.size() * 4));
     MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * sizeof(double)));
     MSAMD_HIP(up(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8));
-    MSAMD_HIP(up(l.d_gram_items, h.gram_items.data(), h.gram_items.size() * 4));
+    MSAMD_HIP(up(l.d_gram1_items, h.gram1_items.data(), h.gram1_items.size() * 4));
+    MSAMD_HIP(up(l.d_gram2_items, h.gram2_items.data(), h.gram2_items.size() * 4));
     MSAMD_HIP(up(l.d_gram_first, h.gram_first.data(), h.gram_first.size() * 4));
     MSAMD_HIP(up(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8));
     MSAMD_HIP(up(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8));
@@ -828,9 +987,9 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     const int32_t G = p.G, T = p.T, nf1 = p.nf1;
     pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
-    const unsigned ngi = static_cast<unsigned>(h.gram_items.size() / 2);
     // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
-    pca_gram_kernel<1><<<ngi, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part);
+    const unsigned ng1 = static_cast<unsigned>(h.gram1_items.size() / 3);
+    pca_gram_kernel<1><<<ng1, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part);
     MSAMD_LAUNCH_CHECK();
     {
         const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(h.maxW) * T * T, 256)),
@@ -842,7 +1001,7 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, P->lds_power1, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     if (!h.j1.empty()) {  // stage 1 projections
-        const unsigned g1 = static_cast<unsigned>(h.j1.size() / 2);
+        const unsigned g1 = static_cast<unsigned>(h.j1.size() / 3);
         if (cpad_for(nf1) == 12)
             pca_project1_kernel<12><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
         else
@@ -850,7 +1009,8 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
         MSAMD_LAUNCH_CHECK();
     }
     // stage 2 Gram
-    pca_gram_kernel<2><<<ngi, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
+    const unsigned ng2 = static_cast<unsigned>(h.gram2_items.size() / 3);
+    pca_gram_kernel<2><<<ng2, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
     MSAMD_LAUNCH_CHECK();
     {
         const int R = nf1 * h.maxW;
@@ -1044,10 +1204,8 @@ extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* 
         for_each_candidate(base, [&](int d, const Tiles& t) {
             if (d != dim || rc != MSAMD_OK) return;
             msamd_pca_plan P;
-            int32_t maxW = 0;
-            for (int32_t g = 0; g < G; ++g) maxW = widths_host[g] > maxW ? widths_host[g] : maxW;
-            if (check_tiles(t, num_features_per_channel, maxW, num_features) != MSAMD_OK) {
-                clear_error();  // candidate does not fit this shape: skipped
+            if (check_tiles(t) != MSAMD_OK) {
+                clear_error();  // candidate outside the tile limits: skipped
                 return;
             }
             rc = plan_build(&P, G, num_events_host, widths_host, T, num_features_per_channel,

@@ -522,10 +522,10 @@ bool features_of(Ctx& c, const float* X, int64_t M, int64_t N, const std::vector
     for (int32_t g = 0; g < G; ++g) {
         num[g] = eo[g + 1] - eo[g];
         width[g] = static_cast<int32_t>(nb[g].size());
-        if (width[g] > 16)
+        if (width[g] > 128)
             return c.fail("neighborhood of channel " + std::to_string(g + 1) + " has " +
                           std::to_string(width[g]) +
-                          " channels; the PCA kernels take at most 16 (reduce adjacency_radius)");
+                          " channels; the PCA kernels take at most 128 (reduce adjacency_radius)");
         nc.insert(nc.end(), nb[g].begin(), nb[g].end());
         no[g + 1] = no[g] + width[g];
         co[g + 1] = co[g] + num[g] * width[g] * T;

@@ -440,3 +440,25 @@ def pca_gram_fp64(X, K, max_samples=10000):
         P = np.eye(M) - np.outer(v, v)
         G = P @ G @ P
     return C, X @ C, sig
+
+
+def dimension_reduce_clips_fp64(clips, nf=10, max_samples=10000):
+    """Stage 1 in exact-product FP64 Gram form (pca_gram_fp64 per channel):
+    the FP64 counterpart of dimension_reduce_clips.  (L, T, W) -> (L, nf, W)
+    float64."""
+    clips = np.asarray(clips)
+    L, T, W = clips.shape
+    out = np.zeros((L, nf, W), np.float64)
+    for w in range(W):
+        _, F, _ = pca_gram_fp64(clips[:, :, w], nf, max_samples)
+        out[:, :, w] = F
+    return out
+
+
+def stage2_features_fp64(reduced, nf=10, max_samples=10000):
+    """Stage 2 in FP64 Gram form on (L, nf1, W) reduced clips (sort_clips.cpp:58-70
+    arithmetic in FP64): (L, nf) float64."""
+    reduced = np.asarray(reduced, np.float64)
+    L = reduced.shape[0]
+    _, F, _ = pca_gram_fp64(reduced.reshape(L, -1), nf, max_samples)
+    return F

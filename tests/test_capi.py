@@ -82,7 +82,12 @@ def test_pca_workspace_query():
     w = np.array([5, 3, 12, 1, 16], np.int32)
     args = (ev.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p))
     assert L.msamd_pca_workspace_size(5, *args, 50, 10, 10, 10000) > 0
-    w_bad = np.array([5, 3, 17, 1, 16], np.int32)
+    # neighborhoods up to 128 channels (the ms3alg default adjacency_radius = -1
+    # puts every channel of a 64-channel probe in each), not beyond
+    w_wide = np.array([5, 64, 128, 1, 17], np.int32)
+    assert L.msamd_pca_workspace_size(5, args[0], w_wide.ctypes.data_as(ctypes.c_void_p),
+                                      50, 10, 10, 10000) > 0
+    w_bad = np.array([5, 3, 129, 1, 16], np.int32)
     assert L.msamd_pca_workspace_size(5, args[0], w_bad.ctypes.data_as(ctypes.c_void_p),
                                       50, 10, 10, 10000) < 0
     assert L.msamd_pca_workspace_size(5, *args, 65, 10, 10, 10000) < 0

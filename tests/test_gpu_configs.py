@@ -66,7 +66,7 @@ def _check_clips_and_pca(gpu, white_o, cnt_o, t_o, nb, label):
         f_o = oc.stage2_features(r_o, 10, 10000)
         e1, e2 = assert_pca_parity(red[ro[g]:ro[g + 1]].reshape(L, 10, W), r_o,
                                    feat[fo[g]:fo[g + 1]].reshape(L, 10), f_o,
-                                   f"{label} nbhd {g} L={L} W={W}")
+                                   f"{label} nbhd {g} L={L} W={W}", clips=c)
         worst1, worst2 = max(worst1, e1), max(worst2, e2)
     say(f"  {label}: PCA worst stage1 {worst1:.2e}, stage2 {worst2:.2e}")
 
@@ -221,12 +221,14 @@ def test_config3_batch_against_oracle(gpu):
         host = clips[co[g]:co[g + 1]].cpu().numpy().reshape(Lg, 50, Wg)
         r_o = oc.dimension_reduce_clips(host, 10, 10000)
         f_o = oc.stage2_features(r_o, 10, 10000)
+        r64 = oc.dimension_reduce_clips_fp64(host, 10, 10000)
+        f64 = oc.stage2_features_fp64(r64, 10, 10000)
         assert_pca_parity(red[ro[g]:ro[g + 1]].reshape(Lg, 10, Wg), r_o,
                           feat[fo[g]:fo[g + 1]].reshape(Lg, 10), f_o,
-                          f"g={g} L={Lg} W={Wg} default tiles")
+                          f"g={g} L={Lg} W={Wg} default tiles", fp64=(r64, f64))
         assert_pca_parity(red2[ro2[g]:ro2[g + 1]].reshape(Lg, 10, Wg), r_o,
                           feat2[fo2[g]:fo2[g + 1]].reshape(Lg, 10), f_o,
-                          f"g={g} L={Lg} W={Wg} tiles {tiles}")
+                          f"g={g} L={Lg} W={Wg} tiles {tiles}", fp64=(r64, f64))
         p1 = fe.PcaPlan(L[g:g + 1], W[g:g + 1], 50, tiles=tiles, device=gpu)
         r1, _, f1, _ = p1.run(clips[co[g]:co[g + 1]].contiguous())
         assert np.array_equal(r1.cpu().numpy()[:ro2[g + 1] - ro2[g]], red2[ro2[g]:ro2[g + 1]])

@@ -50,7 +50,7 @@ def test_pca_features_match_oracle(gpu):
         f_g = feat[fo[g]:fo[g + 1]].reshape(L, 10)
         # components are defined up to the power method's own sign choice;
         # both start from sin(1..n), so signs agree and values compare directly
-        e1, e2 = assert_pca_parity(r_g, r_o, f_g, f_o, f"g={g} L={L} W={W}")
+        e1, e2 = assert_pca_parity(r_g, r_o, f_g, f_o, f"g={g} L={L} W={W}", clips=c)
         worst1, worst2 = max(worst1, e1), max(worst2, e2)
     say(f"  worst stage1 {worst1:.2e}, stage2 {worst2:.2e}")
 
@@ -120,5 +120,32 @@ def test_pca_autotune_tiles_keep_parity(gpu):
         f_o = oc.stage2_features(r_o, 10, 10000)
         r_g = red[ro[g]:ro[g + 1]].reshape(Lg, 10, Wg)
         f_g = feat[fo[g]:fo[g + 1]].reshape(Lg, 10)
-        assert_pca_parity(r_g, r_o, f_g, f_o, f"g={g} tiles {tiles}")
+        assert_pca_parity(r_g, r_o, f_g, f_o, f"g={g} tiles {tiles}", clips=c)
     plan.close()
+
+
+@pytest.mark.parametrize("shapes", [
+    [(12000, 64), (3000, 40), (700, 17)],
+    [(5000, 100), (20000, 128), (40, 64)],
+])
+def test_pca_wide_neighborhoods(gpu, shapes):
+    """Neighborhoods wider than one 16-channel work item (stage-1 channel
+    groups) and stage-2 dimensions above 256 (band pairs of the stage-2 Gram,
+    its FP64 copy in the workspace): the ms3alg default adjacency_radius = -1
+    puts all 64 channels of a 64-channel probe in every neighborhood."""
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(sum(L * W for L, W in shapes))
+    say(f"pca wide neighborhoods (L, W) = {shapes}")
+    clips = [_clips(rng, L, W) for L, W in shapes]
+    flat = torch.from_numpy(np.concatenate([c.ravel() for c in clips])).to(gpu)
+    eo = np.concatenate([[0], np.cumsum([L for L, _ in shapes])])
+    no = np.concatenate([[0], np.cumsum([W for _, W in shapes])])
+    co = np.concatenate([[0], np.cumsum([L * W * 50 for L, W in shapes])])
+    red, ro, feat, fo = fe.pca_features(flat, co, eo, no, 50)
+    red, feat = red.cpu().numpy(), feat.cpu().numpy()
+    for g, ((L, W), c) in enumerate(zip(shapes, clips)):
+        r_o = oc.dimension_reduce_clips(c, 10, 10000)
+        f_o = oc.stage2_features(r_o, 10, 10000)
+        assert_pca_parity(red[ro[g]:ro[g + 1]].reshape(L, 10, W), r_o,
+                          feat[fo[g]:fo[g + 1]].reshape(L, 10), f_o, f"g={g} L={L} W={W}",
+                          clips=c)

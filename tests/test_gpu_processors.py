@@ -199,7 +199,11 @@ def test_detect_clips_and_features_processors(gpu, exe, tmp_path):
         nbm = [c - 1 for c in oc.channels_from_geom(geom, m, 30)]
         clips = oc.extract_clips(Xw, ev[sel, 1].astype(np.int64) - oc.clip_tmid(50), nbm, 50)
         f_o = oc.stage2_features(oc.dimension_reduce_clips(clips, 10, 10000), 10, 10000)
-        e = np.max(np.abs(F[sel] - f_o)) / np.max(np.abs(f_o))
+        f64 = oc.stage2_features_fp64(oc.dimension_reduce_clips_fp64(clips, 10, 10000), 10, 10000)
+        e = np.max(np.abs(F[sel] - f64)) / np.max(np.abs(f64))
+        eo = np.max(np.abs(F[sel] - f_o)) / np.max(np.abs(f_o))
+        oo = np.max(np.abs(f_o - f64)) / np.max(np.abs(f64))
+        assert eo < PCA_STAGE2_BOUND + oo
         worst = max(worst, e)
-    say(f"  pca features: all {F.shape[1]} features max rel err {worst:.2e}")
+    say(f"  pca features: all {F.shape[1]} features, max rel err against FP64 {worst:.2e}")
     assert worst < PCA_STAGE2_BOUND

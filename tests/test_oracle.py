@@ -416,3 +416,25 @@ def test_dimension_reduce_clips_per_channel():
     f2 = oc.stage2_features(red, 10, 10000)
     _, F2, _ = oc.pca_subsampled(red.reshape(300, 30), 10, 10000)
     np.testing.assert_array_equal(f2, F2)
+
+
+def test_two_stage_fp64_form_matches_float_form():
+    """The FP64 Gram form of the two-stage PCA (the precision reference of
+    the GPU features) agrees with the reference's float data form on clips
+    whose spectrum is well separated, and its stage 2 is pca_gram_fp64 of
+    the reduced clips viewed as (nf1 * W) x L."""
+    rng = np.random.default_rng(12)
+    L, T, W = 3000, 50, 4
+    t = np.arange(T)
+    base = np.stack([np.sin(t / (3.0 + k)) * (8.0 / (k + 1)) for k in range(6)])  # 6 x T
+    coef = rng.normal(size=(L, 6, W))
+    clips = (np.einsum("lkw,kt->ltw", coef, base) + 0.05 * rng.normal(size=(L, T, W))).astype(np.float32)
+    r_o = oc.dimension_reduce_clips(clips, 3, 10000)
+    r_64 = oc.dimension_reduce_clips_fp64(clips, 3, 10000)
+    assert r_64.shape == r_o.shape
+    assert np.max(np.abs(r_o - r_64)) / np.max(np.abs(r_64)) < 1e-4
+    f_64 = oc.stage2_features_fp64(r_64, 3, 10000)
+    _, F, _ = oc.pca_gram_fp64(r_64.reshape(L, -1), 3, 10000)
+    np.testing.assert_array_equal(f_64, F)
+    f_o = oc.stage2_features(r_o, 3, 10000)
+    assert np.max(np.abs(f_o - f_64)) / np.max(np.abs(f_64)) < 1e-4
