@@ -359,6 +359,58 @@ int msamd_mask_rows(const float* X, int64_t M, int64_t N, double threshold,
                     int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Templates and the fit stage -- replace p_compute_templates
+ * (packages/ms3/p_compute_templates.cpp:24-118), the global templates of
+ * ms3alg (packages/mountainsortalg/globaltemplatecomputer.cpp:63-135) and
+ * its fit stage (fitstagecomputer.cpp:55-104, fit_stage.cpp:27-275, driven
+ * by p_mountainsort3.cpp:270-356).  Events come in host arrays (times as
+ * 0-based sample indices, labels 1..K; other labels are skipped).
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for msamd_compute_templates. */
+int64_t msamd_templates_workspace_size(int64_t L, int32_t K);
+
+/* templates (M x T x K float32, device): template k = the mean of the
+ * M x T clips starting at time - Tmid (Tmid = (T+1)/2 - 1; samples outside
+ * [0, N) read as zero) of the events labeled k, summed in FP64 in event
+ * order and divided by their count (zero when there are none): the
+ * reference's arithmetic and order.  Synchronizes the stream (the grouped
+ * event lists are uploaded from the host). */
+int msamd_compute_templates(const float* X, int64_t M, int64_t N, const int64_t* times_host,
+                            const int32_t* labels_host, int64_t L, int32_t K, int32_t T,
+                            float* templates, void* workspace, int64_t workspace_bytes,
+                            void* stream);
+
+/* Workspace bytes for msamd_fit_stage (-1 on invalid arguments). */
+int64_t msamd_fit_stage_workspace_size(int64_t M, int64_t N, const int64_t* times_host, int64_t L,
+                                       int32_t K, int32_t T, int64_t chunk_size,
+                                       int64_t padding);
+
+/* The fit stage over the time chunks [t1, t1 + chunk_size) of X (M x N
+ * float32, device), each fitted on its own copy padded by `padding`
+ * zero-filled samples with the events (times ascending) of that padded
+ * window: passes of score (sum x^2 - sum (x - template)^2 over the
+ * template's mask |template| > mask_threshold), local-maximum selection
+ * within T samples and template subtraction, until a pass uses no event.
+ * keep_host[i] = 1 for event i when the chunk whose core holds it used it
+ * (the reference's chunk-local test padding <= t1 < size).  In ms3alg:
+ * chunk_size from the plan get_time_chunk_infos(M, N, threads, threads),
+ * padding 1000, mask_threshold 0.1.  max_passes_host (nullable): the most
+ * passes any chunk took.  Synchronizes the stream. */
+int msamd_fit_stage(const float* X, int64_t M, int64_t N, const int64_t* times_host,
+                    const int32_t* labels_host, int64_t L, const float* templates, int32_t K,
+                    int32_t T, double mask_threshold, int64_t chunk_size, int64_t padding,
+                    uint8_t* keep_host, int32_t* max_passes_host, void* workspace,
+                    int64_t workspace_bytes, void* stream);
+
+/* Chunk length of get_time_chunk_infos(M, N, num_simultaneous,
+ * min_num_chunks) (p_mountainsort3.cpp:507-533): 1e9 / (4 M
+ * num_simultaneous) samples, at most N, at most N / min_num_chunks when
+ * that many chunks would not fit, at least 20000. */
+int64_t msamd_time_chunk_size(int64_t M, int64_t N, int64_t num_simultaneous,
+                              int64_t min_num_chunks);
+
+/* ------------------------------------------------------------------------
  * Learned spike-presence detector.  No counterpart in the reference: a
  * 3-layer temporal convolution network (32 hidden channels, kernel 9, ReLU,
  * strides 8 and 4) over the 7-channel window c-3..c+3 scores every

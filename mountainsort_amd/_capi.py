@@ -59,6 +59,12 @@ SIGNATURES = {
     "msamd_mask_workspace_size": (i64, [i64, i64, i64]),
     "msamd_mask_out_artifacts": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
     "msamd_mask_rows": (cint, [vp, i64, i64, f64, i64, vp, vp, vp, i64, vp]),
+    "msamd_templates_workspace_size": (i64, [i64, i32]),
+    "msamd_compute_templates": (cint, [vp, i64, i64, vp, vp, i64, i32, i32, vp, vp, i64, vp]),
+    "msamd_fit_stage_workspace_size": (i64, [i64, i64, vp, i64, i32, i32, i64, i64]),
+    "msamd_fit_stage": (cint, [vp, i64, i64, vp, vp, i64, vp, i32, i32, f64, i64, i64, vp, vp,
+                               vp, i64, vp]),
+    "msamd_time_chunk_size": (i64, [i64, i64, i64, i64]),
     "msamd_tcn_num_params": (i32, []),
     "msamd_tcn_init": (cint, [vp, ctypes.c_uint64, vp]),
     "msamd_tcn_labels": (cint, [vp, vp, i32, i64, vp, vp, vp]),

@@ -517,3 +517,77 @@ def mask_rows(X, threshold=6.0, interval_size=2000):
     _capi.call("msamd_mask_rows", _ptr(X), M, N, float(threshold), isz, _ptr(rk), _ptr(use_it),
                _ptr(ws), ws.numel(), _stream(dev))
     return rk, use_it
+
+
+# --------------------------------------------------------------------------
+# Templates and the fit stage (p_compute_templates.cpp, globaltemplatecomputer.cpp,
+# fitstagecomputer.cpp, fit_stage.cpp)
+# --------------------------------------------------------------------------
+def time_chunk_size(M, N, num_simultaneous, min_num_chunks):
+    """get_time_chunk_infos chunk length (p_mountainsort3.cpp:507-533)."""
+    return int(_capi.lib().msamd_time_chunk_size(int(M), int(N), int(num_simultaneous),
+                                                  int(min_num_chunks)))
+
+
+def _events_host(times, labels):
+    import numpy as np
+    t = np.ascontiguousarray(torch.as_tensor(times).cpu().numpy(), dtype=np.int64)
+    lab = np.ascontiguousarray(torch.as_tensor(labels).cpu().numpy(), dtype=np.int32)
+    if t.shape != lab.shape:
+        raise ValueError("times and labels must have one length")
+    return t, lab
+
+
+def compute_templates(X, times, labels, K, clip_size):
+    """ms3.compute_templates (p_compute_templates.cpp:24-118): the mean clip of
+    every label 1..K over all channels, FP64 sums in event order.  X: (N, M)
+    float32 on the device; times: 0-based sample indices; labels 1..K (others
+    are skipped).  Returns (K, T, M) float32, the M x T x K .mda memory."""
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    t, lab = _events_host(times, labels)
+    K, T = int(K), int(clip_size)
+    out = torch.zeros((max(K, 0), T, M), dtype=torch.float32, device=dev)
+    if K == 0:
+        return out
+    L = _capi.lib()
+    ws = _workspace(L.msamd_templates_workspace_size(len(t), K), dev)
+    _capi.call("msamd_compute_templates", _ptr(X), M, N, t.ctypes.data_as(ctypes.c_void_p),
+               lab.ctypes.data_as(ctypes.c_void_p), len(t), K, T, _ptr(out), _ptr(ws),
+               ws.numel(), _stream(dev))
+    return out
+
+
+def fit_stage(X, times, labels, templates, chunk_size=None, num_threads=8, padding=1000,
+              mask_threshold=0.1, return_passes=False):
+    """The ms3alg fit stage (fitstagecomputer.cpp:55-104, fit_stage.cpp:27-275):
+    X (N, M) float32 on the device, events with ascending times and labels
+    1..K, templates (K, T, M) on the device.  chunk_size defaults to the
+    reference plan get_time_chunk_infos(M, N, threads, threads).  Returns a
+    numpy bool array: which events the fit stage keeps."""
+    import numpy as np
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    t, lab = _events_host(times, labels)
+    K, T = int(templates.shape[0]), int(templates.shape[1])
+    if templates.shape[2] != M or templates.device != dev:
+        raise ValueError("templates must be (K, T, M) on the timeseries' device")
+    tm = templates.to(torch.float32).contiguous()
+    cs = int(chunk_size) if chunk_size else time_chunk_size(M, N, num_threads, num_threads)
+    L = _capi.lib()
+    pt = t.ctypes.data_as(ctypes.c_void_p)
+    nb = L.msamd_fit_stage_workspace_size(M, N, pt, len(t), K, T, cs, int(padding))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_fit_stage_workspace_size", 1,
+                               "invalid events (times must be ascending) or shapes")
+    ws = _workspace(nb, dev)
+    keep = np.zeros(max(len(t), 1), np.uint8)
+    passes = ctypes.c_int32(0)
+    _capi.call("msamd_fit_stage", _ptr(X), M, N, pt, lab.ctypes.data_as(ctypes.c_void_p), len(t),
+               _ptr(tm), K, T, float(mask_threshold), cs, int(padding),
+               keep.ctypes.data_as(ctypes.c_void_p), ctypes.byref(passes), _ptr(ws), ws.numel(),
+               _stream(dev))
+    keep = keep[:len(t)].astype(bool)
+    return (keep, passes.value) if return_passes else keep

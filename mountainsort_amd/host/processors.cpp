@@ -586,6 +586,147 @@ bool run_pca_features(Ctx& c) {
                      feat.data(), c.err);
 }
 
+// Firings (R x L, R >= 3: central channel, time, label) as times and labels.
+bool read_firings(Ctx& c, std::vector<double>& time, std::vector<int32_t>& label,
+                  std::vector<double>& all, int64_t& R) {
+    MdaHeader h;
+    if (!mda_read_f64(c.str("firings"), all, h, c.err)) return false;
+    if (h.dim(0) < 3) return c.fail("firings must have rows (central channel, time, label)");
+    R = h.dim(0);
+    const int64_t L = h.size() / R;
+    time.resize(L);
+    label.resize(L);
+    for (int64_t i = 0; i < L; ++i) {
+        time[i] = all[R * i + 1];
+        label[i] = static_cast<int32_t>(all[R * i + 2]);
+    }
+    return true;
+}
+
+// ms3.compute_templates (p_compute_templates.cpp:24-118): the mean clip of
+// every listed cluster (all labels 1..max when `clusters` is empty), in list
+// order, clips starting at (int64)(time - Tmid).
+bool run_compute_templates(Ctx& c) {
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X;
+    const int64_t T = c.integer("clip_size");
+    if (T < 1) return c.fail("Unexpected: Clip size is zero.");
+    std::vector<double> time, all;
+    std::vector<int32_t> label;
+    int64_t R = 0;
+    if (!open_timeseries(c, "timeseries", src) || !read_firings(c, time, label, all, R)) return false;
+    std::vector<int> clusters = int_list(c.str("clusters"));
+    if (clusters.empty()) {
+        int kmax = 0;
+        for (int32_t k : label) kmax = std::max(kmax, static_cast<int>(k));
+        for (int k = 1; k <= kmax; ++k) clusters.push_back(k);
+    }
+    const int64_t K = static_cast<int64_t>(clusters.size());
+    if (K == 0) return mda_write(c.str("templates_out"), MDA_FLOAT32, {src.M, T, 0}, nullptr, c.err);
+    int kmax = *std::max_element(clusters.begin(), clusters.end());
+    std::vector<int32_t> map(static_cast<size_t>(std::max(kmax, 0)) + 1, 0);
+    for (int64_t q = 0; q < K; ++q)
+        if (clusters[q] >= 0) map[clusters[q]] = static_cast<int32_t>(q + 1);
+    const int tmid = clip_tmid(T);
+    std::vector<int64_t> t;
+    std::vector<int32_t> k;
+    for (size_t i = 0; i < time.size(); ++i) {
+        const int32_t lab = label[i];
+        if (lab < 0 || lab > kmax || map[lab] == 0) continue;
+        // (bigint)(time - Tmid) is the first sample (:88); the kernel takes time = start + Tmid
+        t.push_back(static_cast<int64_t>(time[i] - tmid) + tmid);
+        k.push_back(map[lab]);
+    }
+    DeviceBuffer tm, ws;
+    const int64_t L = static_cast<int64_t>(t.size());
+    if (!s.create(c.err) || !load_f32(c, src, X, s.get()) || !tm.alloc(src.M * T * K * 4, c.err) ||
+        !ws.alloc(msamd_templates_workspace_size(L, static_cast<int32_t>(K)), c.err))
+        return false;
+    if (!lib_ok(c, msamd_compute_templates(X.as<float>(), src.M, src.N, t.data(), k.data(), L,
+                                           static_cast<int32_t>(K), static_cast<int32_t>(T),
+                                           tm.as<float>(), ws.get(), ws.bytes(), s.get()),
+                "compute_templates"))
+        return false;
+    return write_device(c, c.str("templates_out"), MDA_FLOAT32, {src.M, T, K}, tm.get(), s.get());
+}
+
+// The template and fit phase of mountainsortalg.ms3alg
+// (p_mountainsort3.cpp:270-289,313-344): global templates of every label
+// over the chunks of get_time_chunk_infos(M, N, 1, 1) (an event counts when
+// it lies in its chunk's first size - 1000 samples), then the fit stage over
+// the chunks of get_time_chunk_infos(M, N, threads, threads).  Writes the
+// firings columns of the events kept, in time order.
+bool run_fit_stage(Ctx& c) {
+    TimeseriesSource src;
+    Stream s;
+    DeviceBuffer X;
+    const int64_t T = c.integer("clip_size");
+    if (T < 1) return c.fail("clip_size must be positive");
+    std::vector<double> time, all;
+    std::vector<int32_t> label;
+    int64_t R = 0;
+    if (!open_timeseries(c, "timeseries", src) || !read_firings(c, time, label, all, R)) return false;
+    const int64_t M = src.M, N = src.N, L = static_cast<int64_t>(time.size());
+    std::vector<int64_t> order(L);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return time[a] < time[b]; });
+    std::vector<int64_t> t(L);
+    std::vector<int32_t> k(L);
+    int32_t K = 0;
+    for (int64_t i = 0; i < L; ++i) {
+        t[i] = static_cast<int64_t>(time[order[i]]);
+        k[i] = label[order[i]];
+        K = std::max(K, k[i]);
+    }
+    const int threads = reference_threads(c);
+    const int64_t pad = kDetectPadding;
+    // global templates: the events of each chunk's first size - padding samples
+    const int64_t gcs = msamd_time_chunk_size(M, N, 1, 1);
+    std::vector<int64_t> gt;
+    std::vector<int32_t> gk;
+    for (int64_t i = 0; i < L; ++i) {
+        const int64_t t1 = (t[i] / gcs) * gcs;
+        const int64_t size = std::min(gcs, N - t1);
+        if (t[i] >= 0 && t[i] < N && t[i] - t1 < size - pad) {
+            gt.push_back(t[i]);
+            gk.push_back(k[i]);
+        }
+    }
+    const int64_t fcs = msamd_time_chunk_size(M, N, threads, threads);
+    DeviceBuffer tm, tws, fws;
+    const int64_t fwsb = msamd_fit_stage_workspace_size(M, N, t.data(), L, K, static_cast<int32_t>(T),
+                                                        fcs, pad);
+    if (fwsb < 0) return c.fail("fit_stage: invalid events");
+    if (!s.create(c.err) || !load_f32(c, src, X, s.get()) ||
+        !tm.alloc(std::max<int64_t>(M * T * K, 1) * 4, c.err) ||
+        !tws.alloc(msamd_templates_workspace_size(static_cast<int64_t>(gt.size()), K), c.err) ||
+        !fws.alloc(fwsb, c.err))
+        return false;
+    std::vector<uint8_t> keep(static_cast<size_t>(std::max<int64_t>(L, 1)), 0);
+    if (K > 0) {
+        if (!lib_ok(c, msamd_compute_templates(X.as<float>(), M, N, gt.data(), gk.data(),
+                                               static_cast<int64_t>(gt.size()), K,
+                                               static_cast<int32_t>(T), tm.as<float>(), tws.get(),
+                                               tws.bytes(), s.get()),
+                    "compute_templates") ||
+            !lib_ok(c, msamd_fit_stage(X.as<float>(), M, N, t.data(), k.data(), L, tm.as<float>(), K,
+                                       static_cast<int32_t>(T), 0.1, fcs, pad, keep.data(), nullptr,
+                                       fws.get(), fws.bytes(), s.get()),
+                    "fit_stage"))
+            return false;
+    }
+    std::vector<double> out;
+    int64_t n = 0;
+    for (int64_t i = 0; i < L; ++i)
+        if (keep[i]) {
+            out.insert(out.end(), all.begin() + R * order[i], all.begin() + R * (order[i] + 1));
+            ++n;
+        }
+    return mda_write(c.str("firings_out"), MDA_FLOAT64, {R, n}, out.data(), c.err);
+}
+
 // --------------------------------------------------------------- the registry
 
 std::vector<ProcessorSpec> make_registry() {
@@ -709,6 +850,31 @@ std::vector<ProcessorSpec> make_registry() {
         p.optional("num_features", "", 10LL).optional("num_features_per_channel", "", 10LL);
         p.optional("max_pca_samples", "", 10000LL);
         p.run = run_pca_features;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "ms3.compute_templates";
+        p.version = "0.11";
+        p.input("timeseries").input("firings");
+        p.output("templates_out");
+        p.required("clip_size");
+        p.optional("clusters", "Comma-separated list of clusters to inclue", "");
+        p.run = run_compute_templates;
+        v.push_back(p);
+    }
+    {
+        ProcessorSpec p;
+        p.name = "msamd.fit_stage";
+        p.version = "0.1";
+        p.description = "Global templates and fit stage of mountainsortalg.ms3alg: the firings "
+                        "(central channel, time, label) the template-subtraction fit keeps";
+        p.input("timeseries", "Preprocessed timeseries");
+        p.input("firings", "RxL firings, R >= 3 (central channel, time, label)");
+        p.output("firings_out", "The kept columns of the firings, in time order");
+        p.optional("clip_size", "", 50LL);
+        p.optional("reference_num_threads", kRefThreads, 0LL);
+        p.run = run_fit_stage;
         v.push_back(p);
     }
     return v;

@@ -50,6 +50,9 @@ def lib():
         L.orc_pca_subsample_indices.restype = i64
         L.orc_pca_subsampled.argtypes = [vp, i64, i64, i64, i64, vp, vp, vp]
         L.orc_dimension_reduce_clips.argtypes = [vp, i64, i64, i64, i64, i64, vp]
+        L.orc_compute_templates.argtypes = [vp, i64, i64, vp, vp, i64, i32, i32, vp, vp]
+        L.orc_fit_stage.argtypes = [vp, i64, i64, vp, vp, i64, vp, i32, i32, f64, vp]
+        L.orc_fit_stage.restype = i64
         _LIB = L
     return _LIB
 
@@ -462,3 +465,102 @@ def stage2_features_fp64(reduced, nf=10, max_samples=10000):
     L = reduced.shape[0]
     _, F, _ = pca_gram_fp64(reduced.reshape(L, -1), nf, max_samples)
     return F
+
+
+
+# --------------------------------------------------------------------------
+# Templates and the fit stage (packages/ms3/p_compute_templates.cpp,
+# packages/mountainsortalg/globaltemplatecomputer.cpp, fitstagecomputer.cpp,
+# fit_stage.cpp, p_mountainsort3.cpp:270-356)
+# --------------------------------------------------------------------------
+def time_chunk_infos(M, N, num_simultaneous, min_num_chunks):
+    """get_time_chunk_infos (p_mountainsort3.cpp:507-533): [(t1, size, padding)]."""
+    chunk = 1000000000 // (int(M) * int(num_simultaneous) * 4)
+    if chunk > N:
+        chunk = N
+    if N < chunk * min_num_chunks:
+        chunk = N // int(min_num_chunks)
+    if chunk < 20000:
+        chunk = 20000
+    return [(t, min(chunk, N - t), 1000) for t in range(0, int(N), chunk)]
+
+
+def compute_templates(X, times, labels, K, T):
+    """ms3.compute_templates (p_compute_templates.cpp:24-118): X (N, M), times
+    (L,) sample indices, labels (L,) 1..K.  Returns (templates (K, T, M)
+    float32, i.e. the M x T x K .mda memory, counts (K,))."""
+    X = np.ascontiguousarray(X, np.float32)
+    N, M = X.shape
+    t = np.ascontiguousarray(times, np.float64)
+    lab = np.ascontiguousarray(labels, np.int32)
+    out = np.empty((max(int(K), 0), int(T), M), np.float32)
+    cnt = np.empty(max(int(K), 1), np.float64)
+    lib().orc_compute_templates(_p(X), M, N, _p(t), _p(lab), len(t), int(K), int(T), _p(out),
+                                _p(cnt))
+    return out, cnt[:int(K)]
+
+
+def global_template_events(times, N, M):
+    """Events counted by GlobalTemplateComputer over get_time_chunk_infos(M, N, 1, 1)
+    (globaltemplatecomputer.cpp:86-88, p_mountainsort3.cpp:270-289): an event at
+    t0 in the chunk starting at t1 counts when padding <= t0 - t1 + padding <
+    size, i.e. the last `padding` samples of each chunk's core are left out.
+    Returns a boolean mask over the events."""
+    times = np.asarray(times, np.int64)
+    ok = np.zeros(len(times), bool)
+    for t1, size, pad in time_chunk_infos(M, N, 1, 1):
+        loc = times - t1 + pad
+        ok |= (loc >= pad) & (loc < size)
+    return ok
+
+
+def global_templates(X, times, labels, K, T):
+    """The ms3alg global templates: compute_templates on the events the chunk
+    rule counts (one thread's summation order; the reference merges per-thread
+    partial sums in completion order, so its last bits vary run to run)."""
+    N, M = np.asarray(X).shape
+    sel = global_template_events(times, N, M)
+    return compute_templates(X, np.asarray(times)[sel], np.asarray(labels)[sel], K, T)
+
+
+def fit_stage(Xc, times, labels, templates, thresh=0.1):
+    """fit_stage (fit_stage.cpp:27-174) on one chunk Xc (N2, M), modified in
+    place.  times: chunk-local (ascending), labels 1..K, templates (K, T, M).
+    Returns (keep (L,) bool, passes)."""
+    Xc = np.ascontiguousarray(Xc, np.float32)
+    N2, M = Xc.shape
+    tm = np.ascontiguousarray(templates, np.float32)
+    K, T = tm.shape[0], tm.shape[1]
+    t = np.ascontiguousarray(times, np.float64)
+    lab = np.ascontiguousarray(labels, np.int32)
+    keep = np.zeros(max(len(t), 1), np.uint8)
+    passes = lib().orc_fit_stage(_p(Xc), M, N2, _p(t), _p(lab), len(t), _p(tm), K, T,
+                                 float(thresh), _p(keep))
+    return keep[:len(t)].astype(bool), int(passes)
+
+
+def fit_stage_computer(X, times, labels, templates, num_threads, thresh=0.1):
+    """FitStageComputer over get_time_chunk_infos(M, N, threads, threads)
+    (p_mountainsort3.cpp:313-344, fitstagecomputer.cpp:55-104): every chunk
+    padded by 1000 zero-filled samples fits its own copy with the events of
+    the padded window; an event is used when its chunk-local time t1 satisfies
+    padding <= t1 < size (fitstagecomputer.cpp:92-93).  Returns the sorted
+    global indices of the used events."""
+    X = np.asarray(X, np.float32)
+    N, M = X.shape
+    times = np.asarray(times, np.int64)
+    labels = np.asarray(labels, np.int32)
+    used = []
+    for t1, size, pad in time_chunk_infos(M, N, num_threads, num_threads):
+        N2 = size + 2 * pad
+        Xc = np.zeros((N2, M), np.float32)
+        a, b = t1 - pad, t1 + size + pad
+        sa, sb = max(a, 0), min(b, N)
+        Xc[sa - a:sb - a] = X[sa:sb]
+        loc = times - t1 + pad
+        sel = np.nonzero((loc >= 0) & (loc < N2))[0]
+        keep, _ = fit_stage(Xc, loc[sel], labels[sel], templates, thresh)
+        lt = loc[sel]
+        ok = keep & (lt >= pad) & (lt < size)
+        used.extend(sel[ok].tolist())
+    return np.array(sorted(used), np.int64)

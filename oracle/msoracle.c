@@ -658,3 +658,191 @@ EXPORT void orc_dimension_reduce_clips(const float* clips, int64_t W, int64_t T,
     free(FF);
     free(sg);
 }
+
+/* ------------------------------------------------------------------------
+ * Templates and the fit stage (test infrastructure: the oracle of
+ * msamd_compute_templates / msamd_fit_stage).
+ * ------------------------------------------------------------------------ */
+
+/*
+ * ms3.compute_templates, packages/ms3/p_compute_templates.cpp:24-118.
+ * X M x N (column-major), events (times as doubles, labels 1..K; others are
+ * skipped).  Clip start t1 = (int64)(time - Tmid) (:88), samples outside
+ * [0, N) read as zero (DiskReadMda32::readChunk zero-fills,
+ * diskreadmda32.cpp:631-650).  Double sums in event order (:95-99), divided
+ * by the count when nonzero (:101-108), stored as float (write32).
+ * templates M x T x K; counts K.
+ */
+EXPORT void orc_compute_templates(const float* X, int64_t M, int64_t N, const double* times,
+    const int32_t* labels, int64_t L, int32_t K, int32_t T, float* templates, double* counts)
+{
+    const int64_t Tmid = (T + 1) / 2 - 1;
+    double* sum = (double*)calloc((size_t)(M * T * (K > 0 ? K : 1)), sizeof(double));
+    for (int32_t k = 0; k < K; k++)
+        counts[k] = 0;
+    for (int64_t i = 0; i < L; i++) {
+        const int32_t k = labels[i] - 1;
+        if (k < 0 || k >= K)
+            continue;
+        const int64_t t1 = (int64_t)(times[i] - (double)Tmid);
+        double* s = sum + M * T * k;
+        for (int64_t t = 0; t < T; t++) {
+            const int64_t tt = t1 + t;
+            for (int64_t m = 0; m < M; m++) {
+                const float v = (tt >= 0 && tt < N) ? X[m + M * tt] : 0.0f;
+                s[m + M * t] = s[m + M * t] + (double)v;
+            }
+        }
+        counts[k] = counts[k] + 1.0;
+    }
+    for (int32_t k = 0; k < K; k++)
+        for (int64_t e = 0; e < M * T; e++) {
+            double v = sum[M * T * k + e];
+            if (counts[k] != 0)
+                v = v / counts[k];
+            templates[M * T * k + e] = (float)v;
+        }
+    free(sum);
+}
+
+/* fit_stage.cpp:176-192: indices m + M*t of |template| > thresh, ascending. */
+static int64_t tchmask(const float* tmpl, int64_t M, int64_t T, double thresh, int64_t* out)
+{
+    int64_t n = 0;
+    for (int64_t t = 0; t < T; t++)
+        for (int64_t m = 0; m < M; m++) {
+            const double v = fabs((double)tmpl[m + M * t]);
+            if (v > thresh)
+                out[n++] = m + M * t;
+        }
+    return n;
+}
+
+/*
+ * fit_stage, packages/mountainsortalg/fit_stage.cpp:27-174 (helpers :194-275),
+ * on one (padded) chunk X (M x N2, modified in place as the reference does).
+ * times: chunk-local times (ascending), labels 1..K.  keep[i] = 1 for the
+ * events the fit uses.  Returns the number of passes.
+ */
+EXPORT int64_t orc_fit_stage(float* X, int64_t M, int64_t N2, const double* times,
+    const int32_t* labels, int64_t L, const float* templates, int32_t K, int32_t T,
+    double thresh, uint8_t* keep)
+{
+    const int64_t Tmid = (T + 1) / 2 - 1;
+    int64_t* mask = (int64_t*)malloc(sizeof(int64_t) * (size_t)(M * T * (K > 0 ? K : 1)));
+    int64_t* msz = (int64_t*)malloc(sizeof(int64_t) * (size_t)(K > 0 ? K : 1));
+    for (int32_t k = 0; k < K; k++)
+        msz[k] = tchmask(templates + M * T * k, M, T, thresh, mask + M * T * k);
+    double* scores = (double*)calloc((size_t)(L > 0 ? L : 1), sizeof(double));
+    float* dirty = (float*)malloc(sizeof(float) * (size_t)(M * N2 > 0 ? M * N2 : 1));
+    for (int64_t i = 0; i < M * N2; i++)
+        dirty[i] = 1;
+    int64_t* consider = (int64_t*)malloc(sizeof(int64_t) * (size_t)(L > 0 ? L : 1));
+    int64_t* tri = (int64_t*)malloc(sizeof(int64_t) * (size_t)(L > 0 ? L : 1));
+    double* trs = (double*)malloc(sizeof(double) * (size_t)(L > 0 ? L : 1));
+    double* trt = (double*)malloc(sizeof(double) * (size_t)(L > 0 ? L : 1));
+    int32_t* trl = (int32_t*)malloc(sizeof(int32_t) * (size_t)(L > 0 ? L : 1));
+    uint8_t* use = (uint8_t*)malloc((size_t)(L > 0 ? L : 1));
+    int64_t nc = L;
+    for (int64_t i = 0; i < L; i++) {
+        consider[i] = i;
+        keep[i] = 0;
+    }
+    int changed = 1;
+    int64_t passes = 0;
+    while (changed) { /* :74 */
+        passes++;
+        int64_t nt = 0;
+        for (int64_t kk = 0; kk < nc; kk++) { /* :86-133 */
+            const int64_t i = consider[kk];
+            const double t0 = times[i];
+            const int32_t k0 = labels[i];
+            if (k0 > 0 && k0 <= K) {
+                const int64_t tt = (int64_t)(t0 - (double)Tmid + 0.5); /* :92 */
+                double score0 = 0;
+                if (tt >= 0 && tt + T <= N2) {
+                    const int64_t* mk = mask + M * T * (k0 - 1);
+                    const int64_t n = msz[k0 - 1];
+                    int dz = 0; /* is_dirty, :194-201 */
+                    for (int64_t j = 0; j < n; j++)
+                        if (dirty[M * tt + mk[j]] != 0) {
+                            dz = 1;
+                            break;
+                        }
+                    if (!dz) {
+                        score0 = scores[i];
+                    } else { /* compute_score, :203-219 */
+                        const float* xp = X + M * tt;
+                        const float* tp = templates + M * T * (k0 - 1);
+                        double before = 0, after = 0;
+                        for (int64_t j = 0; j < n; j++) {
+                            double val = xp[mk[j]];
+                            before += val * val;
+                            val -= tp[mk[j]];
+                            after += val * val;
+                        }
+                        score0 = before - after;
+                        scores[i] = score0;
+                    }
+                }
+                if (score0 > 0) { /* neglogprior = 0, :120-125 */
+                    trs[nt] = score0;
+                    trt[nt] = t0;
+                    trl[nt] = k0;
+                    tri[nt] = i;
+                    nt++;
+                }
+            }
+        }
+        /* find_events_to_use, :221-252 */
+        for (int64_t i = 0; i < nt; i++) {
+            use[i] = 0;
+            if (trs[i] > 0) {
+                use[i] = 1;
+                for (int64_t j = i; j >= 0 && trt[j] >= trt[i] - T; j--)
+                    if (i != j && trs[j] >= trs[i])
+                        use[i] = 0;
+                for (int64_t j = i; j < nt && trt[j] <= trt[i] + T; j++)
+                    if (trs[j] > trs[i])
+                        use[i] = 0;
+            }
+        }
+        for (int64_t i = 0; i < M * N2; i++) /* :138-141 */
+            dirty[i] = 0;
+        changed = 0;
+        int64_t nn = 0;
+        for (int64_t a = 0; a < nt; a++) { /* :148-167 */
+            if (use[a]) {
+                const int32_t k0 = trl[a];
+                const int64_t* mk = mask + M * T * (k0 - 1);
+                const int64_t n = msz[k0 - 1];
+                const int64_t tt = (int64_t)(trt[a] - (double)Tmid + 0.5);
+                float* xp = X + M * tt;
+                float* dp = dirty + M * tt;
+                const float* tp = templates + M * T * (k0 - 1);
+                /* subtract_scaled_template with scale_min = scale_max = 1, :254-275:
+                 * alpha is clamped to 1 whatever S12/S22 are */
+                for (int64_t j = 0; j < n; j++) {
+                    xp[mk[j]] -= 1.0 * tp[mk[j]];
+                    dp[mk[j]] = 1;
+                }
+                changed = 1;
+                keep[tri[a]] = 1;
+            } else {
+                consider[nn++] = tri[a];
+            }
+        }
+        nc = nn;
+    }
+    free(mask);
+    free(msz);
+    free(scores);
+    free(dirty);
+    free(consider);
+    free(tri);
+    free(trs);
+    free(trt);
+    free(trl);
+    free(use);
+    return passes;
+}

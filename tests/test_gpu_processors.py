@@ -207,3 +207,36 @@ def test_detect_clips_and_features_processors(gpu, exe, tmp_path):
         worst = max(worst, e)
     say(f"  pca features: all {F.shape[1]} features, max rel err against FP64 {worst:.2e}")
     assert worst < PCA_STAGE2_BOUND
+
+
+def test_templates_and_fit_stage_processors(gpu, tmp_path):
+    """ms3.compute_templates (every cluster, and a cluster subset in list
+    order) and msamd.fit_stage (ms3alg's global templates, then the fit)
+    against the oracle: templates bit-identical, kept firings identical."""
+    from tests.test_gpu_fit import _sorted_events
+    X, times, labels = _sorted_events(12, 240_000, seed=7, units=6)
+    keep = labels > 0
+    times, labels = times[keep], labels[keep]
+    mdaio.writemda(tmp_path / "pre.mda", X)
+    F = np.stack([np.ones(len(times)), times.astype(np.float64), labels.astype(np.float64)])
+    mdaio.writemda(tmp_path / "firings.mda", F.T.copy())   # (L, 3) rows -> 3 x L .mda
+    run("ms3.compute_templates", f"--timeseries={tmp_path / 'pre.mda'}",
+        f"--firings={tmp_path / 'firings.mda'}", f"--templates_out={tmp_path / 't.mda'}",
+        "--clip_size=50")
+    Tg = mdaio.readmda(tmp_path / "t.mda")                  # (K, T, M)
+    To, _ = oc.compute_templates(X, times, labels, 6, 50)
+    assert Tg.shape == To.shape and np.array_equal(Tg, To)
+    run("ms3.compute_templates", f"--timeseries={tmp_path / 'pre.mda'}",
+        f"--firings={tmp_path / 'firings.mda'}", f"--templates_out={tmp_path / 't2.mda'}",
+        "--clip_size=50", "--clusters=5,2")
+    T2 = mdaio.readmda(tmp_path / "t2.mda")
+    assert np.array_equal(T2, To[[4, 1]])
+    run("msamd.fit_stage", f"--timeseries={tmp_path / 'pre.mda'}",
+        f"--firings={tmp_path / 'firings.mda'}", f"--firings_out={tmp_path / 'f2.mda'}",
+        "--clip_size=50", "--reference_num_threads=8")
+    F2 = mdaio.readmda(tmp_path / "f2.mda")                 # (Lk, 3)
+    gt, _ = oc.global_templates(X, times, labels, 6, 50)
+    used = oc.fit_stage_computer(X, times, labels, gt, 8)
+    say(f"  fit stage processor: {F2.shape[0]} of {len(times)} kept (oracle {len(used)})")
+    assert np.array_equal(F2[:, 1].astype(np.int64), times[used])
+    assert np.array_equal(F2[:, 2].astype(np.int64), labels[used])

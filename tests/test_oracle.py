@@ -438,3 +438,126 @@ def test_two_stage_fp64_form_matches_float_form():
     np.testing.assert_array_equal(f_64, F)
     f_o = oc.stage2_features(r_o, 3, 10000)
     assert np.max(np.abs(f_o - f_64)) / np.max(np.abs(f_64)) < 1e-4
+
+
+# ------------------------------------------- templates and the fit stage
+def _spiky(rng, N, M, T, K, n_ev, noise=0.5):
+    X = (rng.normal(size=(N, M)) * noise).astype(np.float32)
+    t = np.arange(T)
+    waves = [(-rng.uniform(3, 6) * np.exp(-0.5 * ((t - 24) / 2.0) ** 2))[:, None]
+             * rng.uniform(0.1, 1.0, M)[None, :] for _ in range(K)]
+    times = np.sort(rng.choice(np.arange(60, N - 60), n_ev, replace=False))
+    labels = rng.integers(1, K + 1, n_ev).astype(np.int32)
+    for t0, k in zip(times, labels):
+        X[t0 - 24:t0 + 26] += waves[k - 1].astype(np.float32)
+    return X, times, labels
+
+
+def test_compute_templates_is_the_label_mean():
+    rng = np.random.default_rng(20)
+    N, M, T, K = 5000, 3, 50, 3
+    X, times, labels = _spiky(rng, N, M, T, K, 60)
+    times = np.concatenate([times, [5, N - 3]])           # clips running off both ends
+    labels = np.concatenate([labels, [1, 2]]).astype(np.int32)
+    labels[3] = 0                                           # unlabeled: skipped
+    tm, cnt = oc.compute_templates(X, times, labels, K, T)
+    Xp = np.concatenate([np.zeros((T, M)), X.astype(np.float64), np.zeros((T, M))])
+    for k in range(1, K + 1):
+        sel = times[labels == k]
+        clips = np.stack([Xp[s - 24 + T:s + 26 + T] for s in sel])
+        assert cnt[k - 1] == len(sel)
+        np.testing.assert_allclose(tm[k - 1], clips.mean(0), rtol=1e-6, atol=1e-6)
+
+
+def _fit_stage_plain(X, times, labels, templates, thresh=0.1):
+    """The fit stage restated without the dirty-flag cache: every considered
+    event's score is recomputed on every pass (equivalent, since the cache
+    only skips events none of whose masked samples changed)."""
+    X = X.astype(np.float32).copy()
+    N2, M = X.shape
+    K, T = templates.shape[:2]
+    masks = [np.nonzero(np.abs(templates[k].astype(np.float64).ravel()) > thresh)[0]
+             for k in range(K)]
+    consider = list(range(len(times)))
+    used = set()
+    while True:
+        tri = []
+        for i in consider:
+            k = int(labels[i])
+            if k <= 0:
+                continue
+            tt = int(times[i] - 24 + 0.5)
+            s = 0.0
+            if tt >= 0 and tt + T <= N2:
+                xv = X[tt:tt + T].ravel()
+                tv = templates[k - 1].ravel()
+                b = a = 0.0
+                for j in masks[k - 1]:
+                    v = float(xv[j])
+                    b += v * v
+                    v -= float(tv[j])
+                    a += v * v
+                s = b - a
+            if s > 0:
+                tri.append((i, s, float(times[i])))
+        if not tri:
+            break
+        use = []
+        for a_, (i, s, t) in enumerate(tri):
+            left = [q for q in tri[:a_] if q[2] >= t - T]
+            right = [q for q in tri[a_ + 1:] if q[2] <= t + T]
+            use.append(all(q[1] < s for q in left) and all(q[1] <= s for q in right))
+        changed = False
+        nxt = []
+        for (i, s, t), u in zip(tri, use):
+            if u:
+                k = int(labels[i])
+                tt = int(t - 24 + 0.5)
+                win = X[tt:tt + T].reshape(-1)
+                for j in masks[k - 1]:
+                    win[j] = np.float32(float(win[j]) - float(templates[k - 1].ravel()[j]))
+                X[tt:tt + T] = win.reshape(T, M)
+                used.add(i)
+                changed = True
+            else:
+                nxt.append(i)
+        consider = nxt
+        if not changed:
+            break
+    keep = np.zeros(len(times), bool)
+    keep[list(used)] = True
+    return keep
+
+
+def test_fit_stage_matches_the_uncached_restatement():
+    rng = np.random.default_rng(21)
+    N, M, T, K = 6000, 4, 50, 3
+    X, times, labels = _spiky(rng, N, M, T, K, 90)
+    # false events: noise-only times, and two duplicates of a real event
+    extra = np.array([200, 211, 3001, times[10], times[10]])
+    times = np.concatenate([times, extra])
+    labels = np.concatenate([labels, [1, 2, 3, labels[10], labels[10]]]).astype(np.int32)
+    o = np.argsort(times, kind="stable")
+    times, labels = times[o], labels[o]
+    tm, _ = oc.compute_templates(X, times, labels, K, T)
+    keep, passes = oc.fit_stage(X.copy(), times.astype(np.float64), labels, tm)
+    ref = _fit_stage_plain(X, times, labels, tm)
+    assert np.array_equal(keep, ref)
+    assert passes >= 2
+    # every planted event survives; of the three copies of event 10 one does
+    assert keep.sum() >= 90 - 3
+    dup = np.nonzero(times == times[np.searchsorted(times, extra[3])])[0]
+    assert keep[dup].sum() == 1
+
+
+def test_time_chunk_infos_closed_form():
+    # p_mountainsort3.cpp:507-533
+    assert oc.time_chunk_infos(384, 9_000_000, 8, 8) == [
+        (t, min(81380, 9_000_000 - t), 1000) for t in range(0, 9_000_000, 81380)]
+    assert oc.time_chunk_infos(4, 60_000, 1, 1) == [(0, 60_000, 1000)]
+    assert oc.time_chunk_infos(64, 30_000, 8, 8) == [(0, 20_000, 1000), (20_000, 10_000, 1000)]
+    # the detection plan is the same rule with one chunk at least and the
+    # neighborhood batch count as the number of simultaneous chunks (:170-191)
+    for M, N, T in ((384, 9_000_000, 8), (4, 1_800_000, 8), (64, 18_000_000, 16)):
+        nb = -(-M // min(T, M))
+        assert oc.detect_chunk_size(M, N, T) == oc.time_chunk_infos(M, N, nb, 1)[0][1]

@@ -34,6 +34,7 @@ REFERENCE_SPECS = [
     ("ms3.whiten_clips", "0.1", [], {"quantization_unit": 0}),
     ("ms3.extract_clips", "0.11", ["clip_size"], {"channels": None}),
     ("ms3.mask_out_artifacts", "0.1", [], {"threshold": 6, "interval_size": 2000}),
+    ("ms3.compute_templates", "0.11", ["clip_size"], {"clusters": ""}),
 ]
 
 
@@ -65,6 +66,8 @@ def test_spec_lists_reference_processors():
                    "num_features_per_channel": 10, "max_pca_samples": 10000}
     inputs = {f["name"]: f["optional"] for f in by_name["msamd.pca_features"]["inputs"]}
     assert inputs == {"timeseries": False, "event_times": False, "geom": True}
+    fit = {q["name"]: q.get("default_value") for q in by_name["msamd.fit_stage"]["parameters"]}
+    assert fit == {"clip_size": 50, "reference_num_threads": 0}
 
 
 def test_spec_of_one_processor_and_unknown_name():
