@@ -220,6 +220,21 @@ int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_
                        float* features, void* workspace, int64_t workspace_bytes, void* stream,
                        int32_t* tiles_out, float* best_ms_out);
 
+/* PCA of index subsets, the PCA of sort_clips_subset
+ * (packages/mountainsortalg/sort_clips.cpp:47-70) that the branch method
+ * (:97-123) repeats on every cluster isosplit5 returns: subset g is the
+ * columns indices[offsets_host[g] .. offsets_host[g+1]) (device int64, in
+ * that order) of data, an R x L float32 array (R <= 2048; the reduced
+ * clips W x nf1 x L viewed as (nf1*W) x L).  features receives
+ * num_features x L_g per subset, packed.  Same arithmetic as the stage-2
+ * PCA of msamd_pca_features (pca_subsampled rule with max_pca_samples). */
+int64_t msamd_pca_subsets_workspace_size(int32_t G, const int64_t* offsets_host, int32_t R,
+                                         int32_t num_features, int64_t max_pca_samples);
+int msamd_pca_subsets(const float* data, int32_t R, int64_t L, const int64_t* indices,
+                      const int64_t* offsets_host, int32_t G, int32_t num_features,
+                      int64_t max_pca_samples, float* features, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Whitening (a4-a6) -- replaces p_whiten (packages/ms3/p_whiten.cpp:45-206)
  * and whitening_matrix_from_XXt (packages/mountainsortalg/pca.cpp:634-650).

@@ -817,8 +817,9 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     return l;
 }
 
+// vectors: the stage-2 PCA alone on vectors of widths[g] entries (T = nf1 = 1).
 int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
-                   int32_t nf1, int32_t nf2, int64_t max_s) {
+                   int32_t nf1, int32_t nf2, int64_t max_s, bool vectors = false) {
     MSAMD_REQUIRE(G >= 1 && num_events && widths, "pca_features: bad neighborhood tables");
     MSAMD_REQUIRE(T >= 1 && T <= kMaxT, "pca_features: clip_size %d outside 1..%d", T, kMaxT);
     MSAMD_REQUIRE(nf1 >= 1 && nf1 <= 16 && nf2 >= 1 && nf2 <= 16,
@@ -826,9 +827,9 @@ int check_pca_args(int32_t G, const int64_t* num_events, const int32_t* widths, 
     MSAMD_REQUIRE(nf1 <= T, "pca_features: num_features_per_channel > clip_size");
     MSAMD_REQUIRE(max_s >= 1 && max_s < (1LL << 30), "pca_features: bad max_pca_samples");
     for (int32_t g = 0; g < G; ++g) {
-        MSAMD_REQUIRE(widths[g] >= 1 && widths[g] <= kMaxW,
+        MSAMD_REQUIRE(widths[g] >= 1 && widths[g] <= (vectors ? kMaxR : kMaxW),
                       "pca_features: neighborhood %d has %d channels (1..%d supported)", g,
-                      widths[g], kMaxW);
+                      widths[g], vectors ? kMaxR : kMaxW);
         MSAMD_REQUIRE(static_cast<int64_t>(nf1) * widths[g] <= kMaxR,
                       "pca_features: neighborhood %d: stage-2 dimension %d exceeds %d", g,
                       nf1 * widths[g], kMaxR);
@@ -891,13 +892,16 @@ struct msamd_pca_plan {
     const float* g_clips = nullptr;
     float* g_reduced = nullptr;
     float* g_features = nullptr;
+    // the stage-2 PCA alone, on vectors held where the reduced clips would be
+    bool vectors = false;
 };
 
 namespace {
 
 int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W, int32_t T,
-               int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl) {
-    int rc = check_pca_args(G, L, W, T, nf1, nf2, max_s);
+               int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl, bool vectors = false) {
+    int rc = check_pca_args(G, L, W, T, nf1, nf2, max_s, vectors);
+    P->vectors = vectors;
     if (rc != MSAMD_OK) return rc;
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = W[g] > maxW ? W[g] : maxW;
@@ -978,6 +982,35 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
     return MSAMD_OK;
 }
 
+// Stage 2 of the launch sequence: Gram, FP64 sums, power method, projection.
+int enqueue_stage2(const msamd_pca_plan* P, float* reduced, float* features, hipStream_t st) {
+    const HostPlan& h = P->h;
+    const PcaPlan& p = P->p;
+    const PcaLayout& l = P->l;
+    const int32_t G = p.G, nf1 = p.nf1;
+    // stage 2 Gram
+    const unsigned ng2 = static_cast<unsigned>(h.gram2_items.size() / 3);
+    pca_gram_kernel<2><<<ng2, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
+    MSAMD_LAUNCH_CHECK();
+    {
+        const int R = nf1 * h.maxW;
+        const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(R) * R, 256)),
+                      static_cast<unsigned>(G));
+        pca_gram_reduce_kernel<2><<<rg, 256, 0, st>>>(p, l.tb, l.part);
+        MSAMD_LAUNCH_CHECK();
+    }
+    // stage 2 power method
+    pca_power_kernel<2><<<G, 256, P->lds_power2, st>>>(p, l.tb);
+    MSAMD_LAUNCH_CHECK();
+    if (!h.j2.empty()) {  // stage 2 projections
+        pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, P->lds_proj2, st>>>(
+            reduced, p, l.tb, features);
+        MSAMD_LAUNCH_CHECK();
+    }
+    return MSAMD_OK;
+}
+
+
 // The launch sequence: nine kernels, each one launch over every neighborhood.
 int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, float* features,
                  hipStream_t st) {
@@ -987,6 +1020,7 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     const int32_t G = p.G, T = p.T, nf1 = p.nf1;
     pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
+    if (P->vectors) return enqueue_stage2(P, reduced, features, st);
     // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
     const unsigned ng1 = static_cast<unsigned>(h.gram1_items.size() / 3);
     pca_gram_kernel<1><<<ng1, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part);
@@ -1008,26 +1042,7 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
             pca_project1_kernel<16><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
         MSAMD_LAUNCH_CHECK();
     }
-    // stage 2 Gram
-    const unsigned ng2 = static_cast<unsigned>(h.gram2_items.size() / 3);
-    pca_gram_kernel<2><<<ng2, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
-    MSAMD_LAUNCH_CHECK();
-    {
-        const int R = nf1 * h.maxW;
-        const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(R) * R, 256)),
-                      static_cast<unsigned>(G));
-        pca_gram_reduce_kernel<2><<<rg, 256, 0, st>>>(p, l.tb, l.part);
-        MSAMD_LAUNCH_CHECK();
-    }
-    // stage 2 power method
-    pca_power_kernel<2><<<G, 256, P->lds_power2, st>>>(p, l.tb);
-    MSAMD_LAUNCH_CHECK();
-    if (!h.j2.empty()) {  // stage 2 projections
-        pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, P->lds_proj2, st>>>(
-            reduced, p, l.tb, features);
-        MSAMD_LAUNCH_CHECK();
-    }
-    return MSAMD_OK;
+    return enqueue_stage2(P, reduced, features, st);
 }
 
 void plan_free_graph(msamd_pca_plan* P) {
@@ -1238,4 +1253,85 @@ extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* 
     tiles_out[2] = best.pe2;
     if (best_ms_out) *best_ms_out = best_ms;
     return MSAMD_OK;
+}
+
+// --------------------------------------------------- PCA of index subsets
+// The PCA of sort_clips_subset (packages/mountainsortalg/sort_clips.cpp:47-70)
+// for many subsets at once: subset g takes the columns indices[off[g] ..
+// off[g+1]) of an R x L array, in that order; each subset's columns are
+// gathered into one packed R x L_g block and the stage-2 kernels run on all
+// blocks (pca_subsampled column rule, power method, projection).
+namespace {
+
+__global__ void __launch_bounds__(256) subset_gather_kernel(const float* __restrict__ data, int64_t R,
+                                                            const int64_t* __restrict__ indices,
+                                                            int64_t total, float* __restrict__ out) {
+    const int64_t n = total * R;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t j = q / R;
+        out[q] = data[indices[j] * R + (q - j * R)];
+    }
+}
+
+int subsets_plan(msamd_pca_plan* P, int32_t G, const int64_t* offsets, int32_t R, int32_t nf,
+                 int64_t max_s, std::vector<int64_t>& sizes) {
+    MSAMD_REQUIRE(G >= 1 && offsets && offsets[0] == 0, "pca_subsets: bad subset offsets");
+    MSAMD_REQUIRE(R >= 1 && R <= kMaxR, "pca_subsets: vector length %d outside 1..%d", R, kMaxR);
+    sizes.resize(G);
+    for (int32_t g = 0; g < G; ++g) {
+        MSAMD_REQUIRE(offsets[g + 1] >= offsets[g], "pca_subsets: offsets not ascending");
+        sizes[g] = offsets[g + 1] - offsets[g];
+    }
+    std::vector<int32_t> widths(G, R);
+    return plan_build(P, G, sizes.data(), widths.data(), 1, 1, nf, max_s, Tiles{}, true);
+}
+
+}  // namespace
+
+extern "C" int64_t msamd_pca_subsets_workspace_size(int32_t G, const int64_t* offsets_host,
+                                                    int32_t R, int32_t num_features,
+                                                    int64_t max_pca_samples) {
+    msamd_pca_plan P;
+    std::vector<int64_t> sizes;
+    if (subsets_plan(&P, G, offsets_host, R, num_features, max_pca_samples, sizes) != MSAMD_OK) {
+        clear_error();
+        return -1;
+    }
+    Carver c(nullptr);
+    c.take<float>(std::max<int64_t>(offsets_host[G] * R, 1));
+    c.take<char>(P.l.bytes);
+    return c.bytes();
+}
+
+extern "C" int msamd_pca_subsets(const float* data, int32_t R, int64_t L, const int64_t* indices,
+                                 const int64_t* offsets_host, int32_t G, int32_t num_features,
+                                 int64_t max_pca_samples, float* features, void* workspace,
+                                 int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(L >= 0, "pca_subsets: bad L");
+    msamd_pca_plan P;
+    std::vector<int64_t> sizes;
+    int rc = subsets_plan(&P, G, offsets_host, R, num_features, max_pca_samples, sizes);
+    if (rc != MSAMD_OK) return rc;
+    const int64_t total = offsets_host[G];
+    Carver c(workspace);
+    float* gathered = c.take<float>(std::max<int64_t>(total * R, 1));
+    char* pws = c.take<char>(P.l.bytes);
+    if (!workspace || workspace_bytes < c.bytes()) {
+        set_error("pca_subsets: workspace too small (%lld < %lld)", (long long)workspace_bytes,
+                  (long long)c.bytes());
+        return MSAMD_ERR_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    if (total > 0) {
+        int64_t blocks = ceil_div(total * R, 256);
+        if (blocks > 65536) blocks = 65536;
+        subset_gather_kernel<<<static_cast<unsigned>(blocks), 256, 0, st>>>(data, R, indices, total,
+                                                                            gathered);
+        MSAMD_LAUNCH_CHECK();
+    }
+    rc = plan_upload(&P, pws, P.l.bytes, st);
+    if (rc != MSAMD_OK) return rc;
+    return plan_enqueue(&P, nullptr, gathered, features, st);
 }

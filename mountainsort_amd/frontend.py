@@ -591,3 +591,91 @@ def fit_stage(X, times, labels, templates, chunk_size=None, num_threads=8, paddi
                _stream(dev))
     keep = keep[:len(t)].astype(bool)
     return (keep, passes.value) if return_passes else keep
+
+
+# --------------------------------------------------------------------------
+# Branch re-sorting: sort_clips (packages/mountainsortalg/sort_clips.cpp:29-125)
+# --------------------------------------------------------------------------
+def pca_subsets(data, indices, offsets, num_features=10, max_pca_samples=10000):
+    """PCA features of index subsets of one (L, R) float32 device array (the
+    R x L .mda memory): subset g is columns indices[offsets[g]:offsets[g+1]].
+    Returns (features_flat, feature_offsets): subset g's (L_g, nf) features
+    at features_flat[feature_offsets[g]:]."""
+    import numpy as np
+    if data.dim() != 2 or data.dtype != torch.float32 or not data.is_contiguous():
+        raise ValueError("data must be a contiguous (L, R) float32 tensor")
+    L, R = data.shape
+    dev = data.device
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    G = len(off) - 1
+    idx = torch.as_tensor(indices, dtype=torch.int64).to(dev).contiguous()
+    nf = int(num_features)
+    lib = _capi.lib()
+    po = off.ctypes.data_as(ctypes.c_void_p)
+    nb = lib.msamd_pca_subsets_workspace_size(G, po, R, nf, int(max_pca_samples))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_pca_subsets_workspace_size", 1, "unsupported subsets")
+    ws = _workspace(nb, dev)
+    foff = off * nf
+    feats = torch.empty(max(int(foff[-1]), 1), dtype=torch.float32, device=dev)
+    _capi.call("msamd_pca_subsets", _ptr(data), R, L, _ptr(idx), po, G, nf,
+               int(max_pca_samples), _ptr(feats), _ptr(ws), ws.numel(), _stream(dev))
+    return feats, foff
+
+
+def sort_clips(clips, cluster, num_features=10, max_pca_samples=10000, max_depth=64):
+    """sort_clips with the branch method (sort_clips.cpp:29-125), the
+    clustering supplied by the caller (isosplit5 in the reference):
+    cluster(features) takes an (L0, nf) float32 numpy array and returns L0
+    integer labels (1..K0; 0 = unassigned).  clips: (L, T, M) float32 on the
+    device (the M x T x L reduced clips).  Each level of the recursion runs
+    the PCA of all its subsets in one batched launch; the labels are composed
+    in the reference's depth-first order (the sub-labels of cluster k offset
+    by the clusters found under clusters 1..k-1).  Returns (L,) int labels."""
+    import numpy as np
+    L = int(clips.shape[0])
+    data = clips.reshape(L, -1).contiguous()
+    root = {"idx": np.arange(L, dtype=np.int64), "children": None, "labels": None}
+    level = [root]
+    depth = 0
+    while level:
+        depth += 1
+        if depth > max_depth:
+            raise RuntimeError(f"sort_clips: recursion deeper than {max_depth}")
+        off = np.concatenate([[0], np.cumsum([len(n["idx"]) for n in level])]).astype(np.int64)
+        feats, foff = pca_subsets(data, np.concatenate([n["idx"] for n in level]), off,
+                                  num_features, max_pca_samples)
+        feats = feats.cpu().numpy()
+        nxt = []
+        for g, node in enumerate(level):
+            Lg = len(node["idx"])
+            F = feats[foff[g]:foff[g + 1]].reshape(Lg, num_features)
+            lab = np.asarray(cluster(F), dtype=np.int64).reshape(Lg)
+            node["labels"] = lab
+            K0 = int(lab.max()) if Lg else 0
+            if K0 > 1:
+                node["children"] = []
+                for k in range(1, K0 + 1):
+                    sel = np.nonzero(lab == k)[0]
+                    child = {"idx": node["idx"][sel], "pos": sel, "children": None,
+                             "labels": None} if len(sel) else None
+                    node["children"].append(child)
+                    if child is not None:
+                        nxt.append(child)
+        level = nxt
+
+    def resolve(node):
+        lab = node["labels"]
+        if node["children"] is None:
+            return lab
+        out = np.zeros(len(lab), np.int64)
+        koff = 0
+        for child in node["children"]:
+            if child is None:
+                continue
+            sub = resolve(child)
+            out[child["pos"]] = koff + sub
+            koff += int(sub.max()) if len(sub) else 0
+        return out
+
+    return resolve(root)

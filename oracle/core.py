@@ -564,3 +564,33 @@ def fit_stage_computer(X, times, labels, templates, num_threads, thresh=0.1):
         ok = keep & (lt >= pad) & (lt < size)
         used.extend(sel[ok].tolist())
     return np.array(sorted(used), np.int64)
+
+
+def sort_clips(clips, cluster, num_features=10, max_samples=10000):
+    """sort_clips / sort_clips_subset (sort_clips.cpp:29-125) with the float-form
+    PCA: the subset's clips viewed as (M*T) x L0 (:58-67), pca_subsampled
+    (:70), the caller's clustering in place of isosplit5 (:73-88), and the
+    branch method, depth first, labels of cluster k offset by the clusters
+    found under clusters 1..k-1 (:97-123).  clips (L, T, M).  Returns (L,)
+    labels."""
+    clips = np.ascontiguousarray(clips, np.float32)
+    L = clips.shape[0]
+
+    def subset(indices):
+        X = clips[indices].reshape(len(indices), -1)
+        _, F, _ = pca_subsampled(X, num_features, max_samples)
+        lab = np.asarray(cluster(F), dtype=np.int64).reshape(len(indices))
+        K0 = int(lab.max()) if len(lab) else 0
+        if K0 <= 1:
+            return lab
+        new = np.zeros(len(indices), np.int64)
+        koff = 0
+        for k in range(1, K0 + 1):
+            sel = np.nonzero(lab == k)[0]
+            if len(sel):
+                sub = subset(indices[sel])
+                new[sel] = koff + sub
+                koff += int(sub.max())
+        return new
+
+    return subset(np.arange(L, dtype=np.int64))

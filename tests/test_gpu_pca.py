@@ -149,3 +149,75 @@ def test_pca_wide_neighborhoods(gpu, shapes):
         assert_pca_parity(red[ro[g]:ro[g + 1]].reshape(L, 10, W), r_o,
                           feat[fo[g]:fo[g + 1]].reshape(L, 10), f_o, f"g={g} L={L} W={W}",
                           clips=c)
+
+
+def _reduced_clusters(rng, sizes, W, nf1=10, sep=6.0):
+    """Reduced clips (L, nf1, W) of well-separated clusters of the given sizes."""
+    parts = [rng.normal(size=(nf1, W)) * sep + rng.normal(size=(n, nf1, W)) for n in sizes]
+    X = np.concatenate(parts).astype(np.float32)
+    truth = np.repeat(np.arange(len(sizes)), sizes)
+    o = rng.permutation(len(truth))
+    return np.ascontiguousarray(X[o]), truth[o]
+
+
+@pytest.mark.parametrize("W", [4, 16, 64])
+def test_pca_subsets_match_oracle(gpu, W):
+    """Every subset's features against pca_subsampled on that subset (the
+    reference's float form) and the FP64 Gram form, including empty, one-event
+    and subsampled (L0 > 10000) subsets."""
+    from mountainsort_amd import frontend as fe
+    from tests.conftest import PCA_STAGE2_BOUND, _rel
+    rng = np.random.default_rng(W)
+    L = 24000
+    X, _ = _reduced_clusters(rng, [6000, 9000, 5000, 4000], W)
+    sizes = [0, 1, 7, 300, 12001, 2500]
+    idx = [np.sort(rng.choice(L, n, replace=False)) if n else np.zeros(0, np.int64)
+           for n in sizes]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    say(f"pca subsets W={W} R={10 * W} sizes={sizes}")
+    feats, foff = fe.pca_subsets(torch.from_numpy(X.reshape(L, -1)).to(gpu),
+                                 np.concatenate(idx), off)
+    feats = feats.cpu().numpy()
+    for g, (n, ix) in enumerate(zip(sizes, idx)):
+        if n == 0:
+            assert foff[g + 1] == foff[g]
+            continue
+        sub = X[ix].reshape(n, -1)
+        f_g = feats[foff[g]:foff[g + 1]].reshape(n, 10)
+        _, f_o, _ = oc.pca_subsampled(sub, 10, 10000)
+        _, f_64, _ = oc.pca_gram_fp64(sub, 10, 10000)
+        # a subset of n < 10 events spans n dimensions: the later components
+        # come from the power method run on the rounding residue of the
+        # deflation and are arbitrary, in the reference as here
+        assert np.all(np.isfinite(f_g))
+        k = min(n, 10)
+        f_g, f_o, f_64 = f_g[:, :k], f_o[:, :k], f_64[:, :k]
+        e, g_o, o = _rel(f_g, f_64), _rel(f_g, f_o), _rel(f_o, f_64)
+        say(f"  subset {g} L0={n}: vs FP64 {e:.2e}, vs oracle {g_o:.2e} (oracle vs FP64 {o:.2e})")
+        assert e < PCA_STAGE2_BOUND
+        assert g_o < PCA_STAGE2_BOUND + o
+
+
+@pytest.mark.parametrize("W,sizes", [(8, [900, 400, 1200, 300, 700, 500]),
+                                     (32, [5000, 6000, 3000, 800, 2500, 4000, 1500, 900])])
+def test_sort_clips_branch_method_matches_oracle(gpu, W, sizes):
+    """The branch-method recursion: labels identical to the oracle's
+    depth-first restatement with the same caller clustering, and the
+    separated clusters recovered."""
+    from mountainsort_amd import frontend as fe
+    from tests.test_oracle import gap_split
+    rng = np.random.default_rng(7 + W)
+    X, truth = _reduced_clusters(rng, sizes, W)
+    say(f"sort_clips W={W} L={len(truth)} clusters={len(sizes)}")
+    calls = []
+
+    def cluster(F):
+        calls.append(len(F))
+        return gap_split(F)
+
+    lab_g = fe.sort_clips(torch.from_numpy(X).to(gpu), cluster)
+    lab_o = oc.sort_clips(X, gap_split)
+    say(f"  {len(calls)} clusterings, {lab_g.max()} clusters (oracle {lab_o.max()})")
+    assert np.array_equal(lab_g, lab_o)
+    pairs = {(int(a), int(b)) for a, b in zip(lab_g, truth)}
+    assert len(pairs) == len(set(lab_g.tolist())) == len(sizes)

@@ -561,3 +561,48 @@ def test_time_chunk_infos_closed_form():
     for M, N, T in ((384, 9_000_000, 8), (4, 1_800_000, 8), (64, 18_000_000, 16)):
         nb = -(-M // min(T, M))
         assert oc.detect_chunk_size(M, N, T) == oc.time_chunk_infos(M, N, nb, 1)[0][1]
+
+
+def gap_split(F, min_side=10, frac=0.3):
+    """A deterministic stand-in for isosplit5 in tests: split the events at
+    the largest gap, relative to the feature's range, over all features when
+    it is a large fraction of that range (labels 1, 2), else one cluster.
+    (The PCA does not center, so the first feature is mostly the mean.)"""
+    F = np.asarray(F, np.float64)
+    L0 = len(F)
+    lab = np.ones(L0, np.int64)
+    if L0 < 2 * min_side + 2:
+        return lab
+    best = (frac, -1, -1)
+    for f in range(F.shape[1]):
+        o = np.argsort(F[:, f], kind="stable")
+        xs = F[o, f]
+        rng_ = xs[-1] - xs[0]
+        if not rng_ > 0:
+            continue
+        gaps = np.diff(xs)[min_side - 1:L0 - min_side] / rng_
+        j = int(np.argmax(gaps))
+        if gaps[j] > best[0]:
+            best = (gaps[j], f, j + min_side - 1)
+    if best[1] >= 0:
+        o = np.argsort(F[:, best[1]], kind="stable")
+        lab[o[best[2] + 1:]] = 2
+    return lab
+
+
+def _clustered_reduced(rng, L_per, K, T=10, M=4, sep=25.0):
+    centers = rng.normal(size=(K, T, M)) * sep
+    truth = np.repeat(np.arange(K), L_per)
+    clips = centers[truth] + rng.normal(size=(K * L_per, T, M))
+    o = rng.permutation(len(truth))
+    return clips[o].astype(np.float32), truth[o]
+
+
+def test_sort_clips_branch_method_recovers_separated_clusters():
+    rng = np.random.default_rng(30)
+    clips, truth = _clustered_reduced(rng, 150, 5)
+    lab = oc.sort_clips(clips, gap_split)
+    assert lab.min() >= 1
+    # every found cluster is pure and every true cluster is found whole
+    pairs = {(int(a), int(b)) for a, b in zip(lab, truth)}
+    assert len(pairs) == len(set(lab.tolist())) == len(set(truth.tolist())) == 5
