@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "libmsoracle.so")
 
 
 def build(verbose=False):
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
            "-std=c11", "-o", OUT + ".tmp", SRC, "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)

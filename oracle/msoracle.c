@@ -846,3 +846,227 @@ EXPORT int64_t orc_fit_stage(float* X, int64_t M, int64_t N2, const double* time
     free(use);
     return passes;
 }
+
+/* ------------------------------------------------------------------------
+ * OpenMP forms of the reference's threaded loops, for timing the CPU path
+ * (the CPU-baseline leg of bench.py).  Per element they do the arithmetic of
+ * the sequential forms above; the threads split the work as the reference's
+ * OpenMP loops do, with the reference's critical sections around its reads
+ * and writes (here in-memory copies in place of its .mda chunk I/O).
+ * ------------------------------------------------------------------------ */
+
+/* p_whiten.cpp:75-111: one OpenMP iteration per chunk of chunk_size samples
+ * (1e5 in the reference, :53), a double M x M partial per chunk, merged in a
+ * critical section in completion order.  Divided by N-1 (:114-118). */
+EXPORT void orc_whiten_covariance_mt(const float* X, int64_t M, int64_t N, int64_t chunk_size,
+    int nthreads, double* XXt)
+{
+    if (N < chunk_size)
+        chunk_size = N;
+    memset(XXt, 0, sizeof(double) * M * M);
+    int64_t nch = chunk_size > 0 ? (N + chunk_size - 1) / chunk_size : 0;
+#pragma omp parallel for num_threads(nthreads)
+    for (int64_t c = 0; c < nch; c++) {
+        int64_t tp = c * chunk_size;
+        int64_t n = chunk_size < N - tp ? chunk_size : N - tp;
+        float* chunk = (float*)malloc(sizeof(float) * M * n);
+        double* part = (double*)calloc(M * M, sizeof(double));
+#pragma omp critical(orc_lock1)
+        memcpy(chunk, X + M * tp, sizeof(float) * M * n);
+        for (int64_t i = 0; i < n; i++) { /* :87-96 */
+            const float* x = chunk + M * i;
+            int64_t bb = 0;
+            for (int64_t m1 = 0; m1 < M; m1++)
+                for (int64_t m2 = 0; m2 < M; m2++) {
+                    part[bb] += x[m1] * x[m2]; /* float product, double sum */
+                    bb++;
+                }
+        }
+#pragma omp critical(orc_lock2)
+        for (int64_t bb = 0; bb < M * M; bb++)
+            XXt[bb] += part[bb];
+        free(chunk);
+        free(part);
+    }
+    if (N > 1)
+        for (int64_t ii = 0; ii < M * M; ii++)
+            XXt[ii] /= (N - 1);
+}
+
+/* p_whiten.cpp:160-200: per chunk the float accumulation into the output
+ * chunk (:174-183), then the 1e-4 quantize and the write in a critical
+ * section (:185-195). */
+EXPORT void orc_whiten_apply_mt(const float* X, int64_t M, int64_t N, const double* W,
+    int64_t chunk_size, int nthreads, float* Y)
+{
+    if (N < chunk_size)
+        chunk_size = N;
+    int64_t nch = chunk_size > 0 ? (N + chunk_size - 1) / chunk_size : 0;
+#pragma omp parallel for num_threads(nthreads)
+    for (int64_t c = 0; c < nch; c++) {
+        int64_t tp = c * chunk_size;
+        int64_t n = chunk_size < N - tp ? chunk_size : N - tp;
+        float* in = (float*)malloc(sizeof(float) * M * n);
+        float* out = (float*)calloc(M * n, sizeof(float));
+#pragma omp critical(orc_lock1)
+        memcpy(in, X + M * tp, sizeof(float) * M * n);
+        for (int64_t i = 0; i < n; i++) {
+            int64_t aa = M * i;
+            int64_t bb = 0;
+            for (int64_t m1 = 0; m1 < M; m1++)
+                for (int64_t m2 = 0; m2 < M; m2++) {
+                    out[aa + m1] += in[aa + m2] * W[bb];
+                    bb++;
+                }
+        }
+#pragma omp critical(orc_lock2)
+        {
+            for (int64_t i = 0; i < M * n; i++)
+                Y[M * tp + i] = (float)quantize_d(out[i], 0.0001);
+        }
+        free(in);
+        free(out);
+    }
+}
+
+/* Clips of the ms3alg read phase, per neighborhood. */
+typedef struct {
+    int64_t G, T;
+    int64_t* W;
+    int64_t* L;
+    int64_t* cap;
+    float** clips; /* neighborhood g: W[g] x T x L[g] (column-major) */
+} orc_clipset;
+
+EXPORT void orc_clipset_free(orc_clipset* cs)
+{
+    if (!cs)
+        return;
+    for (int64_t g = 0; g < cs->G; g++)
+        free(cs->clips[g]);
+    free(cs->clips);
+    free(cs->W);
+    free(cs->L);
+    free(cs->cap);
+    free(cs);
+}
+
+EXPORT void orc_clipset_counts(const orc_clipset* cs, int64_t* L)
+{
+    for (int64_t g = 0; g < cs->G; g++)
+        L[g] = cs->L[g];
+}
+
+EXPORT void orc_clipset_get(const orc_clipset* cs, int64_t g, float* out)
+{
+    memcpy(out, cs->clips[g], sizeof(float) * cs->W[g] * cs->T * cs->L[g]);
+}
+
+/*
+ * The read phase of mountainsortalg.ms3alg (p_mountainsort3.cpp:170-216):
+ * neighborhoods in batches of `batch` (get_neighborhood_batches :47-57); for
+ * each time chunk (chunk_size, padding; zero-filled outside [0, N)), for each
+ * batch, an OpenMP loop over the batch's neighborhoods running
+ * NeighborhoodSorter::addTimeChunk (neighborhoodsorter.cpp:74-110): detect on
+ * the central channel (nb_ch[nb_off[g]]), keep the core events, gather their
+ * clips (get_clip :409-419).  X is N x M row-major (the M x N .mda memory).
+ */
+EXPORT orc_clipset* orc_ms3alg_clips_mt(const float* X, int64_t M, int64_t N, int64_t chunk_size,
+    int64_t padding, const int32_t* nb_ch, const int64_t* nb_off, int64_t G, int64_t batch,
+    double detect_threshold, double detect_interval, int sign, int64_t T, int nthreads)
+{
+    orc_clipset* cs = (orc_clipset*)calloc(1, sizeof(orc_clipset));
+    cs->G = G;
+    cs->T = T;
+    cs->W = (int64_t*)calloc(G > 0 ? G : 1, sizeof(int64_t));
+    cs->L = (int64_t*)calloc(G > 0 ? G : 1, sizeof(int64_t));
+    cs->cap = (int64_t*)calloc(G > 0 ? G : 1, sizeof(int64_t));
+    cs->clips = (float**)calloc(G > 0 ? G : 1, sizeof(float*));
+    for (int64_t g = 0; g < G; g++)
+        cs->W[g] = nb_off[g + 1] - nb_off[g];
+    const int64_t Tmid = (T + 1) / 2 - 1;
+    float* chunk = (float*)malloc(sizeof(float) * M * (chunk_size + 2 * padding));
+    for (int64_t t = 0; t < N; t += chunk_size) { /* get_time_chunk_infos */
+        int64_t size = chunk_size < N - t ? chunk_size : N - t;
+        int64_t len = size + 2 * padding;
+        for (int64_t i = 0; i < len; i++) { /* readChunk, zero fill outside [0, N) */
+            int64_t tt = t - padding + i;
+            if (tt >= 0 && tt < N)
+                memcpy(chunk + M * i, X + M * tt, sizeof(float) * M);
+            else
+                memset(chunk + M * i, 0, sizeof(float) * M);
+        }
+        for (int64_t b0 = 0; b0 < G; b0 += batch) {
+            int64_t b1 = b0 + batch < G ? b0 + batch : G;
+#pragma omp parallel for num_threads(nthreads)
+            for (int64_t g = b0; g < b1; g++) {
+                const int64_t W = cs->W[g];
+                if (W == 0)
+                    continue;
+                const int32_t* ch = nb_ch + nb_off[g];
+                double* X0 = (double*)malloc(sizeof(double) * len);
+                int64_t* ev = (int64_t*)malloc(sizeof(int64_t) * len);
+                for (int64_t i = 0; i < len; i++) /* :83-87 */
+                    X0[i] = chunk[ch[0] + M * i];
+                int64_t ne = orc_detect_events(X0, len, detect_threshold, detect_interval, sign, ev);
+                int64_t nk = 0;
+                for (int64_t k = 0; k < ne; k++) /* :92-97 */
+                    if ((0 <= ev[k] - padding) && (ev[k] - padding < len - padding - padding))
+                        ev[nk++] = ev[k];
+                if (cs->L[g] + nk > cs->cap[g]) {
+                    int64_t nc = 2 * (cs->L[g] + nk) + 1024;
+                    cs->clips[g] = (float*)realloc(cs->clips[g], sizeof(float) * W * T * nc);
+                    cs->cap[g] = nc;
+                }
+                float* out = cs->clips[g] + W * T * cs->L[g];
+                for (int64_t k = 0; k < nk; k++) /* :99-106, get_clip :409-419 */
+                    for (int64_t tt = 0; tt < T; tt++) {
+                        int64_t s = ev[k] + tt - Tmid;
+                        for (int64_t w = 0; w < W; w++)
+                            out[w + W * tt + W * T * k] = (s >= 0 && s < len) ? chunk[ch[w] + M * s] : 0.0f;
+                    }
+                cs->L[g] += nk;
+                free(X0);
+                free(ev);
+            }
+        }
+    }
+    free(chunk);
+    return cs;
+}
+
+/*
+ * The PCA part of NeighborhoodSorter::sort (neighborhoodsorter.cpp:153-160),
+ * neighborhoods in batches of `batch`, an OpenMP loop over each batch
+ * (p_mountainsort3.cpp:222-241): dimension_reduce_clips (stage 1) and the
+ * first sort_clips_subset PCA (sort_clips.cpp:58-70, stage 2) on the
+ * neighborhood's (W*nf) x L reduced clips.  clips[g]: W[g] x T x L[g];
+ * reduced[g]: W[g] x nf x L[g]; features[g]: nf2 x L[g] (either may be NULL).
+ */
+EXPORT void orc_pca_neighborhoods_mt(const float* const* clips, const int64_t* W, const int64_t* L,
+    int64_t G, int64_t T, int64_t nf, int64_t nf2, int64_t max_samples, int64_t batch,
+    int nthreads, float* const* reduced, float* const* features)
+{
+    for (int64_t b0 = 0; b0 < G; b0 += batch) {
+        int64_t b1 = b0 + batch < G ? b0 + batch : G;
+#pragma omp parallel for num_threads(nthreads)
+        for (int64_t g = b0; g < b1; g++) {
+            int64_t Wg = W[g], Lg = L[g];
+            if (Wg == 0 || Lg == 0)
+                continue;
+            float* red = reduced && reduced[g] ? reduced[g] : (float*)malloc(sizeof(float) * Wg * nf * Lg);
+            orc_dimension_reduce_clips(clips[g], Wg, T, Lg, nf, max_samples, red);
+            int64_t R = Wg * nf;
+            float* C = (float*)malloc(sizeof(float) * R * nf2);
+            float* F = features && features[g] ? features[g] : (float*)malloc(sizeof(float) * nf2 * Lg);
+            float* s = (float*)malloc(sizeof(float) * nf2);
+            orc_pca_subsampled(red, R, Lg, nf2, max_samples, C, F, s);
+            if (!(features && features[g]))
+                free(F);
+            if (!(reduced && reduced[g]))
+                free(red);
+            free(C);
+            free(s);
+        }
+    }
+}
