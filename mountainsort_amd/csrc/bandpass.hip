@@ -20,6 +20,8 @@
 // processed in units of (channel group, window) whose scratch spectra stay
 // in the MI355X last-level cache between the three stages.
 #include <cmath>
+#include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -91,6 +93,7 @@ struct BpPlan {
     int64_t nWin, nUnits;
     int32_t U;       // units per batch
     int32_t nTwHi;
+    int32_t streams; // batches go round-robin over this many streams, one spectra buffer each
 };
 
 // Factorizations with kernels compiled in: n = (R1*R2) * (Q1*Q2).
@@ -532,7 +535,7 @@ bool pick_shape(int64_t n, Shape* s) {
 }
 
 BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double qunit,
-                 bool in_i16, const Shape& sh, int64_t scratch_budget) {
+                 bool in_i16, const Shape& sh, int64_t scratch_budget, int streams) {
     BpPlan p{};
     p.M = M;
     p.N = N;
@@ -552,16 +555,20 @@ BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double
     p.nWin = ceil_div(N, cs);
     p.nUnits = p.nWin * p.nGroups;
     const int64_t per_unit = p.n * p.CG * static_cast<int64_t>(sizeof(cf));
-    int64_t U = scratch_budget / per_unit;
+    int64_t U = scratch_budget / streams / per_unit;
     if (U < 1) U = 1;
     if (U > p.nUnits) U = p.nUnits;
     p.U = static_cast<int32_t>(U);
+    const int64_t nbatch = ceil_div(p.nUnits, U);
+    p.streams = static_cast<int32_t>(nbatch < streams ? nbatch : streams);
     p.nTwHi = static_cast<int32_t>(ceil_div(p.n, kTwLo));
     return p;
 }
 
+constexpr int kMaxStreams = 4;
+
 struct BpBuffers {
-    cf* S;
+    cf* S[kMaxStreams];
     float* Ht;
     cf* twN1;
     cf* twN2;
@@ -572,7 +579,7 @@ struct BpBuffers {
 BpBuffers bp_carve(void* ws, const BpPlan& p, int64_t* bytes) {
     Carver c(ws);
     BpBuffers b{};
-    b.S = c.take<cf>(static_cast<int64_t>(p.U) * p.n * p.CG);
+    for (int i = 0; i < p.streams; ++i) b.S[i] = c.take<cf>(static_cast<int64_t>(p.U) * p.n * p.CG);
     b.Ht = c.take<float>(p.n);
     b.twN1 = c.take<cf>(p.N1);
     b.twN2 = c.take<cf>(p.N2);
@@ -583,6 +590,54 @@ BpBuffers bp_carve(void* ws, const BpPlan& p, int64_t* bytes) {
 }
 
 constexpr int64_t kScratchBudget = static_cast<int64_t>(MSAMD_BP_SCRATCH_MB) << 20;  // spectra kept in the last-level cache
+
+// Batches go round-robin over the caller's stream and library-owned side
+// streams (one spectra buffer each), so that one batch's memory-bound stages
+// A and C run beside another's arithmetic-bound stage B.  MSAMD_BP_STREAMS
+// (1..4) in the environment sets the number of streams.
+#ifndef MSAMD_BP_NSTREAMS
+#define MSAMD_BP_NSTREAMS 2
+#endif
+int bp_streams() {
+    static const int n = [] {
+        const char* e = std::getenv("MSAMD_BP_STREAMS");
+        const int v = e ? std::atoi(e) : MSAMD_BP_NSTREAMS;
+        return v < 1 ? 1 : (v > kMaxStreams ? kMaxStreams : v);
+    }();
+    return n;
+}
+
+// Per-device side streams and fork/join events, created on first use.  The
+// issue of one filter call's launches holds the device's lock, so the event
+// pairs of calls from several host threads do not interleave.
+struct SideStream {
+    hipStream_t stream[kMaxStreams - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxStreams - 1] = {};
+    std::mutex mu;
+};
+constexpr int kMaxDevices = 64;
+SideStream g_side[kMaxDevices];
+std::mutex g_side_init;
+
+int side_stream(SideStream** out) {
+    int dev = 0;
+    MSAMD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("bandpass: device %d beyond the supported %d", dev, kMaxDevices);
+        return MSAMD_ERR_UNSUPPORTED;
+    }
+    SideStream& s = g_side[dev];
+    std::lock_guard<std::mutex> lock(g_side_init);
+    if (!s.fork) {
+        for (int i = 0; i < kMaxStreams - 1; ++i) {
+            MSAMD_HIP(hipEventCreateWithFlags(&s.join[i], hipEventDisableTiming));
+            MSAMD_HIP(hipStreamCreateWithFlags(&s.stream[i], hipStreamNonBlocking));
+        }
+        MSAMD_HIP(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
+    }
+    *out = &s;
+    return MSAMD_OK;
+}
 
 }  // namespace
 }  // namespace msamd
@@ -654,7 +709,7 @@ extern "C" int64_t msamd_bandpass_workspace_size(int64_t M, int64_t N, int64_t c
                    ? bluestein_workspace_size(M, N, chunk_size, overlap)
                    : -1;
     const BpPlan p = make_plan(M, N > 0 ? N : 1, chunk_size, overlap, 1, 0.0, false, sh,
-                               kScratchBudget);
+                               kScratchBudget, bp_streams());
     int64_t bytes = 0;
     bp_carve(nullptr, p, &bytes);
     return bytes;
@@ -693,7 +748,7 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
                                 quantization_unit, Y, workspace, workspace_bytes, as_stream(stream));
     }
     const BpPlan p = make_plan(M, N, chunk_size, overlap, k_sub, quantization_unit,
-                               dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget);
+                               dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget, bp_streams());
     int64_t need = 0;
     BpBuffers b = bp_carve(workspace, p, &need);
     if (!workspace || workspace_bytes < need) {
@@ -740,8 +795,22 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&bp_stage_a_wide<20, 20>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(wide_a_lds_bytes<20, 20>())));
-    for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U) {
+    SideStream* side = nullptr;
+    std::unique_lock<std::mutex> side_lock;
+    if (p.streams > 1) {
+        const int rc = side_stream(&side);
+        if (rc != MSAMD_OK) return rc;
+        side_lock = std::unique_lock<std::mutex>(side->mu);
+        MSAMD_HIP(hipEventRecord(side->fork, st));
+        for (int i = 0; i + 1 < p.streams; ++i)
+            MSAMD_HIP(hipStreamWaitEvent(side->stream[i], side->fork, 0));
+    }
+    int64_t batch = 0;
+    for (int64_t u0 = 0; u0 < p.nUnits; u0 += p.U, ++batch) {
         const int64_t nu = (p.nUnits - u0) < p.U ? (p.nUnits - u0) : p.U;
+        const int si = static_cast<int>(batch % p.streams);
+        hipStream_t bs = si == 0 ? st : side->stream[si - 1];
+        cf* S = b.S[si];
         auto grid = [&](int pb_per_wg, int64_t cols_per_unit) {
             const int64_t cols = nu * cols_per_unit;
             const int64_t npb = p.CG / pb_per_wg;
@@ -753,35 +822,39 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         if (sh.R1 == 20 && sh.R2 == 20 && sh.Q1 == 24 && sh.Q2 == 25) {
             if (wide_a) {
                 bp_stage_a_wide<20, 20><<<grid(kWideP, p.N2), kWideP * 20,
-                                          wide_a_lds_bytes<20, 20>(), st>>>(
-                    static_cast<const int16_t*>(X), p, u0, nu, b.twN1, b.twHi, b.twLo, b.S);
+                                          wide_a_lds_bytes<20, 20>(), bs>>>(
+                    static_cast<const int16_t*>(X), p, u0, nu, b.twN1, b.twHi, b.twLo, S);
             } else if (p.in_i16)
-                bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, nu, b.twN1,
-                                                                             b.twHi, b.twLo, b.S);
+                bp_stage_a<20, 20, true, kPBA><<<gridA, kPBA * 20, 0, bs>>>(X, p, u0, nu, b.twN1,
+                                                                             b.twHi, b.twLo, S);
             else
-                bp_stage_a<20, 20, false, kPBA><<<gridA, kPBA * 20, 0, st>>>(X, p, u0, nu, b.twN1,
-                                                                              b.twHi, b.twLo, b.S);
+                bp_stage_a<20, 20, false, kPBA><<<gridA, kPBA * 20, 0, bs>>>(X, p, u0, nu, b.twN1,
+                                                                              b.twHi, b.twLo, S);
             MSAMD_LAUNCH_CHECK();
-            bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, st>>>(p, u0, nu, b.twN2, b.Ht, b.S);
+            bp_stage_b<24, 25, kPBB><<<gridB, kPBB * 25, 0, bs>>>(p, u0, nu, b.twN2, b.Ht, S);
             MSAMD_LAUNCH_CHECK();
             const bool s1 = p.k_sub == 1;
             if (p.out_i16 && s1)
-                bp_stage_c<20, 20, true, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
+                bp_stage_c<20, 20, true, true, kPBC><<<gridC, kPBC * 20, 0, bs>>>(
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, S, Y);
             else if (p.out_i16)
-                bp_stage_c<20, 20, true, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
+                bp_stage_c<20, 20, true, false, kPBC><<<gridC, kPBC * 20, 0, bs>>>(
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, S, Y);
             else if (s1)
-                bp_stage_c<20, 20, false, true, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
+                bp_stage_c<20, 20, false, true, kPBC><<<gridC, kPBC * 20, 0, bs>>>(
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, S, Y);
             else
-                bp_stage_c<20, 20, false, false, kPBC><<<gridC, kPBC * 20, 0, st>>>(
-                    p, u0, nu, b.twN1, b.twHi, b.twLo, b.S, Y);
+                bp_stage_c<20, 20, false, false, kPBC><<<gridC, kPBC * 20, 0, bs>>>(
+                    p, u0, nu, b.twN1, b.twHi, b.twLo, S, Y);
             MSAMD_LAUNCH_CHECK();
         } else {
             set_error("bandpass: shape not instantiated");
             return MSAMD_ERR_UNSUPPORTED;
         }
+    }
+    for (int i = 0; i + 1 < p.streams; ++i) {  // the caller's stream waits for the side streams
+        MSAMD_HIP(hipEventRecord(side->join[i], side->stream[i]));
+        MSAMD_HIP(hipStreamWaitEvent(st, side->join[i], 0));
     }
     return MSAMD_OK;
 }
