@@ -79,6 +79,26 @@ int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_t N, double
                           int64_t overlap, int subsample_factor, double quantization_unit,
                           void* Y, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Number of filter windows of an N-timepoint recording: ceil(N / chunk_size)
+ * (the window loop of p_bandpass_filter.cpp:179), -1 on bad arguments. */
+int64_t msamd_bandpass_windows(int64_t N, int64_t chunk_size);
+
+/* Windows [w0, w1) of msamd_bandpass_filter, for streaming a recording
+ * through the device a segment at a time (the reference reads each window
+ * from disk, p_bandpass_filter.cpp:179-214).  X holds recording rows
+ * [x_t0, x_t0 + x_rows) of the N-timepoint recording and must cover the
+ * windows' input [max(0, w0*cs - overlap), min(N, w1*cs + overlap)); Y row 0
+ * is output row y_t0 and Y must cover output rows [(w0*cs)/k, min(N/k,
+ * (w1*cs)/k)).  The output rows equal those of msamd_bandpass_filter on the
+ * whole recording bit for bit.  The workspace of
+ * msamd_bandpass_workspace_size(M, N, ...) suffices for any window range. */
+int msamd_bandpass_filter_windows(const void* X, int dtype, int64_t M, int64_t N, int64_t x_t0,
+                                  int64_t x_rows, int64_t w0, int64_t w1, double samplerate,
+                                  double freq_min, double freq_max, double freq_wid,
+                                  int64_t chunk_size, int64_t overlap, int subsample_factor,
+                                  double quantization_unit, void* Y, int64_t y_t0, int64_t y_rows,
+                                  void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Event detection (a10) -- replaces detect_events()
  * (packages/mountainsortalg/detect_events.cpp:69-135) as driven by
@@ -249,6 +269,18 @@ int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N);
  * XXt: M x M float64, symmetric.  M <= 1024. */
 int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, double* XXt, void* workspace,
                             int64_t workspace_bytes, void* stream);
+
+/* The covariance over time segments, for streaming a recording through the
+ * device (the reference sums per-chunk partials, p_whiten.cpp:75-111): the
+ * unnormalized sum of x x^T over the N rows of X (row_keep may be NULL, as
+ * in msamd_whiten_covariance_rows) is written to XXt_sum (accumulate == 0)
+ * or added to it (accumulate != 0).  msamd_whiten_covariance_finish then
+ * divides by N_total - 1 (when N_total > 1, p_whiten.cpp:114-118).  Same
+ * workspace as msamd_whiten_covariance for this N. */
+int msamd_whiten_covariance_accumulate(const float* X, int64_t M, int64_t N,
+                                       const uint32_t* row_keep, double* XXt_sum, int accumulate,
+                                       void* workspace, int64_t workspace_bytes, void* stream);
+int msamd_whiten_covariance_finish(double* XXt, int64_t M, int64_t N_total, void* stream);
 
 /* msamd_whiten_covariance of the masked timeseries, with the mask given as
  * msamd_mask_rows' row bitmap: rows whose bit is clear read as zero.  The

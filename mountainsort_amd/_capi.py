@@ -24,6 +24,10 @@ SIGNATURES = {
     "msamd_bandpass_workspace_size": (i64, [i64, i64, i64, i64]),
     "msamd_bandpass_filter": (cint, [vp, cint, i64, i64, f64, f64, f64, f64, i64, i64, cint, f64,
                                      vp, vp, i64, vp]),
+    "msamd_bandpass_windows": (i64, [i64, i64]),
+    "msamd_bandpass_filter_windows": (cint, [vp, cint, i64, i64, i64, i64, i64, i64, f64, f64,
+                                             f64, f64, i64, i64, cint, f64, vp, i64, i64, vp, i64,
+                                             vp]),
     "msamd_detect_chunk_size": (i64, [i64, i64, i64]),
     "msamd_detect_workspace_size": (i64, [i64, i64, i32, f64]),
     "msamd_detect_events": (cint, [vp, i64, i64, i64, i64, i64, i64, vp, i32, f64, f64, cint, vp,
@@ -48,6 +52,8 @@ SIGNATURES = {
     "msamd_whiten_covariance_workspace_size": (i64, [i64, i64]),
     "msamd_whiten_covariance": (cint, [vp, i64, i64, vp, vp, i64, vp]),
     "msamd_whiten_covariance_rows": (cint, [vp, i64, i64, vp, vp, vp, i64, vp]),
+    "msamd_whiten_covariance_accumulate": (cint, [vp, i64, i64, vp, vp, cint, vp, i64, vp]),
+    "msamd_whiten_covariance_finish": (cint, [vp, i64, i64, vp]),
     "msamd_whitening_matrix_workspace_size": (i64, [i64]),
     "msamd_whitening_matrix": (cint, [vp, i64, vp, vp, i64, vp]),
     "msamd_whitening_matrix_exact_workspace_size": (i64, [i64]),

@@ -19,6 +19,7 @@
 // registers (fft_radix.h) with one LDS exchange between them.  Windows are
 // processed in units of (channel group, window) whose scratch spectra stay
 // in the MI355X last-level cache between the three stages.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
@@ -94,6 +95,9 @@ struct BpPlan {
     int32_t U;       // units per batch
     int32_t nTwHi;
     int32_t streams; // batches go round-robin over this many streams, one spectra buffer each
+    int64_t w0;      // first window of this call (nWin windows from there)
+    int64_t xt0;     // recording time of input row 0
+    int64_t yt0;     // output row of Y row 0
 };
 
 // Factorizations with kernels compiled in: n = (R1*R2) * (Q1*Q2).
@@ -160,7 +164,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
         stp[tid] = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
     }
     const int g = static_cast<int>(u / p.nWin);
-    const int64_t w = u % p.nWin;
+    const int64_t w = p.w0 + u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;   // global pair index
     const int ch0 = 2 * pair;
     const int64_t tbase = w * p.cs - p.ov;       // time of window sample 0
@@ -174,7 +178,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
             float a = 0.f, c = 0.f;
             if (pair < p.P && t >= 0 && t < p.N) {
                 if (kI16) {
-                    const int16_t* X = static_cast<const int16_t*>(Xv) + t * p.M + ch0;
+                    const int16_t* X = static_cast<const int16_t*>(Xv) + (t - p.xt0) * p.M + ch0;
                     if (even_m) {  // both channels in one aligned 32-bit word
                         const uint32_t w2 = *reinterpret_cast<const uint32_t*>(X);
                         a = static_cast<float>(static_cast<int16_t>(w2 & 0xffffu));
@@ -184,7 +188,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_a(
                         if (ch0 + 1 < p.M) c = static_cast<float>(X[1]);
                     }
                 } else {
-                    const float* X = static_cast<const float*>(Xv) + t * p.M + ch0;
+                    const float* X = static_cast<const float*>(Xv) + (t - p.xt0) * p.M + ch0;
                     if (even_m) {
                         const float2 v2 = *reinterpret_cast<const float2*>(X);
                         a = v2.x;
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
         stp[tid] = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
     }
     const int g = static_cast<int>(u / p.nWin);
-    const int64_t w = u % p.nWin;
+    const int64_t w = p.w0 + u % p.nWin;
     const int64_t tbase = w * p.cs - p.ov;  // time of window sample 0
     const int64_t ch0 = 2 * (static_cast<int64_t>(g) * p.CG + pb * kWideP);
     uint32_t* stg = reinterpret_cast<uint32_t*>(wbuf);
@@ -285,7 +289,7 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
         const int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
         uint4 v = {0u, 0u, 0u, 0u};
         if (t >= 0 && t < p.N && ch0 + 8 * c < p.M)
-            v = *reinterpret_cast<const uint4*>(X + t * p.M + ch0 + 8 * c);
+            v = *reinterpret_cast<const uint4*>(X + (t - p.xt0) * p.M + ch0 + 8 * c);
         *reinterpret_cast<uint4*>(stg + n1 * kWideP + 4 * c) = v;
     }
     __syncthreads();
@@ -433,7 +437,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
     if (u >= nu) return;  // whole workgroup: past the last column of the batch
     u += unit0;
     const int g = static_cast<int>(u / p.nWin);
-    const int64_t w = u % p.nWin;
+    const int64_t w = p.w0 + u % p.nWin;
     const int pair = g * p.CG + pb * kPB + pl;
     const int ch0 = 2 * pair;
     const cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
@@ -499,9 +503,9 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
                         const float s1 = static_cast<float>(static_cast<double>(vals[1]) * p.qscale);
                         const uint32_t lo = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s0)));
                         const uint32_t hi = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s1)));
-                        *reinterpret_cast<uint32_t*>(static_cast<int16_t*>(Yv) + to * p.M + ch0) = lo | (hi << 16);
+                        *reinterpret_cast<uint32_t*>(static_cast<int16_t*>(Yv) + (to - p.yt0) * p.M + ch0) = lo | (hi << 16);
                     } else {
-                        *reinterpret_cast<float2*>(static_cast<float*>(Yv) + to * p.M + ch0) =
+                        *reinterpret_cast<float2*>(static_cast<float*>(Yv) + (to - p.yt0) * p.M + ch0) =
                             make_float2(vals[0], vals[1]);
                     }
                     continue;
@@ -514,9 +518,9 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
                         // multiply_by_factor (:207-209, float *= double), then the
                         // float -> int16 conversion of the writer (truncation)
                         const float s = static_cast<float>(static_cast<double>(vals[c]) * p.qscale);
-                        static_cast<int16_t*>(Yv)[to * p.M + ch] = static_cast<int16_t>(static_cast<int>(s));
+                        static_cast<int16_t*>(Yv)[(to - p.yt0) * p.M + ch] = static_cast<int16_t>(static_cast<int>(s));
                     } else {
-                        static_cast<float*>(Yv)[to * p.M + ch] = vals[c];
+                        static_cast<float*>(Yv)[(to - p.yt0) * p.M + ch] = vals[c];
                     }
                 }
             }
@@ -535,7 +539,8 @@ bool pick_shape(int64_t n, Shape* s) {
 }
 
 BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double qunit,
-                 bool in_i16, const Shape& sh, int64_t scratch_budget, int streams) {
+                 bool in_i16, const Shape& sh, int64_t scratch_budget, int streams,
+                 int64_t w0 = 0, int64_t nwin = -1) {
     BpPlan p{};
     p.M = M;
     p.N = N;
@@ -552,7 +557,8 @@ BpPlan make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double
     p.P = static_cast<int32_t>((M + 1) / 2);
     p.CG = static_cast<int32_t>(align_up(p.P < kCGMax ? p.P : kCGMax, kCGAlign));
     p.nGroups = static_cast<int32_t>(ceil_div(p.P, p.CG));
-    p.nWin = ceil_div(N, cs);
+    p.w0 = w0;
+    p.nWin = nwin >= 0 ? nwin : ceil_div(N, cs);
     p.nUnits = p.nWin * p.nGroups;
     const int64_t per_unit = p.n * p.CG * static_cast<int64_t>(sizeof(cf));
     int64_t U = scratch_budget / streams / per_unit;
@@ -646,10 +652,10 @@ namespace msamd {
 // Windows of any other length (bandpass_bluestein.hip).
 int64_t bluestein_workspace_size(int64_t M, int64_t N, int64_t cs, int64_t ov);
 bool bluestein_supported(int64_t n);
-int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, double samplerate,
-                     double freq_min, double freq_max, double freq_wid, int64_t cs, int64_t ov,
-                     int k_sub, double qunit, void* Y, void* workspace, int64_t workspace_bytes,
-                     hipStream_t st);
+int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, int64_t xt0, int64_t w0,
+                     int64_t w1, double samplerate, double freq_min, double freq_max,
+                     double freq_wid, int64_t cs, int64_t ov, int k_sub, double qunit, void* Y,
+                     int64_t yt0, void* workspace, int64_t workspace_bytes, hipStream_t st);
 }  // namespace msamd
 
 using namespace msamd;
@@ -715,12 +721,16 @@ extern "C" int64_t msamd_bandpass_workspace_size(int64_t M, int64_t N, int64_t c
     return bytes;
 }
 
-extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_t N,
-                                     double samplerate, double freq_min, double freq_max,
-                                     double freq_wid, int64_t chunk_size, int64_t overlap,
-                                     int subsample_factor, double quantization_unit, void* Y,
-                                     void* workspace, int64_t workspace_bytes, void* stream) {
-    clear_error();
+namespace {
+
+// Windows [w0, w1) of the filter.  X holds recording rows [xt0, xt0 + x_rows)
+// and must cover those windows' input (clipped to [0, N)); Y row 0 is output
+// row yt0.  The whole filter is w0 = 0, w1 = nWin, xt0 = yt0 = 0.
+int bandpass_run(const void* X, int dtype, int64_t M, int64_t N, int64_t xt0, int64_t x_rows,
+                 int64_t w0, int64_t w1, double samplerate, double freq_min, double freq_max,
+                 double freq_wid, int64_t chunk_size, int64_t overlap, int subsample_factor,
+                 double quantization_unit, void* Y, int64_t yt0, int64_t y_rows,
+                 void* workspace, int64_t workspace_bytes, void* stream) {
     MSAMD_REQUIRE(M >= 1 && N >= 0, "bandpass: bad shape");
     MSAMD_REQUIRE(dtype == MSAMD_DTYPE_FLOAT32 || dtype == MSAMD_DTYPE_INT16,
                   "bandpass: input dtype %d not supported (float32 or int16)", dtype);
@@ -728,11 +738,27 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
     MSAMD_REQUIRE(chunk_size >= 1 && overlap >= 0, "bandpass: bad chunk plan");
     MSAMD_REQUIRE(samplerate > 0, "bandpass: samplerate must be positive");
     const int k_sub = subsample_factor > 0 ? subsample_factor : 1;
-    if (k_sub > 1 && chunk_size % k_sub != 0 && N / k_sub > 0) {
+    const int64_t nwin_all = ceil_div(N, chunk_size);
+    MSAMD_REQUIRE(0 <= w0 && w0 <= w1 && w1 <= nwin_all,
+                  "bandpass: windows [%lld, %lld) outside the %lld windows of the recording",
+                  (long long)w0, (long long)w1, (long long)nwin_all);
+    {
+        const int64_t lo = std::max<int64_t>(0, w0 * chunk_size - overlap);
+        const int64_t hi = std::min<int64_t>(N, w1 * chunk_size + overlap);
+        MSAMD_REQUIRE(w0 == w1 || (xt0 <= lo && xt0 + x_rows >= hi),
+                      "bandpass: input rows [%lld, %lld) do not cover the windows' [%lld, %lld)",
+                      (long long)xt0, (long long)(xt0 + x_rows), (long long)lo, (long long)hi);
+        const int64_t ylo = (w0 * chunk_size) / k_sub;
+        const int64_t yhi = std::min<int64_t>(N / k_sub, (w1 * chunk_size) / k_sub);
+        MSAMD_REQUIRE(w0 == w1 || (yt0 <= ylo && yt0 + y_rows >= yhi),
+                      "bandpass: output rows [%lld, %lld) do not cover the windows' [%lld, %lld)",
+                      (long long)yt0, (long long)(yt0 + y_rows), (long long)ylo, (long long)yhi);
+    }
+    if (k_sub > 1 && chunk_size % k_sub != 0 && y_rows > 0) {
         // Window w writes its cs/k_sub outputs from (w*cs)/k_sub on, which
         // leaves a row between windows when k_sub does not divide cs; the
         // reference's output file is zero there (DiskWriteMda pre-zeroes).
-        const int64_t bytes = (N / k_sub) * M * (quantization_unit != 0 ? 2 : 4);
+        const int64_t bytes = y_rows * M * (quantization_unit != 0 ? 2 : 4);
         MSAMD_HIP(hipMemsetAsync(Y, 0, static_cast<size_t>(bytes), as_stream(stream)));
     }
     Shape sh;
@@ -743,12 +769,16 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
                       (long long)(chunk_size + 2 * overlap));
             return MSAMD_ERR_UNSUPPORTED;
         }
-        return bluestein_filter(X, dtype == MSAMD_DTYPE_INT16, M, N, samplerate, freq_min,
-                                freq_max, freq_wid, chunk_size, overlap, k_sub,
-                                quantization_unit, Y, workspace, workspace_bytes, as_stream(stream));
+        return bluestein_filter(X, dtype == MSAMD_DTYPE_INT16, M, N, xt0, w0, w1, samplerate,
+                                freq_min, freq_max, freq_wid, chunk_size, overlap, k_sub,
+                                quantization_unit, Y, yt0, workspace, workspace_bytes,
+                                as_stream(stream));
     }
-    const BpPlan p = make_plan(M, N, chunk_size, overlap, k_sub, quantization_unit,
-                               dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget, bp_streams());
+    BpPlan p = make_plan(M, N, chunk_size, overlap, k_sub, quantization_unit,
+                         dtype == MSAMD_DTYPE_INT16, sh, kScratchBudget, bp_streams(), w0,
+                         w1 - w0);
+    p.xt0 = xt0;
+    p.yt0 = yt0;
     int64_t need = 0;
     BpBuffers b = bp_carve(workspace, p, &need);
     if (!workspace || workspace_bytes < need) {
@@ -756,7 +786,7 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
                   (long long)need);
         return MSAMD_ERR_WORKSPACE;
     }
-    if (N == 0) return MSAMD_OK;
+    if (N == 0 || w0 == w1) return MSAMD_OK;
     hipStream_t st = as_stream(stream);
     // host tables: gain H/n transposed to [k1][k2], Stockham twiddles,
     // four-step twiddle factors (all computed in double, rounded once)
@@ -857,4 +887,38 @@ extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_
         MSAMD_HIP(hipStreamWaitEvent(st, side->join[i], 0));
     }
     return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_bandpass_filter(const void* X, int dtype, int64_t M, int64_t N,
+                                     double samplerate, double freq_min, double freq_max,
+                                     double freq_wid, int64_t chunk_size, int64_t overlap,
+                                     int subsample_factor, double quantization_unit, void* Y,
+                                     void* workspace, int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(chunk_size >= 1, "bandpass: bad chunk plan");
+    const int k_sub = subsample_factor > 0 ? subsample_factor : 1;
+    return bandpass_run(X, dtype, M, N, 0, N, 0, N > 0 ? ceil_div(N, chunk_size) : 0, samplerate,
+                        freq_min, freq_max, freq_wid, chunk_size, overlap, subsample_factor,
+                        quantization_unit, Y, 0, N / k_sub, workspace, workspace_bytes, stream);
+}
+
+extern "C" int64_t msamd_bandpass_windows(int64_t N, int64_t chunk_size) {
+    if (N < 0 || chunk_size < 1) return -1;
+    return ceil_div(N, chunk_size);
+}
+
+extern "C" int msamd_bandpass_filter_windows(const void* X, int dtype, int64_t M, int64_t N,
+                                             int64_t x_t0, int64_t x_rows, int64_t w0, int64_t w1,
+                                             double samplerate, double freq_min, double freq_max,
+                                             double freq_wid, int64_t chunk_size, int64_t overlap,
+                                             int subsample_factor, double quantization_unit,
+                                             void* Y, int64_t y_t0, int64_t y_rows,
+                                             void* workspace, int64_t workspace_bytes,
+                                             void* stream) {
+    clear_error();
+    return bandpass_run(X, dtype, M, N, x_t0, x_rows, w0, w1, samplerate, freq_min, freq_max,
+                        freq_wid, chunk_size, overlap, subsample_factor, quantization_unit, Y,
+                        y_t0, y_rows, workspace, workspace_bytes, stream);
 }

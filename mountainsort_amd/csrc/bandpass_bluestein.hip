@@ -51,6 +51,9 @@ struct BsPlan {
     int64_t nWin, nItems;
     int32_t U;    // window-pairs per batch
     int32_t CW;   // columns per workgroup in the column passes
+    int64_t w0;   // first window of this call (nWin windows from there)
+    int64_t xt0;  // recording time of input row 0
+    int64_t yt0;  // output row of Y row 0
 };
 
 __device__ __forceinline__ int brev(int x, int lg) {
@@ -121,7 +124,7 @@ __global__ void __launch_bounds__(kBsThreads) bs_col_fwd_kernel(
     const int c0 = (blockIdx.x % ngrp) * p.CW;
     cf* B = buf + (item - item0) * p.P;
     const int pair = static_cast<int>(item % p.pairs);
-    const int64_t w = item / p.pairs;
+    const int64_t w = p.w0 + item / p.pairs;
     const int64_t tbase = w * p.cs - p.ov;
     const int64_t ch0 = 2 * static_cast<int64_t>(pair);
     for (int q = threadIdx.x; q < p.CW * p.P1; q += blockDim.x) {
@@ -134,11 +137,11 @@ __global__ void __launch_bounds__(kBsThreads) bs_col_fwd_kernel(
                 float a = 0.f, c = 0.f;
                 if (t >= 0 && t < p.N) {
                     if (kI16) {
-                        const int16_t* X = static_cast<const int16_t*>(Xv) + t * p.M + ch0;
+                        const int16_t* X = static_cast<const int16_t*>(Xv) + (t - p.xt0) * p.M + ch0;
                         a = static_cast<float>(X[0]);
                         if (ch0 + 1 < p.M) c = static_cast<float>(X[1]);
                     } else {
-                        const float* X = static_cast<const float*>(Xv) + t * p.M + ch0;
+                        const float* X = static_cast<const float*>(Xv) + (t - p.xt0) * p.M + ch0;
                         a = X[0];
                         if (ch0 + 1 < p.M) c = X[1];
                     }
@@ -212,7 +215,7 @@ __global__ void __launch_bounds__(kBsThreads) bs_col_inv_kernel(
     lds_ifft_dif(sd, p.lg1, p.CW, tw1);
     const float invP = 1.0f / static_cast<float>(p.P);   // a power of two: exact
     const int pair = static_cast<int>(item % p.pairs);
-    const int64_t w = item / p.pairs;
+    const int64_t w = p.w0 + item / p.pairs;
     const int64_t ch0 = 2 * static_cast<int64_t>(pair);
     for (int q = threadIdx.x; q < p.CW * p.P1; q += blockDim.x) {
         const int col = q / p.P1, n1 = q % p.P1;
@@ -248,16 +251,16 @@ __global__ void __launch_bounds__(kBsThreads) bs_col_inv_kernel(
                 // multiply_by_factor (:207-209, float *= double), then the
                 // float -> int16 conversion of the writer (truncation)
                 const float s = static_cast<float>(static_cast<double>(vals[c]) * p.qscale);
-                static_cast<int16_t*>(Yv)[to * p.M + ch] = static_cast<int16_t>(static_cast<int>(s));
+                static_cast<int16_t*>(Yv)[(to - p.yt0) * p.M + ch] = static_cast<int16_t>(static_cast<int>(s));
             } else {
-                static_cast<float*>(Yv)[to * p.M + ch] = vals[c];
+                static_cast<float*>(Yv)[(to - p.yt0) * p.M + ch] = vals[c];
             }
         }
     }
 }
 
 bool bs_make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, double qunit,
-                  bool in_i16, BsPlan* out) {
+                  bool in_i16, BsPlan* out, int64_t w0 = 0, int64_t nwin = -1) {
     BsPlan p{};
     p.M = M;
     p.N = N;
@@ -282,7 +285,8 @@ bool bs_make_plan(int64_t M, int64_t N, int64_t cs, int64_t ov, int k_sub, doubl
     if (p.CW < 1) p.CW = 1;
     if (p.CW > p.P2) p.CW = p.P2;
     p.pairs = static_cast<int32_t>((M + 1) / 2);
-    p.nWin = ceil_div(N, cs);
+    p.w0 = w0;
+    p.nWin = nwin >= 0 ? nwin : ceil_div(N, cs);
     p.nItems = p.nWin * p.pairs;
     int64_t U = kBsBudget / (p.P * static_cast<int64_t>(sizeof(cf)));
     if (U < 1) U = 1;
@@ -327,12 +331,12 @@ int64_t bluestein_workspace_size(int64_t M, int64_t N, int64_t cs, int64_t ov) {
 
 bool bluestein_supported(int64_t n) { return n >= 1 && 2 * n - 1 <= (int64_t{1} << kBsMaxLog); }
 
-int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, double samplerate,
-                     double freq_min, double freq_max, double freq_wid, int64_t cs, int64_t ov,
-                     int k_sub, double qunit, void* Y, void* workspace, int64_t workspace_bytes,
-                     hipStream_t st) {
+int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, int64_t xt0, int64_t w0,
+                     int64_t w1, double samplerate, double freq_min, double freq_max,
+                     double freq_wid, int64_t cs, int64_t ov, int k_sub, double qunit, void* Y,
+                     int64_t yt0, void* workspace, int64_t workspace_bytes, hipStream_t st) {
     BsPlan p;
-    if (!bs_make_plan(M, N, cs, ov, k_sub, qunit, in_i16, &p)) {
+    if (!bs_make_plan(M, N, cs, ov, k_sub, qunit, in_i16, &p, w0, w1 - w0)) {
         set_error("bandpass: window length %lld is too long", (long long)(cs + 2 * ov));
         return MSAMD_ERR_UNSUPPORTED;
     }
@@ -343,7 +347,9 @@ int bluestein_filter(const void* X, bool in_i16, int64_t M, int64_t N, double sa
                   (long long)need);
         return MSAMD_ERR_WORKSPACE;
     }
-    if (N == 0) return MSAMD_OK;
+    p.xt0 = xt0;
+    p.yt0 = yt0;
+    if (N == 0 || p.nItems == 0) return MSAMD_OK;
     {
         auto lds_attr = [](const void* f, size_t bytes) {
             return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,

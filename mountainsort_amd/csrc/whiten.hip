@@ -295,9 +295,11 @@ __global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float*
     }
 }
 
-// FP64 sum of the block partials in split order, / (N-1), mirrored.
+// FP64 sum of the block partials in split order, / (N-1) when normalize,
+// added to C when accumulate, mirrored.
 __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict__ part,
-                                                         CovPlan p, double* __restrict__ C) {
+                                                         CovPlan p, double* __restrict__ C,
+                                                         int accumulate, int normalize) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= static_cast<int64_t>(p.nTiles) * 256) return;
     const int tile = static_cast<int>(q >> 8);
@@ -311,9 +313,17 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
     double s = 0.0;
     for (int64_t sp = 0; sp < p.nsplit; ++sp)
         s = s + static_cast<double>(part[(sp * p.nTiles + tile) * 256 + e]);
-    if (p.N > 1) s = s / static_cast<double>(p.N - 1);
+    if (normalize && p.N > 1) s = s / static_cast<double>(p.N - 1);
+    if (accumulate) s = C[ii * p.M + jj] + s;
     C[ii * p.M + jj] = s;
     C[jj * p.M + ii] = s;
+}
+
+// C /= (N-1) for N > 1 (the accumulated sums of msamd_whiten_covariance_accumulate).
+__global__ void __launch_bounds__(256) cov_finish_kernel(double* __restrict__ C, int64_t MM,
+                                                         int64_t N) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < MM && N > 1) C[q] = C[q] / static_cast<double>(N - 1);
 }
 
 // ------------------------------------------------------- whitening matrix
@@ -984,12 +994,13 @@ extern "C" int64_t msamd_whiten_covariance_workspace_size(int64_t M, int64_t N) 
 namespace {
 
 int covariance_impl(const float* X, int64_t M, int64_t N, const uint32_t* rowkeep, double* XXt,
-                    void* workspace, int64_t workspace_bytes, void* stream) {
+                    void* workspace, int64_t workspace_bytes, void* stream,
+                    bool accumulate = false, bool normalize = true) {
     MSAMD_REQUIRE(M >= 1 && M <= 1024, "whiten_covariance: M=%lld outside 1..1024", (long long)M);
     MSAMD_REQUIRE(N >= 0, "whiten_covariance: negative N");
     hipStream_t st = as_stream(stream);
     if (N == 0) {
-        MSAMD_HIP(hipMemsetAsync(XXt, 0, sizeof(double) * M * M, st));
+        if (!accumulate) MSAMD_HIP(hipMemsetAsync(XXt, 0, sizeof(double) * M * M, st));
         return MSAMD_OK;
     }
     CovPlan p = cov_plan(M, N);
@@ -1027,7 +1038,8 @@ int covariance_impl(const float* X, int64_t M, int64_t N, const uint32_t* rowkee
         MSAMD_LAUNCH_CHECK();
     }
     const int64_t nred = static_cast<int64_t>(p.nTiles) * 256;
-    cov_reduce_kernel<<<static_cast<unsigned>(ceil_div(nred, 256)), 256, 0, st>>>(part, p, XXt);
+    cov_reduce_kernel<<<static_cast<unsigned>(ceil_div(nred, 256)), 256, 0, st>>>(
+        part, p, XXt, accumulate ? 1 : 0, normalize ? 1 : 0);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }
@@ -1038,6 +1050,26 @@ extern "C" int msamd_whiten_covariance(const float* X, int64_t M, int64_t N, dou
                                        void* workspace, int64_t workspace_bytes, void* stream) {
     clear_error();
     return covariance_impl(X, M, N, nullptr, XXt, workspace, workspace_bytes, stream);
+}
+
+extern "C" int msamd_whiten_covariance_accumulate(const float* X, int64_t M, int64_t N,
+                                                  const uint32_t* row_keep, double* XXt_sum,
+                                                  int accumulate, void* workspace,
+                                                  int64_t workspace_bytes, void* stream) {
+    clear_error();
+    return covariance_impl(X, M, N, row_keep, XXt_sum, workspace, workspace_bytes, stream,
+                           accumulate != 0, false);
+}
+
+extern "C" int msamd_whiten_covariance_finish(double* XXt, int64_t M, int64_t N_total,
+                                              void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(M >= 1 && M <= 1024 && N_total >= 0, "whiten_covariance_finish: bad shape");
+    const int64_t MM = M * M;
+    cov_finish_kernel<<<static_cast<unsigned>(ceil_div(MM, 256)), 256, 0, as_stream(stream)>>>(
+        XXt, MM, N_total);
+    MSAMD_LAUNCH_CHECK();
+    return MSAMD_OK;
 }
 
 extern "C" int msamd_whiten_covariance_rows(const float* X, int64_t M, int64_t N,

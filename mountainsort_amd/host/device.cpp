@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace msamd_host {
@@ -182,6 +183,206 @@ bool to_host(void* host, const void* dev, int64_t bytes, hipStream_t s, std::str
     return hip_ok(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync",
                   err) &&
            hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize", err);
+}
+
+
+RowReader::~RowReader() {
+    for (FILE* f : f_)
+        if (f) fclose(f);
+}
+
+bool RowReader::read(int32_t target, int64_t t0, int64_t t1, void* dst, std::string& err) {
+    const int tb = mda_bytes_per_entry(target);
+    const int64_t M = src_.M;
+    if (target == MDA_INT16 && src_.common_type != MDA_INT16) {
+        err = "internal: int16 read of a non-int16 timeseries";
+        return false;
+    }
+    if (f_.empty()) f_.assign(src_.files.size(), nullptr);
+    char* out = static_cast<char*>(dst);
+    // rows before 0 and at or past N are zero
+    if (t0 < 0) {
+        const int64_t z = std::min<int64_t>(t1, 0) - t0;
+        std::memset(out, 0, static_cast<size_t>(z * M * tb));
+        out += z * M * tb;
+        t0 += z;
+    }
+    int64_t file_t0 = 0;
+    for (size_t fi = 0; fi < src_.files.size() && t0 < t1; ++fi) {
+        const MdaHeader& h = src_.headers[fi];
+        const int64_t n = h.dim(1);
+        const int64_t lo = std::max(t0, file_t0), hi = std::min(t1, file_t0 + n);
+        if (hi > lo) {
+            if (!f_[fi]) {
+                f_[fi] = fopen(src_.files[fi].c_str(), "rb");
+                if (!f_[fi]) {
+                    err = "unable to open " + src_.files[fi];
+                    return false;
+                }
+            }
+            FILE* f = f_[fi];
+            const int sb = h.bytes_per_entry;
+            if (fseeko(f, h.header_size + (lo - file_t0) * M * sb, SEEK_SET) != 0) {
+                err = "seek failed in " + src_.files[fi];
+                return false;
+            }
+            const int64_t cnt = (hi - lo) * M;
+            size_t got;
+            if (h.type == target) {
+                got = fread(out, sb, static_cast<size_t>(cnt), f);
+            } else {
+                raw_.resize(static_cast<size_t>(cnt * sb));
+                got = fread(raw_.data(), sb, static_cast<size_t>(cnt), f);
+                mda_convert_to_f32(h.type, raw_.data(), reinterpret_cast<float*>(out), cnt);
+            }
+            if (static_cast<int64_t>(got) != cnt) {
+                err = "truncated data in " + src_.files[fi];
+                return false;
+            }
+            out += cnt * tb;
+            t0 = hi;
+        }
+        file_t0 += n;
+    }
+    if (t1 > t0) std::memset(out, 0, static_cast<size_t>((t1 - t0) * M * tb));
+    return true;
+}
+
+SegmentPipeline::~SegmentPipeline() {
+    for (int b = 0; b < 2; ++b) {
+        if (host_in_[b]) (void)hipHostFree(host_in_[b]);
+        if (host_out_[b]) (void)hipHostFree(host_out_[b]);
+        if (up_[b]) (void)hipEventDestroy(up_[b]);
+        if (comp_[b]) (void)hipEventDestroy(comp_[b]);
+        if (down_[b]) (void)hipEventDestroy(down_[b]);
+    }
+    if (su_) (void)hipStreamDestroy(su_);
+    if (sc_) (void)hipStreamDestroy(sc_);
+    if (sd_) (void)hipStreamDestroy(sd_);
+}
+
+bool SegmentPipeline::init(int64_t max_in, int64_t max_out, bool dev_in_buffers,
+                           std::string& err) {
+    max_in_ = std::max<int64_t>(max_in, 256);
+    max_out_ = std::max<int64_t>(max_out, 256);
+    for (int b = 0; b < 2; ++b) {
+        if (!hip_ok(hipHostMalloc(&host_in_[b], static_cast<size_t>(max_in_), hipHostMallocDefault),
+                    "hipHostMalloc", err) ||
+            !hip_ok(hipHostMalloc(&host_out_[b], static_cast<size_t>(max_out_), hipHostMallocDefault),
+                    "hipHostMalloc", err) ||
+            (dev_in_buffers && !dev_in_[b].alloc(max_in_, err)) || !dev_out_[b].alloc(max_out_, err) ||
+            !hip_ok(hipEventCreateWithFlags(&up_[b], hipEventDisableTiming), "hipEventCreate", err) ||
+            !hip_ok(hipEventCreateWithFlags(&comp_[b], hipEventDisableTiming), "hipEventCreate", err) ||
+            !hip_ok(hipEventCreateWithFlags(&down_[b], hipEventDisableTiming), "hipEventCreate", err))
+            return false;
+    }
+    return hip_ok(hipStreamCreateWithFlags(&su_, hipStreamNonBlocking), "hipStreamCreate", err) &&
+           hip_ok(hipStreamCreateWithFlags(&sc_, hipStreamNonBlocking), "hipStreamCreate", err) &&
+           hip_ok(hipStreamCreateWithFlags(&sd_, hipStreamNonBlocking), "hipStreamCreate", err);
+}
+
+bool SegmentPipeline::run(int64_t K, const Stage& st, std::string& err) {
+    const void* dev_src[2] = {nullptr, nullptr};
+    int64_t out_n[2] = {0, 0};
+    bool used_up[2] = {false, false}, used_comp[2] = {false, false}, used_down[2] = {false, false};
+    auto drain = [&]() {
+        (void)hipStreamSynchronize(su_);
+        (void)hipStreamSynchronize(sc_);
+        (void)hipStreamSynchronize(sd_);
+    };
+    // iteration k: read + upload segment k, compute + download segment k-1,
+    // write segment k-2
+    for (int64_t k = 0; k < K + 2; ++k) {
+        if (k < K) {
+            const int b = static_cast<int>(k & 1);
+            // the pinned buffer's previous upload (segment k-2) is complete
+            if (used_up[b] && !hip_ok(hipEventSynchronize(up_[b]), "hipEventSynchronize", err)) {
+                drain();
+                return false;
+            }
+            int64_t in_bytes = 0;
+            void* dst = nullptr;
+            if (!st.prepare(st.ctx, k, host_in_[b], &in_bytes, &dst, err)) {
+                drain();
+                return false;
+            }
+            if (in_bytes > max_in_) {
+                drain();
+                err = "internal: segment input larger than its buffer";
+                return false;
+            }
+            dev_src[b] = nullptr;
+            if (in_bytes > 0) {
+                if (!dst) dst = dev_in_[b].get();
+                // the device input buffer's previous reader (segment k-2) is done
+                if ((used_comp[b] && !hip_ok(hipStreamWaitEvent(su_, comp_[b], 0), "hipStreamWaitEvent", err)) ||
+                    !hip_ok(hipMemcpyAsync(dst, host_in_[b], static_cast<size_t>(in_bytes),
+                                           hipMemcpyHostToDevice, su_),
+                            "hipMemcpyAsync", err) ||
+                    !hip_ok(hipEventRecord(up_[b], su_), "hipEventRecord", err)) {
+                    drain();
+                    return false;
+                }
+                used_up[b] = true;
+                dev_src[b] = dst;
+            }
+        }
+        if (k >= 1 && k - 1 < K) {
+            const int64_t j = k - 1;
+            const int b = static_cast<int>(j & 1);
+            int64_t ob = 0;
+            if ((dev_src[b] && !hip_ok(hipStreamWaitEvent(sc_, up_[b], 0), "hipStreamWaitEvent", err)) ||
+                (used_down[b] && !hip_ok(hipStreamWaitEvent(sc_, down_[b], 0), "hipStreamWaitEvent", err)) ||
+                !st.compute(st.ctx, j, dev_src[b], dev_out_[b].get(), &ob, sc_, err) ||
+                !hip_ok(hipEventRecord(comp_[b], sc_), "hipEventRecord", err)) {
+                drain();
+                return false;
+            }
+            used_comp[b] = true;
+            if (ob > max_out_) {
+                drain();
+                err = "internal: segment output larger than its buffer";
+                return false;
+            }
+            out_n[b] = ob;
+            if (ob > 0) {
+                if (!hip_ok(hipStreamWaitEvent(sd_, comp_[b], 0), "hipStreamWaitEvent", err) ||
+                    !hip_ok(hipMemcpyAsync(host_out_[b], dev_out_[b].get(), static_cast<size_t>(ob),
+                                           hipMemcpyDeviceToHost, sd_),
+                            "hipMemcpyAsync", err) ||
+                    !hip_ok(hipEventRecord(down_[b], sd_), "hipEventRecord", err)) {
+                    drain();
+                    return false;
+                }
+                used_down[b] = true;
+            }
+        }
+        if (k >= 2) {
+            const int64_t j = k - 2;
+            const int b = static_cast<int>(j & 1);
+            const hipEvent_t done = out_n[b] > 0 ? down_[b] : comp_[b];
+            if (!hip_ok(hipEventSynchronize(done), "hipEventSynchronize", err) ||
+                (st.consume && !st.consume(st.ctx, j, host_out_[b], out_n[b], err))) {
+                drain();
+                return false;
+            }
+        }
+    }
+    return hip_ok(hipStreamSynchronize(sc_), "hipStreamSynchronize", err);
+}
+
+int64_t segment_bytes(int64_t dflt) {
+    if (const char* e = getenv("MSAMD_SEGMENT_BYTES")) {
+        const long long v = atoll(e);
+        if (v > 0) return v;
+    }
+    return dflt;
+}
+
+int64_t device_free_bytes() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return static_cast<int64_t>(fr);
 }
 
 }  // namespace msamd_host

@@ -160,43 +160,277 @@ int clip_tmid(int64_t T) { return static_cast<int>((T + 1) / 2) - 1; }
 
 // ---------------------------------------------------------------- processors
 
+// Segments of about this many input bytes stream through the device
+// (MSAMD_SEGMENT_BYTES overrides it).
+constexpr int64_t kSegmentBytes = 1ll << 30;
+
+// Pinned staging bytes of a segment pipeline (two input and two output buffers).
+double pipeline_ram_mb(int64_t in_bytes, int64_t out_bytes) {
+    return 2.0 * static_cast<double>(in_bytes + out_bytes) / (1024.0 * 1024.0);
+}
+
+// ms3.bandpass_filter, streamed: segments of whole windows of the reference's
+// chunk plan (p_bandpass_filter.cpp:147-155,179-214); segment k reads the
+// input rows of its windows (with the 60000-sample halos, zero outside the
+// recording), filters them on the device and writes its output rows, while
+// the next segment is read and the previous one written.
+struct BandpassStream {
+    RowReader* rd;
+    MdaWriter* wr;
+    int64_t M, N, cs, ov, nwin, S, k;
+    int32_t in_type, out_type;
+    double fs, fmin, fmax, fwid, q;
+    void* ws;
+    int64_t wsb;
+    int64_t w0(int64_t s) const { return s * S; }
+    int64_t w1(int64_t s) const { return std::min(nwin, (s + 1) * S); }
+    int64_t xlo(int64_t s) const { return std::max<int64_t>(0, w0(s) * cs - ov); }
+    int64_t xhi(int64_t s) const { return std::min<int64_t>(N, w1(s) * cs + ov); }
+    int64_t ylo(int64_t s) const { return (w0(s) * cs) / k; }
+    int64_t yhi(int64_t s) const { return std::min<int64_t>(N / k, (w1(s) * cs) / k); }
+
+    static bool prepare(void* ctx, int64_t s, void* host, int64_t* nb, void**, std::string& err) {
+        auto* b = static_cast<BandpassStream*>(ctx);
+        *nb = (b->xhi(s) - b->xlo(s)) * b->M * mda_bytes_per_entry(b->in_type);
+        return b->rd->read(b->in_type, b->xlo(s), b->xhi(s), host, err);
+    }
+    static bool compute(void* ctx, int64_t s, const void* din, void* dout, int64_t* ob,
+                        hipStream_t st, std::string& err) {
+        auto* b = static_cast<BandpassStream*>(ctx);
+        const int rc = msamd_bandpass_filter_windows(
+            din, b->in_type == MDA_INT16 ? MSAMD_DTYPE_INT16 : MSAMD_DTYPE_FLOAT32, b->M, b->N,
+            b->xlo(s), b->xhi(s) - b->xlo(s), b->w0(s), b->w1(s), b->fs, b->fmin, b->fmax, b->fwid,
+            b->cs, b->ov, static_cast<int>(b->k), b->q, dout, b->ylo(s), b->yhi(s) - b->ylo(s),
+            b->ws, b->wsb, st);
+        if (rc != MSAMD_OK) {
+            const char* m = msamd_last_error();
+            err = std::string("bandpass_filter: ") + (m ? m : "error");
+            return false;
+        }
+        *ob = (b->yhi(s) - b->ylo(s)) * b->M * mda_bytes_per_entry(b->out_type);
+        return true;
+    }
+    static bool consume(void* ctx, int64_t, const void* host, int64_t nb, std::string& err) {
+        return static_cast<BandpassStream*>(ctx)->wr->write(host, nb, err);
+    }
+};
+
 bool run_bandpass(Ctx& c) {
     const double fs = c.num("samplerate"), fmin = c.num("freq_min"), fmax = c.num("freq_max");
     const double fwid = c.num("freq_wid"), q = c.num("quantization_unit");
     long long k = c.integer("subsample_factor");
     if (k == 0) k = 1;  // p_bandpass_filter.cpp:121-122
     if (k < 0) return c.fail("subsample_factor must be positive");
-    if (c.requirements_only) {
-        c.peak_ram_mb = staging_bytes() / (1024.0 * 1024.0);
-        return true;
+    if (fmax == 0) {
+        if (c.requirements_only) return true;
+        return copy_file(c, c.str("timeseries"), c.str("timeseries_out"));  // :117-119
     }
-    if (fmax == 0) return copy_file(c, c.str("timeseries"), c.str("timeseries_out"));  // :117-119
     TimeseriesSource src;
     if (!open_timeseries(c, "timeseries", src)) return false;
-    const int64_t M = src.M, N = src.N, N2 = N / k;
-    const int64_t chunk = msamd_bandpass_chunk_size(M, reference_threads(c));
-    if (!msamd_bandpass_supported(chunk, kBandpassOverlap))
-        return c.fail("bandpass window length " + std::to_string(chunk + 2 * kBandpassOverlap) +
-                      " (chunk plan for M=" + std::to_string(M) + ", " +
+    BandpassStream b{};
+    b.M = src.M;
+    b.N = src.N;
+    b.k = k;
+    b.cs = msamd_bandpass_chunk_size(b.M, reference_threads(c));
+    b.ov = kBandpassOverlap;
+    if (!msamd_bandpass_supported(b.cs, b.ov))
+        return c.fail("bandpass window length " + std::to_string(b.cs + 2 * b.ov) +
+                      " (chunk plan for M=" + std::to_string(b.M) + ", " +
                       std::to_string(reference_threads(c)) + " threads) is not supported");
-    const bool in16 = src.common_type == MDA_INT16;
-    const int32_t out_type = q != 0 ? MDA_INT16 : MDA_FLOAT32;
+    b.in_type = src.common_type == MDA_INT16 ? MDA_INT16 : MDA_FLOAT32;
+    b.out_type = q != 0 ? MDA_INT16 : MDA_FLOAT32;
+    b.nwin = msamd_bandpass_windows(b.N, b.cs);
+    const int64_t in_row = b.M * mda_bytes_per_entry(b.in_type);
+    b.S = std::max<int64_t>(1, segment_bytes(kSegmentBytes) / (b.cs * in_row));
+    b.S = std::min<int64_t>(b.S, std::max<int64_t>(1, b.nwin));
+    const int64_t max_in = (b.S * b.cs + 2 * b.ov) * in_row;
+    const int64_t max_out = (b.S * b.cs / k + 1) * b.M * mda_bytes_per_entry(b.out_type);
+    if (c.requirements_only) {
+        c.peak_ram_mb = pipeline_ram_mb(max_in, max_out);
+        return true;
+    }
+    b.fs = fs;
+    b.fmin = fmin;
+    b.fmax = fmax;
+    b.fwid = fwid;
+    b.q = q;
+    DeviceBuffer ws;
+    b.wsb = msamd_bandpass_workspace_size(b.M, std::max<int64_t>(b.N, 1), b.cs, b.ov);
+    if (b.wsb < 0) return c.fail("bandpass: unsupported window length");
+    RowReader rd(src);
+    MdaWriter wr;
+    SegmentPipeline pipe;
+    if (!ws.alloc(b.wsb, c.err) || !pipe.init(max_in, max_out, true, c.err) ||
+        !wr.open(c.str("timeseries_out"), b.out_type, {b.M, b.N / k}, c.err))
+        return false;
+    b.ws = ws.get();
+    b.rd = &rd;
+    b.wr = &wr;
+    const int64_t K = (b.nwin + b.S - 1) / b.S;
+    SegmentPipeline::Stage st{&BandpassStream::prepare, &BandpassStream::compute,
+                              &BandpassStream::consume, &b};
+    if (!pipe.run(K, st, c.err)) {
+        wr.abort();
+        return false;
+    }
+    return wr.close(c.err);
+}
+
+// ms3.whiten and ms3.apply_whitening_matrix, streamed over row segments
+// (p_whiten.cpp:64-206 reads the recording in 1e5-sample chunks, twice).
+// Pass 1 accumulates the covariance segment by segment; pass 2 applies W to
+// each segment and writes it.  When the float32 recording fits in device
+// memory it is kept there after pass 1 and read from disk once; otherwise
+// pass 2 reads it again.  The covariance sums are the same either way.
+struct WhitenStream {
+    RowReader* rd;
+    MdaWriter* wr;
+    int64_t M, N, R;
+    float* resident;  // the device copy of the recording, or nullptr
+    double* C;        // device M x M: covariance sums, then the covariance
+    const double* W;  // device M x M (column-major), pass 2
+    bool with_cov;    // pass 2 bounds |x| by the covariance diagonal
+    double q;
+    int32_t out_type;
+    void* ws;
+    int64_t wsb;
+    int64_t r0(int64_t s) const { return s * R; }
+    int64_t r1(int64_t s) const { return std::min(N, (s + 1) * R); }
+
+    static bool lib(int rc, const char* what, std::string& err) {
+        if (rc == MSAMD_OK) return true;
+        const char* m = msamd_last_error();
+        err = std::string(what) + ": " + (m ? m : "error");
+        return false;
+    }
+    static bool read(void* ctx, int64_t s, void* host, int64_t* nb, void** dst, std::string& err) {
+        auto* w = static_cast<WhitenStream*>(ctx);
+        *nb = (w->r1(s) - w->r0(s)) * w->M * 4;
+        if (w->resident) *dst = w->resident + w->r0(s) * w->M;
+        return w->rd->read(MDA_FLOAT32, w->r0(s), w->r1(s), host, err);
+    }
+    static bool reread(void* ctx, int64_t s, void* host, int64_t* nb, void** dst, std::string& err) {
+        auto* w = static_cast<WhitenStream*>(ctx);
+        if (w->resident) {
+            *nb = 0;
+            return true;
+        }
+        return read(ctx, s, host, nb, dst, err);
+    }
+    static bool covariance(void* ctx, int64_t s, const void* din, void*, int64_t* ob,
+                           hipStream_t st, std::string& err) {
+        auto* w = static_cast<WhitenStream*>(ctx);
+        *ob = 0;
+        return lib(msamd_whiten_covariance_accumulate(static_cast<const float*>(din), w->M,
+                                                      w->r1(s) - w->r0(s), nullptr, w->C,
+                                                      s > 0 ? 1 : 0, w->ws, w->wsb, st),
+                   "whiten_covariance", err);
+    }
+    static bool apply(void* ctx, int64_t s, const void* din, void* dout, int64_t* ob,
+                      hipStream_t st, std::string& err) {
+        auto* w = static_cast<WhitenStream*>(ctx);
+        const float* X = w->resident ? w->resident + w->r0(s) * w->M : static_cast<const float*>(din);
+        const int64_t n = w->r1(s) - w->r0(s);
+        float* Yf = w->q > 0 ? nullptr : static_cast<float*>(dout);
+        int16_t* Yq = w->q > 0 ? static_cast<int16_t*>(dout) : nullptr;
+        const int rc = w->with_cov
+                           ? msamd_whiten_apply_cov(X, w->M, n, w->W, w->C, nullptr, Yf, w->q, Yq,
+                                                    w->ws, w->wsb, st)
+                           : msamd_whiten_apply(X, w->M, n, w->W, Yf, w->q, Yq, w->ws, w->wsb, st);
+        *ob = n * w->M * mda_bytes_per_entry(w->out_type);
+        return lib(rc, "whiten_apply", err);
+    }
+    static bool write(void* ctx, int64_t, const void* host, int64_t nb, std::string& err) {
+        return static_cast<WhitenStream*>(ctx)->wr->write(host, nb, err);
+    }
+};
+
+// W given (apply_whitening_matrix, device column-major) or computed
+// (whiten).  Returns false with c.err on failure.
+bool whiten_streamed(Ctx& c, const TimeseriesSource& src, const std::vector<double>* Wgiven) {
+    WhitenStream w{};
+    w.M = src.M;
+    w.N = src.N;
+    w.q = c.num("quantization_unit");
+    w.out_type = w.q > 0 ? MDA_INT16 : MDA_FLOAT32;
+    const int64_t M = w.M;
+    w.R = std::max<int64_t>(1, segment_bytes(kSegmentBytes) / (M * 4));
+    w.R = std::min<int64_t>(w.R, std::max<int64_t>(1, w.N));
+    const int64_t max_in = w.R * M * 4;
+    const int64_t max_out = w.R * M * mda_bytes_per_entry(w.out_type);
+    if (c.requirements_only) {
+        c.peak_ram_mb = pipeline_ram_mb(max_in, max_out);
+        return true;
+    }
+    const int64_t K = (w.N + w.R - 1) / w.R;
+    DeviceBuffer Wd, Cd, ws, res;
+    // the covariance workspace of a full segment and of the last, shorter one
+    const int64_t last = w.N - (K > 0 ? K - 1 : 0) * w.R;
+    const int64_t wsb = std::max({msamd_whiten_covariance_workspace_size(M, w.R),
+                                  msamd_whiten_covariance_workspace_size(M, std::max<int64_t>(last, 1)),
+                                  msamd_whiten_apply_workspace_size(M)});
+    if (!Wd.alloc(M * M * 8, c.err) || !Cd.alloc(M * M * 8, c.err) || !ws.alloc(wsb, c.err))
+        return false;
+    w.ws = ws.get();
+    w.wsb = wsb;
+    w.C = Cd.as<double>();
+    w.W = Wd.as<double>();
+    // keep the recording on the device when it fits with room to spare
+    const char* e = getenv("MSAMD_STREAM_RESIDENT");
+    const bool allow_resident = !(e && std::string(e) == "0");
+    const int64_t need = M * std::max<int64_t>(w.N, 1) * 4;
+    const bool resident = Wgiven == nullptr && allow_resident &&
+                          need + 4 * (max_in + max_out) < device_free_bytes() * 4 / 5;
+    if (resident) {
+        if (!res.alloc(need, c.err)) return false;
+        w.resident = res.as<float>();
+    }
+    RowReader rd(src);
+    MdaWriter wr;
+    w.rd = &rd;
+    w.wr = &wr;
+    SegmentPipeline pipe;
+    if (!pipe.init(max_in, max_out, !resident, c.err)) return false;
     Stream s;
-    DeviceBuffer X, Y, ws;
-    if (!s.create(c.err) || !X.alloc(M * N * (in16 ? 2 : 4), c.err) ||
-        !upload_timeseries(src, in16 ? MDA_INT16 : MDA_FLOAT32, X.get(), s.get(), c.err))
+    if (!s.create(c.err)) return false;
+    if (Wgiven) {
+        if (!to_device(Wd.get(), Wgiven->data(), M * M * 8, s.get(), c.err)) return false;
+        w.with_cov = false;
+    } else {
+        // pass 1: covariance sums, then / (N-1) and the whitening matrix
+        SegmentPipeline::Stage p1{&WhitenStream::read, &WhitenStream::covariance, nullptr, &w};
+        if (w.N == 0) {
+            if (!lib_ok(c, msamd_whiten_covariance_accumulate(nullptr, M, 0, nullptr, w.C, 0, w.ws,
+                                                              w.wsb, s.get()),
+                        "whiten_covariance"))
+                return false;
+        } else if (!pipe.run(K, p1, c.err)) {
+            return false;
+        }
+        DeviceBuffer ws2;
+        if (!ws2.alloc(msamd_whitening_matrix_workspace_size(M), c.err) ||
+            !lib_ok(c, msamd_whiten_covariance_finish(w.C, M, w.N, s.get()), "whiten_covariance") ||
+            !lib_ok(c, msamd_whitening_matrix(w.C, M, Wd.as<double>(), ws2.get(), ws2.bytes(),
+                                              s.get()),
+                    "whitening_matrix") ||
+            !s.sync(c.err))
+            return false;
+        w.with_cov = true;
+    }
+    // pass 2: apply and write
+    if (!wr.open(c.str("timeseries_out"), w.out_type, {M, w.N}, c.err)) return false;
+    SegmentPipeline::Stage p2{&WhitenStream::reread, &WhitenStream::apply, &WhitenStream::write, &w};
+    if (w.N > 0 && !pipe.run(K, p2, c.err)) {
+        wr.abort();
         return false;
-    const int64_t wsb = msamd_bandpass_workspace_size(M, N, chunk, kBandpassOverlap);
-    if (!Y.alloc(M * N2 * mda_bytes_per_entry(out_type), c.err) || !ws.alloc(wsb, c.err))
-        return false;
-    if (!lib_ok(c,
-                msamd_bandpass_filter(X.get(), in16 ? MSAMD_DTYPE_INT16 : MSAMD_DTYPE_FLOAT32, M, N,
-                                      fs, fmin, fmax, fwid, chunk, kBandpassOverlap,
-                                      static_cast<int>(k), q, Y.get(), ws.get(), ws.bytes(),
-                                      s.get()),
-                "bandpass_filter"))
-        return false;
-    return write_device(c, c.str("timeseries_out"), out_type, {M, N2}, Y.get(), s.get());
+    }
+    return wr.close(c.err);
+}
+
+bool run_whiten(Ctx& c) {
+    TimeseriesSource src;
+    if (!open_timeseries(c, "timeseries", src)) return false;
+    return whiten_streamed(c, src, nullptr);
 }
 
 // Covariance and whitening matrix of a device timeseries (M x N float32) into
@@ -214,40 +448,6 @@ bool whitening_matrix_of(Ctx& c, const float* X, int64_t M, int64_t N, DeviceBuf
            lib_ok(c, msamd_whitening_matrix(C.as<double>(), M, Wd.as<double>(), ws2.get(),
                                             ws2.bytes(), s),
                   "whitening_matrix");
-}
-
-// Applies W (device, column-major M x M) and writes the result.  C (nullable):
-// the covariance of X, which bounds its range for the apply's scaling.
-bool apply_and_write(Ctx& c, const float* X, int64_t M, int64_t N, const double* W, double q,
-                     hipStream_t s, const double* C = nullptr) {
-    DeviceBuffer Y, ws;
-    const int32_t out_type = q > 0 ? MDA_INT16 : MDA_FLOAT32;
-    if (!Y.alloc(M * N * mda_bytes_per_entry(out_type), c.err) ||
-        !ws.alloc(msamd_whiten_apply_workspace_size(M), c.err))
-        return false;
-    float* Yf = q > 0 ? nullptr : Y.as<float>();
-    int16_t* Yq = q > 0 ? Y.as<int16_t>() : nullptr;
-    const int st = C ? msamd_whiten_apply_cov(X, M, N, W, C, nullptr, Yf, q, Yq, ws.get(),
-                                              ws.bytes(), s)
-                     : msamd_whiten_apply(X, M, N, W, Yf, q, Yq, ws.get(), ws.bytes(), s);
-    if (!lib_ok(c, st, "whiten_apply")) return false;
-    return write_device(c, c.str("timeseries_out"), out_type, {M, N}, Y.get(), s);
-}
-
-bool run_whiten(Ctx& c) {
-    if (c.requirements_only) {
-        c.peak_ram_mb = staging_bytes() / (1024.0 * 1024.0);
-        return true;
-    }
-    TimeseriesSource src;
-    Stream s;
-    DeviceBuffer X, W;
-    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err) || !load_f32(c, src, X, s.get()))
-        return false;
-    DeviceBuffer C;
-    return whitening_matrix_of(c, X.as<float>(), src.M, src.N, W, s.get(), &C) &&
-           apply_and_write(c, X.as<float>(), src.M, src.N, W.as<double>(),
-                           c.num("quantization_unit"), s.get(), C.as<double>());
 }
 
 bool run_compute_whitening_matrix(Ctx& c) {
@@ -288,20 +488,14 @@ bool run_compute_whitening_matrix(Ctx& c) {
 
 bool run_apply_whitening_matrix(Ctx& c) {
     TimeseriesSource src;
-    Stream s;
-    DeviceBuffer X, W;
-    if (!open_timeseries(c, "timeseries", src) || !s.create(c.err)) return false;
+    if (!open_timeseries(c, "timeseries", src)) return false;
     std::vector<double> w;
     MdaHeader h;
     if (!mda_read_f64(c.str("whitening_matrix"), w, h, c.err)) return false;
     if (h.dim(0) != src.M || h.size() != src.M * src.M)
         return c.fail("whitening matrix is not " + std::to_string(src.M) + " x " +
                       std::to_string(src.M));
-    if (!load_f32(c, src, X, s.get()) || !W.alloc(src.M * src.M * 8, c.err) ||
-        !to_device(W.get(), w.data(), src.M * src.M * 8, s.get(), c.err))
-        return false;
-    return apply_and_write(c, X.as<float>(), src.M, src.N, W.as<double>(),
-                           c.num("quantization_unit"), s.get());
+    return whiten_streamed(c, src, &w);
 }
 
 // ms3.whiten_clips (p_whiten.cpp:295-345): clips M x T x L, W M x M.

@@ -17,10 +17,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "mountainsort_amd", "bin", "msamd.mp")
 
 
-def run(*args):
-    say("msamd.mp", args[0])
+def run(*args, env=None):
+    say("msamd.mp", args[0], env or "")
+    full = dict(os.environ, **env) if env else None
     r = subprocess.run([EXE, *map(str, args)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                       text=True, timeout=300)
+                       text=True, timeout=300, env=full)
     assert r.returncode == 0, r.stderr[-2000:]
     return r
 
@@ -64,6 +65,77 @@ def test_bandpass_filter_processor(gpu, exe, tmp_path):
     err = np.max(np.abs(Y - ref)) / np.sqrt(np.mean(ref ** 2))
     say(f"  bandpass max err / rms = {err:.2e}")
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("M,N,threads,k,q,dtype", [
+    (8, 1_000_000, 32, 1, 0.0, np.int16),       # four-step windows, int16 in, f32 out
+    (6, 700_000, 32, 7, 0.0, np.float32),       # subsample 7 does not divide the chunk
+    (5, 600_000, 32, 1, 0.05, np.int16),        # int16 output
+    (4, 3_000_000, 8, 1, 0.0, np.float32),      # Bluestein windows (chunk 1278101)
+])
+def test_bandpass_streamed_segments(gpu, exe, tmp_path, M, N, threads, k, q, dtype):
+    """Streaming a few windows per segment from a recording split over two
+    files gives the single-segment output bit for bit, and the oracle's
+    within the filter bound."""
+    X = np.clip(_recording(M, N, 11 + M), -32768, 32767).astype(dtype)
+    d = tmp_path / "parts"
+    d.mkdir()
+    cut = N // 3 + 17
+    mdaio.writemda(d / "a.mda", X[:cut])
+    mdaio.writemda(d / "b.mda", X[cut:])
+    args = [f"--timeseries={d}", "--samplerate=30000", "--freq_min=300", "--freq_max=6000",
+            f"--_request_num_threads={threads}", f"--subsample_factor={k}",
+            f"--quantization_unit={q}"]
+    cs = oc.bandpass_chunk_size(M, threads)
+    row = M * X.dtype.itemsize
+    run("ms3.bandpass_filter", *args, f"--timeseries_out={tmp_path / 'one.mda'}")
+    run("ms3.bandpass_filter", *args, f"--timeseries_out={tmp_path / 'seg.mda'}",
+        env={"MSAMD_SEGMENT_BYTES": str(2 * cs * row)})
+    one = mdaio.readmda(tmp_path / "one.mda")
+    seg = mdaio.readmda(tmp_path / "seg.mda")
+    nseg = -(-(-(-N // cs)) // 2)
+    say(f"  {M} ch x {N}, chunk {cs}, {nseg} segments of 2 windows")
+    assert one.shape == (N // k, M) and np.array_equal(one, seg)
+    ref = oc.bandpass_filter(X, 30000, 300, 6000, num_threads=threads, subsample_factor=k,
+                             quantization_unit=q)
+    if q:
+        assert one.dtype == np.int16
+        assert np.max(np.abs(one.astype(np.int64) - ref.astype(np.int64))) <= 1
+    else:
+        err = np.max(np.abs(one - ref)) / np.sqrt(np.mean(ref ** 2))
+        say(f"  max err / rms = {err:.2e}")
+        assert err < 1e-4
+
+
+def test_whiten_streamed_segments(gpu, exe, tmp_path):
+    """ms3.whiten over row segments: the device-resident and the two-pass
+    (re-read) forms agree bit for bit, and both agree with the oracle to one
+    quantum; ms3.apply_whitening_matrix streams the same way."""
+    M, N = 12, 400_000
+    X = _recording(M, N, 21).astype(np.float32)
+    mdaio.writemda(tmp_path / "x.mda", X)
+    seg = {"MSAMD_SEGMENT_BYTES": str(37_001 * M * 4)}
+    run("ms3.whiten", f"--timeseries={tmp_path / 'x.mda'}", f"--timeseries_out={tmp_path / 'r.mda'}",
+        env=seg)
+    run("ms3.whiten", f"--timeseries={tmp_path / 'x.mda'}", f"--timeseries_out={tmp_path / 't.mda'}",
+        env=dict(seg, MSAMD_STREAM_RESIDENT="0"))
+    Yr = mdaio.readmda(tmp_path / "r.mda")
+    Yt = mdaio.readmda(tmp_path / "t.mda")
+    assert np.array_equal(Yr, Yt)
+    ref = oc.whiten_apply(X, oc.whitening_matrix(oc.whiten_covariance(X)))
+    d = _quanta(Yr, ref)
+    say(f"  whiten streamed: max |diff| = {d} quanta of 1e-4")
+    assert d <= 1
+    rng = np.random.default_rng(5)
+    W = rng.normal(size=(M, M)) * 0.1
+    mdaio.writemda(tmp_path / "Wn.mda", np.ascontiguousarray(W.T))
+    run("ms3.apply_whitening_matrix", f"--timeseries={tmp_path / 'x.mda'}",
+        f"--whitening_matrix={tmp_path / 'Wn.mda'}", f"--timeseries_out={tmp_path / 'a.mda'}",
+        "--quantization_unit=0.01", env=seg)
+    A = mdaio.readmda(tmp_path / "a.mda")
+    _, refA = oc.whiten_apply(X, W, quantization_unit=0.01)
+    assert A.dtype == np.int16
+    assert np.max(np.abs(A.astype(np.int64) - refA.astype(np.int64))) <= 1
 
 
 def test_mask_processor(gpu, exe, tmp_path):
