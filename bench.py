@@ -516,6 +516,74 @@ def cpu_baseline(M, N_full, counts_full, widths, thread_counts, log_fn=log, devi
     }
 
 
+def file_to_file_leg(dev, cfg_name, seconds, workdir, log_fn=log):
+    """The processor chain file to file, as the MountainLab runner would call
+    it: ms3.bandpass_filter -> ms3.mask_out_artifacts -> ms3.whiten ->
+    msamd.detect_events -> msamd.pca_features, each a separate msamd.mp
+    process reading and writing .mda files in workdir (every intermediate is
+    a file, as between the reference's processors).  The synthetic recording
+    is written first (not timed).  Returns seconds per processor and the
+    whole chain's channel-samples/s."""
+    import subprocess
+
+    import numpy as np
+    from mountainsort_amd import mdaio, synth
+    exe = os.path.join(ROOT, "mountainsort_amd", "bin", "msamd.mp")
+    N = int(round(seconds * 30000))
+    cfg = synth.config1(N=N, seed=1) if cfg_name == "1" else synth.config2(N=N, seed=2)
+    os.makedirs(workdir, exist_ok=True)
+    raw_path = os.path.join(workdir, "raw.mda")
+    t0 = time.time()
+    raw = synth.generate(cfg, device=dev).cpu().numpy()
+    mdaio.writemda(raw_path, raw)
+    geom = os.path.join(workdir, "geom.csv")
+    np.savetxt(geom, synth.probe_geometry(cfg.M, cfg.columns, cfg.pitch), delimiter=",")
+    del raw
+    log_fn(f"[bench] file-to-file: wrote {cfg.M} x {N} {cfg_name == '2' and 'int16' or 'float32'} "
+           f"recording in {time.time() - t0:.1f}s")
+    f = lambda name: os.path.join(workdir, name)
+    steps = [
+        ("ms3.bandpass_filter", [f"--timeseries={raw_path}", f"--timeseries_out={f('filt.mda')}",
+                                 "--samplerate=30000", "--freq_min=300", "--freq_max=6000",
+                                 "--_request_num_threads=8"]),
+        ("ms3.mask_out_artifacts", [f"--timeseries={f('filt.mda')}",
+                                    f"--timeseries_out={f('masked.mda')}", "--threshold=6",
+                                    "--interval_size=2000"]),
+        ("ms3.whiten", [f"--timeseries={f('masked.mda')}", f"--timeseries_out={f('white.mda')}"]),
+        ("msamd.detect_events", [f"--timeseries={f('white.mda')}",
+                                 f"--event_times_out={f('events.mda')}", "--detect_threshold=3",
+                                 "--detect_sign=-1", "--detect_interval=10",
+                                 "--_request_num_threads=8"]),
+        ("msamd.pca_features", [f"--timeseries={f('white.mda')}", f"--event_times={f('events.mda')}",
+                                f"--geom={geom}", "--adjacency_radius=40", "--clip_size=50",
+                                f"--features_out={f('features.mda')}"]),
+    ]
+    secs = {}
+    sizes_done = {}
+    for name, args in steps:
+        t = time.perf_counter()
+        r = subprocess.run([exe, name, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=900)
+        secs[name] = round(time.perf_counter() - t, 3)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} failed: {r.stderr[-1000:]}")
+        log_fn(f"[bench] file-to-file: {name} {secs[name]:.2f}s")
+        # inputs no later step reads (keeps the disk footprint to three arrays)
+        for done in {"ms3.mask_out_artifacts": ["filt.mda"], "ms3.whiten": ["masked.mda"]}.get(name, []):
+            sizes_done[done] = os.path.getsize(f(done))
+            os.remove(f(done))
+    total = sum(secs.values())
+    sizes = dict(sizes_done, **{"white.mda": os.path.getsize(f("white.mda"))})
+    for n in ("raw.mda", "filt.mda", "masked.mda", "white.mda", "events.mda", "features.mda"):
+        try:
+            os.remove(f(n))
+        except OSError:
+            pass
+    return {"workload": f"config {cfg_name}: {cfg.M} ch x {N} samples", "seconds": secs,
+            "total_seconds": round(total, 3), "channel_samples_per_s": cfg.M * N / total,
+            "intermediate_bytes": sizes, "workdir": workdir}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -526,6 +594,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-pca-batching", action="store_true",
                     help="leave out the batched-PCA leg (it runs after the timed steps)")
+    ap.add_argument("--file-to-file", metavar="DIR",
+                    help="instead of the device-resident step: time the processor chain "
+                         ".mda file to file in DIR (config --f2f-config, --seconds long)")
+    ap.add_argument("--f2f-config", choices=["1", "2"], default="2")
     ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS),
                     help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
                          "elementwise negation (counter calibration) and the named stage, "
@@ -542,6 +614,10 @@ def main():
     from mountainsort_amd import synth
 
     dev = torch.device("cuda:0")
+    if args.file_to_file:
+        print(json.dumps({"file_to_file": file_to_file_leg(dev, args.f2f_config, args.seconds,
+                                                           args.file_to_file)}), flush=True)
+        return
     N = int(round(args.seconds * 30000))
     cfg = synth.config2(N=N, seed=2)
     t0 = time.time()

@@ -5,6 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+
+#include <unistd.h>
 
 namespace msamd_host {
 
@@ -79,57 +82,26 @@ bool upload_timeseries(const TimeseriesSource& src, int32_t target, void* dev, h
                     err))
             return false;
     }
-    std::vector<char> raw;
+    RowReader rd(src);
+    const int64_t row = std::max<int64_t>(1, src.M * tb);
+    const int64_t per = std::max<int64_t>(1, kStage / row);
     char* out = static_cast<char*>(dev);
     int cur = 0;
     bool used[2] = {false, false};
-    for (size_t fi = 0; fi < src.files.size(); ++fi) {
-        const MdaHeader& h = src.headers[fi];
-        FILE* f = fopen(src.files[fi].c_str(), "rb");
-        if (!f) {
-            err = "unable to open " + src.files[fi];
+    // read block k while block k-1 is copied
+    for (int64_t t = 0; t < src.N; t += per) {
+        const int64_t t1 = std::min(src.N, t + per);
+        if (used[cur] && !hip_ok(hipEventSynchronize(done[cur].e), "hipEventSynchronize", err))
             return false;
-        }
-        if (fseeko(f, h.header_size, SEEK_SET) != 0) {
-            fclose(f);
-            err = "seek failed in " + src.files[fi];
+        if (!rd.read(target, t, t1, buf[cur].p, err)) return false;
+        const int64_t n = (t1 - t) * row;
+        if (!hip_ok(hipMemcpyAsync(out, buf[cur].p, static_cast<size_t>(n), hipMemcpyHostToDevice, s),
+                    "hipMemcpyAsync", err) ||
+            !hip_ok(hipEventRecord(done[cur].e, s), "hipEventRecord", err))
             return false;
-        }
-        const int sb = h.bytes_per_entry;
-        const int64_t per = kStage / std::max(sb, tb);
-        int64_t left = h.dim(0) * h.dim(1);
-        while (left > 0) {
-            const int64_t n = std::min(per, left);
-            if (used[cur] && !hip_ok(hipEventSynchronize(done[cur].e), "hipEventSynchronize", err)) {
-                fclose(f);
-                return false;
-            }
-            void* dst = buf[cur].p;
-            size_t got;
-            if (h.type == target) {
-                got = fread(dst, sb, static_cast<size_t>(n), f);
-            } else {
-                raw.resize(static_cast<size_t>(n * sb));
-                got = fread(raw.data(), sb, static_cast<size_t>(n), f);
-                mda_convert_to_f32(h.type, raw.data(), static_cast<float*>(dst), n);
-            }
-            if (static_cast<int64_t>(got) != n) {
-                fclose(f);
-                err = "truncated data in " + src.files[fi];
-                return false;
-            }
-            if (!hip_ok(hipMemcpyAsync(out, dst, n * tb, hipMemcpyHostToDevice, s),
-                        "hipMemcpyAsync", err) ||
-                !hip_ok(hipEventRecord(done[cur].e, s), "hipEventRecord", err)) {
-                fclose(f);
-                return false;
-            }
-            used[cur] = true;
-            out += n * tb;
-            left -= n;
-            cur ^= 1;
-        }
-        fclose(f);
+        used[cur] = true;
+        out += n;
+        cur ^= 1;
     }
     return hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize", err);
 }
@@ -220,22 +192,42 @@ bool RowReader::read(int32_t target, int64_t t0, int64_t t1, void* dst, std::str
                     return false;
                 }
             }
-            FILE* f = f_[fi];
+            const int fd = fileno(f_[fi]);
             const int sb = h.bytes_per_entry;
-            if (fseeko(f, h.header_size + (lo - file_t0) * M * sb, SEEK_SET) != 0) {
-                err = "seek failed in " + src_.files[fi];
-                return false;
-            }
             const int64_t cnt = (hi - lo) * M;
-            size_t got;
-            if (h.type == target) {
-                got = fread(out, sb, static_cast<size_t>(cnt), f);
+            const off_t base = h.header_size + (lo - file_t0) * M * sb;
+            // positioned reads, split over threads for large ranges (each
+            // thread converts its own part)
+            const int P = cnt * sb >= kParallelIoBytes ? kParallelIoThreads : 1;
+            const int64_t part = (cnt + P - 1) / P;
+            if (h.type != target) raw_.resize(static_cast<size_t>(cnt * sb));
+            std::vector<char> ok(P, 1);
+            auto work = [&](int i) {
+                const int64_t a = i * part, b = std::min(cnt, a + part);
+                if (b <= a) return;
+                char* dst = h.type == target ? out + a * sb : raw_.data() + a * sb;
+                const int64_t want = (b - a) * sb;
+                int64_t done = 0;
+                while (done < want) {
+                    const ssize_t r = pread(fd, dst + done, static_cast<size_t>(want - done),
+                                            base + a * sb + done);
+                    if (r <= 0) {
+                        ok[i] = 0;
+                        return;
+                    }
+                    done += r;
+                }
+                if (h.type != target)
+                    mda_convert_to_f32(h.type, dst, reinterpret_cast<float*>(out) + a, b - a);
+            };
+            if (P == 1) {
+                work(0);
             } else {
-                raw_.resize(static_cast<size_t>(cnt * sb));
-                got = fread(raw_.data(), sb, static_cast<size_t>(cnt), f);
-                mda_convert_to_f32(h.type, raw_.data(), reinterpret_cast<float*>(out), cnt);
+                std::vector<std::thread> th;
+                for (int i = 0; i < P; ++i) th.emplace_back(work, i);
+                for (auto& t : th) t.join();
             }
-            if (static_cast<int64_t>(got) != cnt) {
+            if (std::find(ok.begin(), ok.end(), 0) != ok.end()) {
                 err = "truncated data in " + src_.files[fi];
                 return false;
             }
@@ -285,7 +277,19 @@ bool SegmentPipeline::run(int64_t K, const Stage& st, std::string& err) {
     const void* dev_src[2] = {nullptr, nullptr};
     int64_t out_n[2] = {0, 0};
     bool used_up[2] = {false, false}, used_comp[2] = {false, false}, used_down[2] = {false, false};
+    // consume (the file write) of segment j runs on its own thread while the
+    // main thread reads segment j+2; it is joined before host_out[j & 1] is
+    // filled again
+    std::thread writer;
+    bool wrote_ok = true;
+    std::string werr;
+    auto join_writer = [&]() {
+        if (writer.joinable()) writer.join();
+        if (!wrote_ok && err.empty()) err = werr;
+        return wrote_ok;
+    };
     auto drain = [&]() {
+        join_writer();
         (void)hipStreamSynchronize(su_);
         (void)hipStreamSynchronize(sc_);
         (void)hipStreamSynchronize(sd_);
@@ -346,6 +350,11 @@ bool SegmentPipeline::run(int64_t K, const Stage& st, std::string& err) {
             }
             out_n[b] = ob;
             if (ob > 0) {
+                // the writer of segment j-2 (the same host buffer) has finished
+                if (!join_writer()) {
+                    drain();
+                    return false;
+                }
                 if (!hip_ok(hipStreamWaitEvent(sd_, comp_[b], 0), "hipStreamWaitEvent", err) ||
                     !hip_ok(hipMemcpyAsync(host_out_[b], dev_out_[b].get(), static_cast<size_t>(ob),
                                            hipMemcpyDeviceToHost, sd_),
@@ -361,12 +370,23 @@ bool SegmentPipeline::run(int64_t K, const Stage& st, std::string& err) {
             const int64_t j = k - 2;
             const int b = static_cast<int>(j & 1);
             const hipEvent_t done = out_n[b] > 0 ? down_[b] : comp_[b];
-            if (!hip_ok(hipEventSynchronize(done), "hipEventSynchronize", err) ||
-                (st.consume && !st.consume(st.ctx, j, host_out_[b], out_n[b], err))) {
+            // segments are written in order: the previous writer finishes first
+            if (!join_writer() || !hip_ok(hipEventSynchronize(done), "hipEventSynchronize", err)) {
                 drain();
                 return false;
             }
+            if (st.consume) {
+                const void* buf = host_out_[b];
+                const int64_t nb = out_n[b];
+                writer = std::thread([&st, &wrote_ok, &werr, j, buf, nb]() {
+                    wrote_ok = st.consume(st.ctx, j, buf, nb, werr);
+                });
+            }
         }
+    }
+    if (!join_writer()) {
+        drain();
+        return false;
     }
     return hip_ok(hipStreamSynchronize(sc_), "hipStreamSynchronize", err);
 }

@@ -3,12 +3,15 @@
 
 #include <dirent.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <thread>
+#include <vector>
 
 namespace msamd_host {
 
@@ -204,7 +207,43 @@ bool MdaWriter::write(const void* data, int64_t bytes, std::string& err) {
         err = "writer is not open";
         return false;
     }
-    if (bytes > 0 && fwrite(data, 1, static_cast<size_t>(bytes), f_) != static_cast<size_t>(bytes)) {
+    if (bytes >= kParallelIoBytes) {
+        // large blocks: positioned writes from several threads at once (the
+        // page-cache copy is the cost, and one thread moves ~5 GB/s)
+        if (fflush(f_) != 0) {
+            err = "problem writing " + path_ + ".tmp";
+            return false;
+        }
+        const off_t off = ftello(f_);
+        const int fd = fileno(f_);
+        const int P = kParallelIoThreads;
+        const int64_t part = (bytes + P - 1) / P;
+        std::vector<char> ok(P, 1);
+        std::vector<std::thread> th;
+        for (int i = 0; i < P; ++i) {
+            const int64_t a = i * part, b = std::min(bytes, a + part);
+            if (b <= a) break;
+            th.emplace_back([&, i, a, b]() {
+                const char* src = static_cast<const char*>(data) + a;
+                int64_t done = 0;
+                while (done < b - a) {
+                    const ssize_t w = pwrite(fd, src + done, static_cast<size_t>(b - a - done),
+                                             off + a + done);
+                    if (w <= 0) {
+                        ok[i] = 0;
+                        return;
+                    }
+                    done += w;
+                }
+            });
+        }
+        for (auto& t : th) t.join();
+        if (std::find(ok.begin(), ok.end(), 0) != ok.end() || fseeko(f_, off + bytes, SEEK_SET) != 0) {
+            err = "problem writing " + path_ + ".tmp";
+            return false;
+        }
+    } else if (bytes > 0 &&
+               fwrite(data, 1, static_cast<size_t>(bytes), f_) != static_cast<size_t>(bytes)) {
         err = "problem writing " + path_ + ".tmp";
         return false;
     }

@@ -55,6 +55,11 @@ bool mda_read_f64(const std::string& path, std::vector<double>& out, MdaHeader& 
 bool mda_read_f32(const std::string& path, std::vector<float>& out, MdaHeader& h,
                   std::string& err);
 
+// Reads and writes of at least this many bytes are split over
+// kParallelIoThreads threads (positioned I/O on the same file).
+constexpr int64_t kParallelIoBytes = 32ll << 20;
+constexpr int kParallelIoThreads = 4;
+
 // Streamed writer: writes path + ".tmp", renamed to path by close()
 // (packages/ms3/mda/diskwritemda.cpp:84,142-146).  The number of dimensions
 // written drops trailing 1s (minimum 2); 64-bit dimensions are used when any

@@ -162,7 +162,7 @@ int clip_tmid(int64_t T) { return static_cast<int>((T + 1) / 2) - 1; }
 
 // Segments of about this many input bytes stream through the device
 // (MSAMD_SEGMENT_BYTES overrides it).
-constexpr int64_t kSegmentBytes = 1ll << 30;
+constexpr int64_t kSegmentBytes = 256ll << 20;
 
 // Pinned staging bytes of a segment pipeline (two input and two output buffers).
 double pipeline_ram_mb(int64_t in_bytes, int64_t out_bytes) {
@@ -226,7 +226,13 @@ bool run_bandpass(Ctx& c) {
         return copy_file(c, c.str("timeseries"), c.str("timeseries_out"));  // :117-119
     }
     TimeseriesSource src;
-    if (!open_timeseries(c, "timeseries", src)) return false;
+    if (!open_timeseries(c, "timeseries", src)) {
+        if (!c.requirements_only) return false;
+        // inputs not there yet: the staging of the largest segments (float32
+        // in, float32 out)
+        c.peak_ram_mb = pipeline_ram_mb(segment_bytes(kSegmentBytes), segment_bytes(kSegmentBytes));
+        return true;
+    }
     BandpassStream b{};
     b.M = src.M;
     b.N = src.N;
@@ -429,7 +435,11 @@ bool whiten_streamed(Ctx& c, const TimeseriesSource& src, const std::vector<doub
 
 bool run_whiten(Ctx& c) {
     TimeseriesSource src;
-    if (!open_timeseries(c, "timeseries", src)) return false;
+    if (!open_timeseries(c, "timeseries", src)) {
+        if (!c.requirements_only) return false;
+        c.peak_ram_mb = pipeline_ram_mb(segment_bytes(kSegmentBytes), segment_bytes(kSegmentBytes));
+        return true;
+    }
     return whiten_streamed(c, src, nullptr);
 }
 
