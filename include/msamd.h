@@ -36,6 +36,7 @@ extern "C" {
 #define MSAMD_ERR_WORKSPACE 3
 #define MSAMD_ERR_CAPACITY 4
 #define MSAMD_ERR_UNSUPPORTED 5
+#define MSAMD_ERR_TIMEOUT 6   /* a bounded wait between workgroups gave up */
 
 /* .mda element type codes (packages/ms3/mda/mdaio.h:55-62) accepted as input. */
 #define MSAMD_DTYPE_FLOAT32 (-3)
@@ -296,7 +297,11 @@ int64_t msamd_whitening_matrix_workspace_size(int64_t M);
 /* W from XXt by the reference's method: per component 10 power steps from
  * normalize(sin(1..M)), sigma = |G v|, deflation G <- (I-vv')G(I-vv'), then
  * W = C diag(sigma^-1/2) C' (zero where sigma == 0).  FP64 on the GPU.
- * XXt and W: M x M float64 (device). */
+ * XXt and W: M x M float64 (device).  The power steps exchange values
+ * between workgroups that must run at the same time; when other work on the
+ * device keeps some from starting, the bounded waits give up and the call
+ * returns MSAMD_ERR_TIMEOUT (W is then not written; callers may retry or use
+ * msamd_whitening_matrix_exact). */
 int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, void* workspace,
                            int64_t workspace_bytes, void* stream);
 

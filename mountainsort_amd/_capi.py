@@ -92,6 +92,9 @@ class MsamdLibraryMissing(RuntimeError):
     pass
 
 
+ERR_TIMEOUT = 6  # MSAMD_ERR_TIMEOUT
+
+
 class MsamdError(RuntimeError):
     def __init__(self, name, code, msg):
         super().__init__(f"{name} failed with code {code}: {msg}")

@@ -372,6 +372,18 @@ __global__ void __launch_bounds__(256) cov_finish_kernel(double* __restrict__ C,
 constexpr int kWMaxThreads = 512;
 constexpr unsigned kWSpinLimit = 1u << 20;
 
+// Polls a wait may spin before it gives up: kWSpinLimit, or
+// MSAMD_W_SPIN_LIMIT from the environment (read at every call; a small
+// value makes waits give up early, which tests use to drive the timeout
+// path while every wavefront still runs to its end).
+inline unsigned w_spin_limit() {
+    if (const char* e = std::getenv("MSAMD_W_SPIN_LIMIT")) {
+        const long long v = std::atoll(e);
+        if (v >= 0 && v < static_cast<long long>(kWSpinLimit)) return static_cast<unsigned>(v);
+    }
+    return kWSpinLimit;
+}
+
 struct WPlan {
     int M, Mp, S, L, J, RW, NP, T;
 };
@@ -449,7 +461,7 @@ __device__ __forceinline__ void w_publish(u64* slot, double x, unsigned tag) {
 // after a timeout or when another wavefront has raised the error flag.
 template <int L, int J>
 __device__ __forceinline__ bool w_fetch(const u64* pub, int q, int M, unsigned tag, double* out,
-                                        int* err) {
+                                        int* err, unsigned limit) {
     unsigned spins = 0;
     for (;;) {
         bool ready = true;
@@ -461,8 +473,9 @@ __device__ __forceinline__ bool w_fetch(const u64* pub, int q, int M, unsigned t
             out[j] = __builtin_bit_cast(double, b & ~15ull);
         }
         if (ready) return true;
-        if ((++spins & 63u) == 0 &&
-            (spins > kWSpinLimit ||
+        ++spins;
+        if (((spins & 63u) == 0 || spins > limit) &&
+            (spins > limit ||
              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
@@ -475,6 +488,7 @@ struct WBufs {
     u64* wbuf;  // [2][Mp] published rows of G v (tagged words)
     u64* ybuf;  // [2][Mp] published rows of G v0 (deflation exchange)
     int* err;
+    unsigned spin_limit;
 };
 
 template <int L, int J>
@@ -526,7 +540,7 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
                 for (int j = 0; j < J; ++j) acc = fma(g[j], v[j], acc);
                 acc = group_sum<L>(acc);
                 if (q == 0) w_publish(bufs.wbuf + par * Mp + row, acc, tag);
-                w_fetch<L, J>(wpub + par * Mp, q, M, tag, v, bufs.err);
+                w_fetch<L, J>(wpub + par * Mp, q, M, tag, v, bufs.err, bufs.spin_limit);
             }
             double ss = 0.0;
 #pragma unroll
@@ -571,8 +585,8 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
                 w_publish(bufs.wbuf + par * Mp + row, au, tag);
                 w_publish(bufs.ybuf + par * Mp + row, ay, tag);
             }
-            w_fetch<L, J>(wpub + par * Mp, q, M, tag, u, bufs.err);
-            w_fetch<L, J>(ypub + par * Mp, q, M, tag, y, bufs.err);
+            w_fetch<L, J>(wpub + par * Mp, q, M, tag, u, bufs.err, bufs.spin_limit);
+            w_fetch<L, J>(ypub + par * Mp, q, M, tag, y, bufs.err, bufs.spin_limit);
         }
         double d_uv0 = 0.0, d_vv0 = 0.0, alpha = 0.0;
 #pragma unroll
@@ -1138,6 +1152,7 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     wb.wbuf = wbuf;
     wb.ybuf = ybuf;
     wb.err = err;
+    wb.spin_limit = w_spin_limit();
     // published words start at 0.  A word is first written at step 1 or 2
     // (G v slots) or 11 (G v0 slot), whose tags are not 0; later a waiting
     // lane can only find the tag of an earlier use (step e-2, or e-10 for the
@@ -1161,8 +1176,11 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     MSAMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
     MSAMD_HIP(hipStreamSynchronize(st));
     if (herr) {
-        set_error("whitening_matrix: exchange between workgroups timed out");
-        return MSAMD_ERR_HIP;
+        set_error("whitening_matrix: the exchange between the %d workgroups of the power method "
+                  "timed out (other work on the device kept some of them from running); "
+                  "msamd_whitening_matrix_exact needs no co-resident workgroups",
+                  wp.NP);
+        return MSAMD_ERR_TIMEOUT;
     }
     whitening_assemble_kernel<<<static_cast<unsigned>(ceil_div(M * M, 256)), 256, 0, st>>>(
         comp, sig, static_cast<int>(M), W);

@@ -375,14 +375,31 @@ def whiten_covariance(X, row_keep=None):
     return C
 
 
-def whitening_matrix(C, exact=False, return_info=False):
+def whitening_matrix(C, exact=False, return_info=False, fallback=None):
     """W from the covariance.
 
     exact=False: the reference's deflated power method (pca.cpp:634-650).
     exact=True: W = C^{-1/2}, the symmetric inverse square root the power
     method approximates (coupled Newton-Schulz in FP64; zero rows and columns
     for channels with a zero diagonal).  With return_info the exact mode also
-    returns (iterations, residual)."""
+    returns (iterations, residual).
+
+    The power method's workgroups exchange values and must run together; if
+    other work on the device keeps them apart its waits give up
+    (MsamdError with code ERR_TIMEOUT).  fallback="exact" then warns and
+    returns the exact form instead (a different W: the power method's is not
+    converged)."""
+    if not exact and fallback not in (None, "exact"):
+        raise ValueError("fallback must be None or 'exact'")
+    if not exact and fallback == "exact":
+        try:
+            return whitening_matrix(C, exact=False, return_info=return_info)
+        except _capi.MsamdError as e:
+            if e.code != _capi.ERR_TIMEOUT:
+                raise
+            import warnings
+            warnings.warn(f"{e}; using the exact whitening matrix", RuntimeWarning)
+            return whitening_matrix(C, exact=True, return_info=return_info)
     M = C.shape[0]
     dev = C.device
     Ccm = C.to(torch.float64).t().contiguous()

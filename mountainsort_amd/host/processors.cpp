@@ -351,6 +351,26 @@ struct WhitenStream {
     }
 };
 
+// The reference's power-method W; if its workgroups could not run together
+// (MSAMD_ERR_TIMEOUT: other work held the device), one retry, then the
+// exact inverse square root with a warning.
+bool whitening_matrix_or_exact(Ctx& c, const double* C, int64_t M, double* W, hipStream_t s) {
+    DeviceBuffer ws;
+    if (!ws.alloc(msamd_whitening_matrix_workspace_size(M), c.err)) return false;
+    int rc = MSAMD_ERR_TIMEOUT;
+    for (int attempt = 0; attempt < 2 && rc == MSAMD_ERR_TIMEOUT; ++attempt)
+        rc = msamd_whitening_matrix(C, M, W, ws.get(), ws.bytes(), s);
+    if (rc != MSAMD_ERR_TIMEOUT) return lib_ok(c, rc, "whitening_matrix");
+    const char* m = msamd_last_error();
+    fprintf(stderr, "warning: %s; using the exact whitening matrix instead\n", m ? m : "timeout");
+    DeviceBuffer wx;
+    int32_t iters = 0;
+    double resid = 0.0;
+    if (!wx.alloc(msamd_whitening_matrix_exact_workspace_size(M), c.err)) return false;
+    return lib_ok(c, msamd_whitening_matrix_exact(C, M, W, wx.get(), wx.bytes(), s, &iters, &resid),
+                  "whitening_matrix_exact");
+}
+
 // W given (apply_whitening_matrix, device column-major) or computed
 // (whiten).  Returns false with c.err on failure.
 bool whiten_streamed(Ctx& c, const TimeseriesSource& src, const std::vector<double>* Wgiven) {
@@ -413,13 +433,8 @@ bool whiten_streamed(Ctx& c, const TimeseriesSource& src, const std::vector<doub
         } else if (!pipe.run(K, p1, c.err)) {
             return false;
         }
-        DeviceBuffer ws2;
-        if (!ws2.alloc(msamd_whitening_matrix_workspace_size(M), c.err) ||
-            !lib_ok(c, msamd_whiten_covariance_finish(w.C, M, w.N, s.get()), "whiten_covariance") ||
-            !lib_ok(c, msamd_whitening_matrix(w.C, M, Wd.as<double>(), ws2.get(), ws2.bytes(),
-                                              s.get()),
-                    "whitening_matrix") ||
-            !s.sync(c.err))
+        if (!lib_ok(c, msamd_whiten_covariance_finish(w.C, M, w.N, s.get()), "whiten_covariance") ||
+            !whitening_matrix_or_exact(c, w.C, M, Wd.as<double>(), s.get()) || !s.sync(c.err))
             return false;
         w.with_cov = true;
     }
@@ -447,17 +462,14 @@ bool run_whiten(Ctx& c) {
 // a device M x M float64 column-major buffer.
 bool whitening_matrix_of(Ctx& c, const float* X, int64_t M, int64_t N, DeviceBuffer& Wd,
                          hipStream_t s, DeviceBuffer* Cout = nullptr) {
-    DeviceBuffer Cl, ws1, ws2;
+    DeviceBuffer Cl, ws1;
     DeviceBuffer& C = Cout ? *Cout : Cl;
     if (!C.alloc(M * M * 8, c.err) || !Wd.alloc(M * M * 8, c.err) ||
-        !ws1.alloc(msamd_whiten_covariance_workspace_size(M, N), c.err) ||
-        !ws2.alloc(msamd_whitening_matrix_workspace_size(M), c.err))
+        !ws1.alloc(msamd_whiten_covariance_workspace_size(M, N), c.err))
         return false;
     return lib_ok(c, msamd_whiten_covariance(X, M, N, C.as<double>(), ws1.get(), ws1.bytes(), s),
                   "whiten_covariance") &&
-           lib_ok(c, msamd_whitening_matrix(C.as<double>(), M, Wd.as<double>(), ws2.get(),
-                                            ws2.bytes(), s),
-                  "whitening_matrix");
+           whitening_matrix_or_exact(c, C.as<double>(), M, Wd.as<double>(), s);
 }
 
 bool run_compute_whitening_matrix(Ctx& c) {

@@ -355,3 +355,31 @@ def test_whiten_apply_scaling_extremes(gpu, scale):
         d = np.abs(np.round(Y.astype(np.float64) / 1e-4) - k64)
         say(f"  xxt={'yes' if xxt is not None else 'no'}: off-by-one {np.mean(d > 0):.4f}, max {d.max():.0f}")
         assert d.max() <= 1
+
+
+def test_whitening_matrix_wait_timeout_is_reported(gpu, monkeypatch):
+    """The power method's bounded waits, with their spin limit forced to
+    zero from the host: the first wait gives up, every wavefront still runs
+    to its end, and the call returns ERR_TIMEOUT with its message;
+    fallback='exact' returns the exact form with a warning; the next call
+    with the normal limit succeeds and matches a call made before."""
+    import torch
+
+    from mountainsort_amd import _capi
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(6)
+    M = 384
+    A = rng.normal(size=(M, 2 * M))
+    C = torch.from_numpy(A @ A.T / (2 * M) + np.eye(M)).to(gpu)
+    W_before = fe.whitening_matrix(C).cpu().numpy()
+    say("whitening matrix, spin limit 0")
+    monkeypatch.setenv("MSAMD_W_SPIN_LIMIT", "0")
+    with pytest.raises(_capi.MsamdError) as ei:
+        fe.whitening_matrix(C)
+    assert ei.value.code == _capi.ERR_TIMEOUT and "timed out" in str(ei.value)
+    with pytest.warns(RuntimeWarning, match="exact"):
+        Wx = fe.whitening_matrix(C, fallback="exact").cpu().numpy()
+    assert np.array_equal(Wx, fe.whitening_matrix(C, exact=True).cpu().numpy())
+    monkeypatch.delenv("MSAMD_W_SPIN_LIMIT")
+    W_after = fe.whitening_matrix(C).cpu().numpy()
+    assert np.array_equal(W_after, W_before)

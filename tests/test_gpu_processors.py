@@ -184,6 +184,24 @@ def test_whiten_processors(gpu, exe, tmp_path):
     assert d <= 1
 
 
+def test_whiten_processor_falls_back_to_exact(gpu, exe, tmp_path):
+    """When the power method's waits give up (spin limit forced to zero),
+    ms3.whiten retries, then warns and whitens with the exact C^-1/2.  (At
+    384 channels the power method runs 96 workgroups, so a wait is certain.)"""
+    M, N = 384, 40_000
+    X = _recording(M, N, 31).astype(np.float32)
+    mdaio.writemda(tmp_path / "x.mda", X)
+    r = run("ms3.whiten", f"--timeseries={tmp_path / 'x.mda'}",
+            f"--timeseries_out={tmp_path / 'w.mda'}", env={"MSAMD_W_SPIN_LIMIT": "0"})
+    assert "exact whitening matrix" in r.stderr
+    Y = mdaio.readmda(tmp_path / "w.mda")
+    w, V = np.linalg.eigh(oc.covariance_fp64(X))
+    Wx = (V / np.sqrt(w)) @ V.T
+    d = _quanta(Y, oc.whiten_apply(X, Wx))
+    say(f"  exact fallback: max |diff| = {d} quanta of 1e-4")
+    assert d <= 1
+
+
 def test_whiten_clips_processor(gpu, exe, tmp_path):
     """ms3.whiten_clips: W on every time sample of every clip, no 1e-4 quantize;
     float output within FP32 rounding of the reference's float loop, int16
