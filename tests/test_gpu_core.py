@@ -165,7 +165,10 @@ def test_whitening_matrix_quality_m64(gpu):
     qo = np.max(np.abs(Wo @ C @ Wo.T - np.eye(64)))
     qg = np.max(np.abs(Wg @ C @ Wg.T - np.eye(64)))
     say(f"  max|WCW'-I| gpu={qg:.3e} oracle={qo:.3e} |dW|/|W|={np.linalg.norm(Wg - Wo) / np.linalg.norm(Wo):.3e}")
-    assert qg < max(2 * qo, 1e-6) + 0.05
+    # Rounding-level perturbations of C (1e-14 relative, 30 draws) move the
+    # oracle's own max|WCW'-I| over [0.96, 1.09] x its unperturbed value here;
+    # the GPU's W must stay within 1.25 x.
+    assert qg < 1.25 * qo
 
 
 @pytest.mark.parametrize("M", [24, 100, 200, 384, 500, 1000])
