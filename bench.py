@@ -132,6 +132,10 @@ class Pipeline:
 # component).  split_terms is the MFMA products the kernel issues per
 # algorithmic product (split-FP16 / split-BF16 pieces).
 ROOFLINE_KERNELS = {
+    # the filter stage is one library call of three kernels per window batch
+    # (225 batches at 384 ch x 300 s); its launch is the call
+    "bandpass": {"kernel": "bp_stage_a_wide,bp_stage_b,bp_stage_c", "bound": "hbm",
+                 "peak": 8000.0, "unit": "GB/s", "bytes_per_chs": 6},
     "apply": {"kernel": "apply_kernel", "bound": "hbm", "peak": 8000.0, "unit": "GB/s",
               "bytes_per_chs": 8, "mfma_peak": 2500.0, "dtype_peak": "fp16 dense",
               "split_terms": 3},
@@ -144,6 +148,21 @@ ROOFLINE_KERNELS = {
 
 def roofline_of(stage, ms, M, N, traffic_file):
     k = ROOFLINE_KERNELS[stage]
+    if stage == "bandpass":
+        alg = float(k["bytes_per_chs"]) * M * N
+        achieved = alg / (ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "kernel": k["kernel"], "achieved": round(achieved, 1),
+               "peak": k["peak"], "unit": k["unit"], "frac": round(achieved / k["peak"], 5),
+               "traffic": None, "algorithmic_bytes_per_launch": alg, "launch_ms": round(ms, 4)}
+        try:
+            with open(traffic_file) as f:
+                t = json.load(f)
+            if t.get("kernel") == k["kernel"] and "traffic_bytes" in t:
+                out["traffic"] = t["traffic_bytes"]
+                out["traffic_source"] = os.path.basename(traffic_file)
+        except (OSError, ValueError):
+            pass
+        return out
     if stage == "whitening_matrix":
         flops = 11.0 * 2.0 * M * M * M
     else:
@@ -634,7 +653,9 @@ def main():
         for _ in range(3):  # calibration: reads and writes copy_bytes (torch's neg kernel)
             torch.neg(src, out=dst)
         for _ in range(3):
-            if args.pmc_pass == "apply":
+            if args.pmc_pass == "bandpass":
+                fe.bandpass_filter(raw, 30000, 300, 6000, num_threads=8, out=pipe.filt)
+            elif args.pmc_pass == "apply":
                 fe.whiten_apply(pipe.filt, pipe.W, out=pipe.white, row_keep=pipe.rows, xxt=pipe.C)
             elif args.pmc_pass == "covariance":
                 fe.whiten_covariance(pipe.filt, row_keep=pipe.rows)
@@ -680,9 +701,19 @@ def main():
             rl[k] = {"achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
     # the single-launch kernel with the largest launch time in the step
     dom = max((k for k in ROOFLINE_KERNELS if k in stage_ms), key=lambda k: stage_ms[k])
+    tname = "bandpass" if dom == "bandpass" else ROOFLINE_KERNELS[dom]["kernel"]
     traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                f"{ROOFLINE_KERNELS[dom]['kernel']}_384ch_300s_traffic.json")
+                                f"{tname}_384ch_300s_traffic.json")
     roofline = roofline_of(dom, stage_ms[dom], cfg.M, N, traffic_file)
+    # the same figures for every stage that has one
+    rooflines = {}
+    for st_name in ROOFLINE_KERNELS:
+        if st_name in stage_ms:
+            tn = "bandpass" if st_name == "bandpass" else ROOFLINE_KERNELS[st_name]["kernel"]
+            rooflines[st_name] = roofline_of(
+                st_name, stage_ms[st_name], cfg.M, N,
+                os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                             f"{tn}_384ch_300s_traffic.json"))
     total_alg = sum(alg_bytes.values()) * chs
     line = {
         "metric": "channel-samples/sec through filter->whiten->detect->clips->PCA; % HBM peak",
@@ -706,6 +737,7 @@ def main():
         "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_roofline": rl,
         "roofline": roofline,
+        "rooflines": rooflines,
         "cpu_baseline": None,
     }
     if not args.no_learned:

@@ -7,11 +7,13 @@ synthetic-data generation outside the timed region, are summed into one
 line).
 
 ``python -m mountainsort_amd.profiling traffic FETCH.db WRITE.db KERNEL COPY_BYTES
-PREFIX "fetch command" "write command"`` turns two ``rocprofv3 --pmc
+PREFIX "fetch command" "write command" [CALLS]`` turns two ``rocprofv3 --pmc
 FETCH_SIZE`` / ``--pmc WRITE_SIZE`` results databases into the HBM bytes per
 launch of KERNEL, each counter scaled by what it reads for a copy kernel of
 known size (an elementwise negation) run in the same program (PREFIX.md and
-PREFIX.json).
+PREFIX.json).  KERNEL may list the kernels of one stage, comma-separated;
+the figures are then per call of the stage, CALLS being its calls in the
+program.
 """
 import csv
 import re
@@ -114,12 +116,15 @@ def pmc_counters(db_path):
     return agg, dur
 
 
-def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, command_write):
+def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, command_write,
+                   calls=1):
     """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by the factor that makes the same counter read the
     known bytes of the calibration kernel (an elementwise negation, whose name
     contains 'neg', reading and writing copy_bytes per launch).  Returns
-    (dict, text)."""
+    (dict, text).  kernel may be a comma-separated list of the kernels of one
+    stage; its figures are then per call of the stage (calls of it in the
+    measured program)."""
     fa, fdur = pmc_counters(fetch_db)
     wa, _ = pmc_counters(write_db)
 
@@ -139,12 +144,21 @@ def traffic_report(fetch_db, write_db, kernel, copy_bytes, command_fetch, comman
 
     calib.used = {}
 
-    is_k = lambda k: k.endswith(kernel) or k.split("<")[0].endswith(kernel)
+    names = kernel.split(",")
+    is_k = lambda k: any(k.endswith(n) or k.split("<")[0].endswith(n) for n in names)
+    if len(names) > 1:
+        # a stage of several kernels: the counters of all their dispatches,
+        # per call of the stage (calls = the stage's calls in the program)
+        def avg(agg, pred, counter):
+            vals = [v for k, c in agg.items() if pred(k) for v in c.get(counter, [])]
+            return sum(vals) / calls if vals else None
     f_k, w_k = avg(fa, is_k, "FETCH_SIZE"), avg(wa, is_k, "WRITE_SIZE")
     f_c, w_c = calib(fa, "FETCH_SIZE"), calib(wa, "WRITE_SIZE")
     f_scale = copy_bytes / (f_c * 1024.0) if f_c else None
     w_scale = copy_bytes / (w_c * 1024.0) if w_c else None
     durs = [d for k, v in fdur.items() if is_k(k) for d in v]
+    if len(names) > 1 and durs:
+        durs = [sum(durs) / calls]
     out = {"kernel": kernel,
            "fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k,
            "copy_bytes": copy_bytes, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
@@ -167,8 +181,11 @@ def main_traffic(argv):
     """python -m mountainsort_amd.profiling traffic FETCH.db WRITE.db KERNEL COPY_BYTES
     OUT_PREFIX "fetch command" "write command" -> OUT_PREFIX.md and OUT_PREFIX.json"""
     import json
-    fetch_db, write_db, kernel, copy_bytes, prefix, cf, cw = argv
-    out, text = traffic_report(fetch_db, write_db, kernel, float(copy_bytes), cf, cw)
+    fetch_db, write_db, kernel, copy_bytes, prefix, cf, cw = argv[:7]
+    calls = int(argv[7]) if len(argv) > 7 else 1
+    out, text = traffic_report(fetch_db, write_db, kernel, float(copy_bytes), cf, cw, calls)
+    if calls > 1:
+        out["stage_calls"] = calls
     open(prefix + ".md", "w").write(text)
     json.dump(out, open(prefix + ".json", "w"), indent=1)
     sys.stdout.write(text)
