@@ -31,7 +31,7 @@
 namespace msamd {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -149,21 +149,20 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 // partial Gram, and pca_gram_reduce_kernel sums a neighborhood's chunk
 // partials in chunk order.
 //
-// Precision.  Every float is split exactly into three bf16 pieces,
-// x = h + m + l (h = bf16(x), m = bf16(x - h), l = x - h - m, each
-// difference exact in FP32), and the six products of weight >= 2^-16 are
-// accumulated: hh + hm + mh + hl + lh + mm (the dropped ml, lm, ll are
-// below 2^-23 of |x y| and of random sign).  bf16 products are exact in
-// FP32; the FP32 accumulator is cleared at every 32-column step and its
-// step sum added to FP64 accumulators, so rounding grows with one step's
-// sum, not with the whole chunk's.  The Gram comes out within ~1e-8 of
-// FP64.  (Two-piece split-BF16 or split-FP16 with FP32 accumulation over a
-// chunk leaves 1e-6 .. 1e-5, which the power method's later components
-// amplify past 1e-3 on some neighborhoods.)
+// Precision.  Each 32-column step is scaled by a power of two s that puts
+// the step's largest |x| in [2^14, 2^15) (exact), and every scaled value is
+// split into two fp16 pieces, x s = h + l (h = fp16(x s), l = fp16(x s - h),
+// 22 significant bits), of which the three products of weight >= 2^-11 are
+// accumulated: hh + hl + lh (the dropped ll is below 2^-22 of |x y|).  fp16
+// products are exact in FP32; the FP32 accumulator is cleared at every step
+// and its sum, times 1/s^2 (exact), added to FP64 accumulators, so rounding
+// grows with one step's sum, not with the whole chunk's.  (FP32
+// accumulation over a whole chunk leaves 1e-6 .. 1e-5, which the power
+// method's later components amplify past 1e-3 on some neighborhoods.)
 //
 // Each K step stages 32 columns -- the item's rows of each: T rows per
 // channel of the group for stage 1, the rows of bands A and B of the nf1*W
-// reduced vector for stage 2 -- in LDS as the three bf16 pieces laid out
+// reduced vector for stage 2 -- in LDS as the two fp16 pieces laid out
 // [sub-matrix][row][k]: thread o loads element o of the 32 columns
 // (coalesced across threads; the column bases are uniform) and writes each
 // piece of 8 columns as one 16-byte store, and an MFMA operand fragment
@@ -175,7 +174,7 @@ __global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
 // are not written, so the loads need no lane mask.  Waves own Gram tiles
 // (q, I, J).
 constexpr int kRowStride = 40;
-constexpr int kPieces = 3;
+constexpr int kPieces = 2;
 
 // rows allocated per sub-matrix: odd
 __host__ __device__ inline int gram_rows_alloc(int nrows) { return nrows | 1; }
@@ -183,7 +182,7 @@ __host__ __device__ inline int gram_rows_alloc(int nrows) { return nrows | 1; }
 template <int kStage>
 __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
                                                        PcaTables tb, double* __restrict__ part) {
-    extern __shared__ __bf16 gl[];
+    extern __shared__ _Float16 gl[];
     const int32_t* item = (kStage == 1 ? tb.gram1_items : tb.gram2_items) + 3 * blockIdx.x;
     const int g = item[0];
     const int chunk = item[1];
@@ -255,7 +254,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
 #pragma unroll
     for (int k = 0; k < kMaxTiles; ++k) acc[k] = f64x4{0.0, 0.0, 0.0, 0.0};
     const int bOff = pair ? nA : 0;  // local row of band B's first row
-    const bf16x8* pv = reinterpret_cast<const bf16x8*>(gl);
+    const f16x8* pv = reinterpret_cast<const f16x8*>(gl);
     const int lfrag = (lane & 15) * (kRowStride / 8) + (lane >> 4);
     const int cbeg = chunk * p.gram_chunk;
     int cend = cbeg + p.gram_chunk;
@@ -278,7 +277,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
         soff = o < nA ? A * bs + o : B * bs + (o - nA);
         srow = o;
     }
-    bf16x8* stg = reinterpret_cast<bf16x8*>(gl) + srow * (kRowStride / 8);
+    f16x8* stg = reinterpret_cast<f16x8*>(gl) + srow * (kRowStride / 8);
     // the next step's samples are loaded into registers before the current
     // step's MFMAs, so the load latency overlaps them
     float nxt[kKC];
@@ -288,6 +287,8 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
     // publish them a step later.  Indices are clamped into
     // the chunk (the sample is then replaced by zero), so loads need no branch.
     __shared__ int colbuf[kKC];
+    // per-wave max |x| of a step's samples, by step parity
+    __shared__ float wmax[2][kGramThreads / 64];
     const auto colidx = [&](int c) {
         c = c < cend ? c : cend - 1;
         return sub_cols ? idx[c] : c;
@@ -301,6 +302,19 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
             nxt[j] = cb + j < cend ? x : 0.0f;
         }
     };
+    // this wave's max |x| of the fetched step into wmax[par]
+    const auto publish_max = [&](int par) {
+        float m = 0.0f;
+        if (stager) {
+#pragma unroll
+            for (int j = 0; j < kKC; ++j) {
+                if (j >= 8 * ng) break;
+                m = fmaxf(m, fabsf(nxt[j]));
+            }
+        }
+        for (int o2 = 32; o2 > 0; o2 >>= 1) m = fmaxf(m, __shfl_xor(m, o2, 64));
+        if (lane == 0) wmax[par][wid] = m;
+    };
     int cnext = 0;
     if (cbeg < cend) {
         if (threadIdx.x < kKC) {
@@ -309,33 +323,37 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
         }
         __syncthreads();
         if (stager) fetch(cbeg);
+        publish_max(0);
     }
-    for (int c0 = cbeg; c0 < cend; c0 += kKC) {
-        __syncthreads();  // previous step's fragments and index reads consumed
+    int par = 0;
+    for (int c0 = cbeg; c0 < cend; c0 += kKC, par ^= 1) {
+        __syncthreads();  // previous step's fragments and index reads consumed; wmax[par] written
         if (threadIdx.x < kKC) colbuf[threadIdx.x] = cnext;  // indices of step c0 + kKC
+        // the step's scale 2^(14 - e), 2^e <= max |x| < 2^(e+1) (1 for an all-zero step)
+        float smax = 0.0f;
+        for (int w2 = 0; w2 < nw; ++w2) smax = fmaxf(smax, wmax[par][w2]);
+        const float sc = smax > 0.0f ? ldexpf(1.0f, 14 - ilogbf(smax)) : 1.0f;
+        const double unsc = 1.0 / (static_cast<double>(sc) * static_cast<double>(sc));
         if (stager) {
 #pragma unroll
             for (int g8 = 0; g8 < 4; ++g8) {
                 if (g8 >= ng) break;
-                bf16x8 h8, m8, l8;
+                f16x8 h8, l8;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const float x = nxt[8 * g8 + e];
-                    const __bf16 h = static_cast<__bf16>(x);
-                    const float r1 = x - static_cast<float>(h);  // exact
-                    const __bf16 m = static_cast<__bf16>(r1);
+                    const _Float16 h = static_cast<_Float16>(__builtin_fmaf(x, sc, 0.0f));
                     h8[e] = h;
-                    m8[e] = m;
-                    l8[e] = static_cast<__bf16>(r1 - static_cast<float>(m));  // exact
+                    l8[e] = static_cast<_Float16>(__builtin_fmaf(x, sc, -static_cast<float>(h)));
                 }
                 stg[kg0 + g8] = h8;
-                stg[plane / 8 + kg0 + g8] = m8;
-                stg[2 * (plane / 8) + kg0 + g8] = l8;
+                stg[plane / 8 + kg0 + g8] = l8;
             }
         }
         __syncthreads();
         if (threadIdx.x < kKC) cnext = colidx(c0 + 2 * kKC + threadIdx.x);
-        if (stager && c0 + kKC < cend) fetch(c0 + kKC);
+        const bool more = c0 + kKC < cend;
+        if (stager && more) fetch(c0 + kKC);
         // keep the per-tile fragment addresses inside the loop: hoisted out of
         // it they would exceed the registers and spill
         int lf = lfrag;
@@ -348,19 +366,18 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
                 const int q = tile[k] >> 16, I = (tile[k] >> 8) & 255, J = tile[k] & 255;
                 const int fa = lf + (q * ra + I * 16) * (kRowStride / 8);
                 const int fb = lf + (q * ra + bOff + J * 16) * (kRowStride / 8);
-                const bf16x8 ah = pv[fa], am = pv[plane / 8 + fa], al = pv[2 * (plane / 8) + fa];
-                const bf16x8 bh = pv[fb], bm = pv[plane / 8 + fb], bl = pv[2 * (plane / 8) + fb];
+                const f16x8 ah = pv[fa], al = pv[plane / 8 + fa];
+                const f16x8 bh = pv[fb], bl = pv[plane / 8 + fb];
                 f32x4 s = {0.f, 0.f, 0.f, 0.f};
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, s, 0, 0, 0);
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, s, 0, 0, 0);
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, s, 0, 0, 0);
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, s, 0, 0, 0);
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, s, 0, 0, 0);
-                s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, s, 0, 0, 0);
+                s = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, s, 0, 0, 0);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[k][r] += static_cast<double>(s[r]);
+                for (int r = 0; r < 4; ++r) acc[k][r] += static_cast<double>(s[r]) * unsc;
             }
         }
+        // the next step's max, once its samples have arrived (after the MFMAs)
+        if (more) publish_max(par ^ 1);
     }
     // this item's part of the chunk's partial Gram ([channel][row][row] for
     // stage 1, [row][row] for stage 2), each tile's transpose mirrored.
@@ -655,7 +672,7 @@ constexpr int64_t kMaxLds = 160 * 1024;
 // Stage-1 channels per work item: at most kGroupW, fewer when that many
 // channels' staged T-row sub-matrices would not fit the LDS.
 inline int32_t group_width(int32_t T, int32_t maxW) {
-    const int64_t per = static_cast<int64_t>(sizeof(__bf16)) * kPieces * gram_rows_alloc(T) * kRowStride;
+    const int64_t per = static_cast<int64_t>(sizeof(_Float16)) * kPieces * gram_rows_alloc(T) * kRowStride;
     int64_t gw = (kMaxLds - 2048) / per;
     if (gw > kGroupW) gw = kGroupW;
     if (gw > maxW) gw = maxW;
@@ -919,10 +936,10 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     const int n2 = nf1 * P->h.maxW;
     // stage-2 Gram items stage one band (up to 240 rows) or a pair of 128-row bands
     const int rows2 = n2 <= 240 ? n2 : 256;
-    P->lds_gram1 = sizeof(__bf16) * (kPieces * gw * gram_rows_alloc(T) + 16) * kRowStride;
+    P->lds_gram1 = sizeof(_Float16) * (kPieces * gw * gram_rows_alloc(T) + 16) * kRowStride;
     P->lds_power1 = sizeof(double) * (T * pow_ld(T) + 2 * T + 16);
     P->lds_proj1 = sizeof(float) * pw * T * cpad_for(nf1);
-    P->lds_gram2 = sizeof(__bf16) * (kPieces * gram_rows_alloc(rows2) + 16) * kRowStride;
+    P->lds_gram2 = sizeof(_Float16) * (kPieces * gram_rows_alloc(rows2) + 16) * kRowStride;
     P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * pow_ld(n2) : 0) + 2 * n2 + 16);
     P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, t.pe2));
     return MSAMD_OK;
