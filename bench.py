@@ -46,7 +46,7 @@ def neighborhoods(M, columns, pitch, radius):
 
 
 class Pipeline:
-    def __init__(self, dev, M, N, cfg, radius=40.0, num_threads=8, with_pca=True):
+    def __init__(self, dev, M, N, cfg, radius=40.0, num_threads=8, with_pca=True, fused=True):
         import numpy as np
         import torch
         from mountainsort_amd import frontend as fe
@@ -56,6 +56,8 @@ class Pipeline:
         self.M, self.N = M, N
         self.num_threads = num_threads
         self.with_pca = with_pca
+        # fused: the clip gather runs inside the stage-1 PCA kernels (no clip array)
+        self.fused = fused and with_pca
         nb = neighborhoods(M, cfg.columns, cfg.pitch, radius)
         self.nb_channels = torch.tensor(np.concatenate(nb), dtype=torch.int32, device=dev)
         self.nb_offsets = torch.tensor(np.concatenate([[0], np.cumsum([len(c) for c in nb])]),
@@ -68,8 +70,13 @@ class Pipeline:
         self.pca_tiles = None   # default tiles until tune_pca() has run
 
     def tune_pca(self):
-        """Autotune the PCA tiles on the last run's clips (the same shapes every step)."""
+        """Autotune the PCA tiles on the last run's events (the same shapes every step)."""
         import numpy as np
+        if self.fused:
+            self.pca_tiles, ms = self.fe.pca_autotune_timeseries(
+                self.white, self.last_times, self.last_offsets, self.nb_channels,
+                self.nb_offsets, 50)
+            return self.pca_tiles, ms
         off = self.last_offsets.cpu().numpy().astype(np.int64)
         L = off[1:] - off[:-1]
         self.pca_tiles, ms = self.fe.pca_autotune(self.last_clips, L, np.array(self.widths), 50)
@@ -102,22 +109,28 @@ class Pipeline:
         mark("apply")
         offsets, times = fe.detect_events(self.white, 3.0, 10, -1, num_threads=self.num_threads)
         mark("detect")
-        clips, clip_offsets = fe.extract_clips_batched(self.white, times, offsets, self.nb_channels,
-                                                       self.nb_offsets, 50)
-        mark("clips")
         feats = None
-        if self.with_pca:
-            feats = fe.pca_features(clips, clip_offsets, offsets, self.nb_offsets, 50,
-                                    tiles=self.pca_tiles)
-            mark("pca")
-        self.last_clips, self.last_offsets = clips, offsets
+        clips = None
+        if self.fused:
+            # clip gather fused into the stage-1 PCA kernels
+            feats = fe.pca_features_from_timeseries(self.white, times, offsets, self.nb_channels,
+                                                    self.nb_offsets, 50, tiles=self.pca_tiles)
+            mark("clips_pca")
+        else:
+            clips, clip_offsets = fe.extract_clips_batched(self.white, times, offsets,
+                                                           self.nb_channels, self.nb_offsets, 50)
+            mark("clips")
+            if self.with_pca:
+                feats = fe.pca_features(clips, clip_offsets, offsets, self.nb_offsets, 50,
+                                        tiles=self.pca_tiles)
+                mark("pca")
+        self.last_clips, self.last_offsets, self.last_times = clips, offsets, times
         if timers is not None:
             torch.cuda.synchronize()
             names = list(ev.keys())
             for a, b in zip(names[:-1], names[1:]):
                 timers.setdefault(b, []).append(ev[a].elapsed_time(ev[b]))
         self.stats["events"] = int(times.numel())
-        self.stats["clip_floats"] = int(clips.numel())
         return feats
 
 
@@ -232,9 +245,13 @@ def learned_leg(dev, pipe, cfg, reps=3, train_seconds=60.0, steps=4000):
         return off, t
 
     def features(off, t):
+        if pipe.fused:
+            fe_.pca_features_from_timeseries(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50,
+                                             tiles=pipe.pca_tiles)
+            return
         clips, co = fe_.extract_clips_batched(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50)
         if pipe.with_pca:
-            fe_.pca_features(clips, co, off, pipe.nb_offsets, 50)
+            fe_.pca_features(clips, co, off, pipe.nb_offsets, 50, tiles=pipe.pca_tiles)
 
     def timed(fn, with_features):
         ms_d, ms_all = [], []
@@ -308,6 +325,56 @@ def exact_whitening_leg(pipe, reps=3):
         "whitening_error_reference_method": float(np.linalg.norm(Wr @ Cn @ Wr - I) / np.sqrt(M)),
     }
     log("[bench] exact whitening: " + json.dumps(out))
+    return out
+
+
+def pca_fusion_leg(pipe, reps=3):
+    """Clips + PCA on the benchmark's events, unfused (clip array written by
+    extract_clips_batched, then the batched PCA; tiles autotuned on the
+    clips) against the fused form the timed steps run (clip gather inside
+    the stage-1 PCA kernels; tiles autotuned on the timeseries).  Also
+    checks that the two forms give bit-identical features at equal tiles."""
+    import numpy as np
+    import torch
+    from mountainsort_amd import frontend as fe
+    X, off, t = pipe.white, pipe.last_offsets, pipe.last_times
+    clips, co = fe.extract_clips_batched(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50)
+    eo = off.cpu().numpy().astype(np.int64)
+    tiles_u, _ = fe.pca_autotune(clips, eo[1:] - eo[:-1], np.array(pipe.widths), 50)
+    del clips
+    torch.cuda.empty_cache()
+
+    def unfused():
+        c, co_ = fe.extract_clips_batched(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50)
+        return fe.pca_features(c, co_, off, pipe.nb_offsets, 50, tiles=tiles_u)
+
+    def fused(tiles):
+        return fe.pca_features_from_timeseries(X, t, off, pipe.nb_channels, pipe.nb_offsets, 50,
+                                               tiles=tiles)
+
+    def timed(fn):
+        ms = []
+        for _ in range(reps + 1):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms[1:]))
+
+    ms_u = timed(unfused)
+    ms_f = timed(lambda: fused(pipe.pca_tiles))
+    r1, _, f1, _ = unfused()
+    r1, f1 = r1.clone(), f1.clone()
+    r2, _, f2, _ = fused(tiles_u)
+    same = bool(torch.equal(r1, r2) and torch.equal(f1, f2))
+    out = {"events": int(t.numel()), "tiles_unfused": list(tiles_u),
+           "tiles_fused": list(pipe.pca_tiles) if pipe.pca_tiles else None,
+           "ms": {"unfused_clips_plus_pca": round(ms_u, 3), "fused": round(ms_f, 3)},
+           "features_identical_at_equal_tiles": same}
+    log("[bench] pca fusion: " + json.dumps(out))
     return out
 
 
@@ -617,7 +684,9 @@ def main():
                     help="instead of the device-resident step: time the processor chain "
                          ".mda file to file in DIR (config --f2f-config, --seconds long)")
     ap.add_argument("--f2f-config", choices=["1", "2"], default="2")
-    ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS),
+    ap.add_argument("--unfused", action="store_true",
+                    help="time clips and PCA as two stages (clip array written, then read)")
+    ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS) + ["pca"],
                     help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
                          "elementwise negation (counter calibration) and the named stage, "
                          "three times each")
@@ -643,7 +712,7 @@ def main():
     raw = synth.generate(cfg, device=dev)
     torch.cuda.synchronize()
     log(f"[bench] generated {cfg.M} x {N} int16 in {time.time() - t0:.1f}s")
-    pipe = Pipeline(dev, cfg.M, N, cfg)
+    pipe = Pipeline(dev, cfg.M, N, cfg, fused=not args.unfused)
     if args.pmc_pass:
         from mountainsort_amd import frontend as fe
         pipe.run(raw)
@@ -659,6 +728,10 @@ def main():
                 fe.whiten_apply(pipe.filt, pipe.W, out=pipe.white, row_keep=pipe.rows, xxt=pipe.C)
             elif args.pmc_pass == "covariance":
                 fe.whiten_covariance(pipe.filt, row_keep=pipe.rows)
+            elif args.pmc_pass == "pca":
+                fe.pca_features_from_timeseries(pipe.white, pipe.last_times, pipe.last_offsets,
+                                                pipe.nb_channels, pipe.nb_offsets, 50,
+                                                tiles=(2560, 32, 32))
             else:
                 fe.whitening_matrix(pipe.C)
         torch.cuda.synchronize()
@@ -743,6 +816,8 @@ def main():
     if not args.no_learned:
         line["learned"] = learned_leg(dev, pipe, cfg)
     line["whitening_exact"] = exact_whitening_leg(pipe)
+    if pipe.fused:
+        line["pca_fusion"] = pca_fusion_leg(pipe)
     if not args.no_pca_batching:
         line["pca_batching"] = pca_batching_leg(dev)
     if not args.no_cpu_baseline:

@@ -186,9 +186,10 @@ int64_t msamd_pca_workspace_size(int32_t G, const int64_t* num_events_host,
  * components; features receives nf2 x L per neighborhood, packed (the
  * isosplit5 input).  Both PCAs use the pca_subsampled column rule with
  * max_pca_samples, the 10-step power method with deflation from
- * normalize(sin(1..n)) (FP64), Gram matrices from exact three-piece BF16
- * MFMA products (FP32 within 32-column steps, FP64 across them: within
- * ~1e-8 of the FP64 Gram), and FP32 projections over all columns. */
+ * normalize(sin(1..n)) (FP64), Gram matrices from two-piece FP16 MFMA
+ * products after a power-of-two scale per 32-column step (hh + hl + lh, 22
+ * significant bits; FP32 within a step, FP64 across steps), and FP32
+ * projections over all columns. */
 int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_host,
                        const int32_t* widths_host, int32_t T, int32_t num_features_per_channel,
                        int32_t num_features, int64_t max_pca_samples, float* reduced,
@@ -203,7 +204,12 @@ int msamd_pca_features(const float* clips, int32_t G, const int64_t* num_events_
  * hipGraphLaunch on the given stream.  tiles (nullable) = {Gram columns per
  * work item (multiple of 32), stage-1 events per projection item, stage-2
  * events per projection item}; null selects {2560, 64, 64}.  The plan keeps
- * pointers into the workspace, which must outlive it.  No reference
+ * pointers into the workspace, which must outlive it.  With workspace NULL
+ * and workspace_bytes 0, msamd_pca_plan_create builds the host tables only
+ * (the tables are built once, not once for the size query and once for the
+ * plan): msamd_pca_plan_workspace_bytes gives the workspace it needs and
+ * msamd_pca_plan_attach uploads the tables into one (it may be called again
+ * with another workspace); runs are refused until then.  No reference
  * counterpart: the reference computes each neighborhood's PCA serially on
  * the CPU (neighborhoodsorter.cpp:236-276); callers that process many
  * recordings of one probe reuse a plan. */
@@ -222,6 +228,26 @@ int msamd_pca_plan_launch_graph(msamd_pca_plan* plan, const float* clips, float*
                                 float* features, void* stream);
 void msamd_pca_plan_tiles(const msamd_pca_plan* plan, int32_t* tiles);
 void msamd_pca_plan_destroy(msamd_pca_plan* plan);
+int64_t msamd_pca_plan_workspace_bytes(const msamd_pca_plan* plan);
+int msamd_pca_plan_attach(msamd_pca_plan* plan, void* workspace, int64_t workspace_bytes,
+                          void* stream);
+
+/* The plan's computation with the clip gather fused into the stage-1
+ * kernels: no clip array is written or read.  The stage-1 Gram and
+ * projection read each clip's samples from the timeseries X (M x N,
+ * x[m + M*t]) as msamd_extract_clips_batched would write them: neighborhood
+ * g's events are times[ev_g .. ev_g + num_events_host[g]) (centers, packed
+ * neighborhood after neighborhood, ev_g the sum of the earlier counts), its
+ * channels nbhd_channels[nbhd_offsets[g] .. nbhd_offsets[g+1]) (0-based,
+ * central first; the counts must equal the plan's widths), clip start time -
+ * Tmid, zero outside [0, N).  reduced and features are bit-identical to
+ * msamd_pca_plan_run on the gathered clips.  The replaced pair is get_clip
+ * (neighborhoodsorter.cpp:409-419) feeding dimension_reduce_clips
+ * (neighborhoodsorter.cpp:236-276). */
+int msamd_pca_plan_run_timeseries(msamd_pca_plan* plan, const float* X, int64_t M, int64_t N,
+                                  const int64_t* times, const int32_t* nbhd_channels,
+                                  const int64_t* nbhd_offsets, float* reduced, float* features,
+                                  void* stream);
 
 /* Autotuner of the plan's tiles for one workload: a coordinate search over
  * Gram columns per item {1280, 2560, 5120, 10240}, stage-1 events per item
@@ -240,6 +266,15 @@ int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_
                        int32_t num_features, int64_t max_pca_samples, float* reduced,
                        float* features, void* workspace, int64_t workspace_bytes, void* stream,
                        int32_t* tiles_out, float* best_ms_out);
+/* The same search for the fused form (msamd_pca_plan_run_timeseries). */
+int msamd_pca_autotune_timeseries(const float* X, int64_t M, int64_t N, const int64_t* times,
+                                  const int32_t* nbhd_channels, const int64_t* nbhd_offsets,
+                                  int32_t G, const int64_t* num_events_host,
+                                  const int32_t* widths_host, int32_t T,
+                                  int32_t num_features_per_channel, int32_t num_features,
+                                  int64_t max_pca_samples, float* reduced, float* features,
+                                  void* workspace, int64_t workspace_bytes, void* stream,
+                                  int32_t* tiles_out, float* best_ms_out);
 
 /* PCA of index subsets, the PCA of sort_clips_subset
  * (packages/mountainsortalg/sort_clips.cpp:47-70) that the branch method

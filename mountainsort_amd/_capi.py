@@ -42,13 +42,18 @@ SIGNATURES = {
     "msamd_pca_plan_create": (cint, [i32, vp, vp, i32, i32, i32, i64, vp, vp, i64, vp, vp]),
     "msamd_pca_plan_run": (cint, [vp, vp, vp, vp, vp]),
     "msamd_pca_plan_launch_graph": (cint, [vp, vp, vp, vp, vp]),
+    "msamd_pca_plan_run_timeseries": (cint, [vp, vp, i64, i64, vp, vp, vp, vp, vp, vp]),
     "msamd_pca_plan_tiles": (None, [vp, vp]),
     "msamd_pca_plan_destroy": (None, [vp]),
+    "msamd_pca_plan_workspace_bytes": (i64, [vp]),
+    "msamd_pca_plan_attach": (cint, [vp, vp, i64, vp]),
     "msamd_pca_subsets_workspace_size": (i64, [i32, vp, i32, i32, i64]),
     "msamd_pca_subsets": (cint, [vp, i32, i64, vp, vp, i32, i32, i64, vp, vp, i64, vp]),
     "msamd_pca_autotune_workspace_size": (i64, [i32, vp, vp, i32, i32, i32, i64]),
     "msamd_pca_autotune": (cint, [vp, i32, vp, vp, i32, i32, i32, i64, vp, vp, vp, i64, vp, vp,
                                   vp]),
+    "msamd_pca_autotune_timeseries": (cint, [vp, i64, i64, vp, vp, vp, i32, vp, vp, i32, i32,
+                                             i32, i64, vp, vp, vp, i64, vp, vp, vp]),
     "msamd_whiten_covariance_workspace_size": (i64, [i64, i64]),
     "msamd_whiten_covariance": (cint, [vp, i64, i64, vp, vp, i64, vp]),
     "msamd_whiten_covariance_rows": (cint, [vp, i64, i64, vp, vp, vp, i64, vp]),

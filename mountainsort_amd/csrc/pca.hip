@@ -10,8 +10,9 @@
 // stage 2, per g: R = reduced clips viewed as (nf1*W) x L.  Each PCA:
 //   columns: all L, or the pca_subsampled indices (int64)i for
 //            i = 0, inc, 2inc, ... < L, inc = L/max (double accumulation)
-//   Gram:    G = A_s A_s^T  -- three-piece BF16 MFMA (6 product terms, exact
-//            products), FP32 within a 32-column step, FP64 across steps
+//   Gram:    G = A_s A_s^T  -- two-piece FP16 MFMA after a power-of-two scale
+//            per 32-column step (3 product terms), FP32 within a step, FP64
+//            across steps
 //   power:   per component k < K: v = normalize(sin(1..n)); 10 times
 //            v <- G v / |G v| (sigma = |G v|); deflate G <- (I-vv')G(I-vv')
 //            (FP64; the reference iterates X X^T v on the data, equal in
@@ -19,8 +20,10 @@
 //   project: F = C^T A on all L columns (FP32 FMA).
 // Phases run as one launch each over every neighborhood (items sized for
 // load balance: Gram and power per neighborhood / channel, projections per
-// chunk of events).
+// chunk of events).  The stage-1 Gram and projection read either a clip
+// array or, fused with the clip gather, the timeseries itself (GatherSrc).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -100,6 +103,20 @@ struct PcaTables {
     const int32_t* gram_first;   // [G+1] first chunk of each neighborhood
     const int64_t* gp1_off;      // [nchunks] floats: stage-1 partial of each chunk
     const int64_t* gp2_off;      // [nchunks] floats: stage-2 partial of each chunk
+    const int64_t* ev_off;       // [G+1] first event of each neighborhood (CSR of L)
+};
+
+// Stage-1 input read straight from the timeseries instead of a clip array:
+// clip element (w, t, i) of neighborhood g is X[ch_w + M (time_i - tmid + t)],
+// zero outside [0, N) -- the same values extract_clips_batched writes, so
+// the features are bit-identical to those of the clip form.
+struct GatherSrc {
+    const float* X;          // M x N, x[m + M t]
+    int64_t M, N;
+    const int64_t* times;    // event centres, neighborhood after neighborhood (ev_off)
+    const int32_t* nc;       // neighborhood channels, 0-based, central first
+    const int64_t* no;       // [G+1] channel offsets
+    int32_t tmid;            // (T + 1) / 2 - 1
 };
 
 // ----------------------------------------------------- subsample indices
@@ -179,12 +196,15 @@ constexpr int kPieces = 2;
 // rows allocated per sub-matrix: odd
 __host__ __device__ inline int gram_rows_alloc(int nrows) { return nrows | 1; }
 
-template <int kStage>
+template <int kStage, bool kGather = false>
 __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __restrict__ src, PcaPlan p,
-                                                       PcaTables tb, double* __restrict__ part) {
+                                                       PcaTables tb, double* __restrict__ part,
+                                                       GatherSrc gs) {
+    static_assert(kStage == 1 || !kGather, "only stage 1 reads the timeseries");
     extern __shared__ _Float16 gl[];
     const int32_t* item = (kStage == 1 ? tb.gram1_items : tb.gram2_items) + 3 * blockIdx.x;
     const int g = item[0];
+    if (g < 0) return;  // padding item (xcd_order)
     const int chunk = item[1];
     const int sub = item[2];
     const int L = tb.L[g];
@@ -211,7 +231,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
     }
     const int Sfull = kStage == 1 ? W * p.T : p.nf1 * W;  // floats per source column
     const int ra = gram_rows_alloc(nrows);
-    const float* base = src + (kStage == 1 ? tb.clip_off[g] : tb.red_off[g]);
+    const float* base = kGather ? nullptr : src + (kStage == 1 ? tb.clip_off[g] : tb.red_off[g]);
     const int32_t* idx = tb.idx + g * p.idx_stride;
     const bool sub_cols = L > p.max_s;
     // tiles: stage 1 and stage 2 with A == B: J >= I over nb row blocks;
@@ -268,11 +288,19 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
     const int kg0 = (threadIdx.x / S) * ng;
     const bool stager = kg0 < 4;
     int soff = 0, srow = 0;
+    // gathering: the slot's channel (as an offset into a sample row) and clip row
+    int64_t gch = 0;
+    const int64_t* gtimes = nullptr;
     if (kStage == 1) {
         const int q = o % nsub;
         const int r = o / nsub;
         soff = r * W + q0 + q;
         srow = q * ra + r;
+        if (kGather) {
+            gch = gs.nc[gs.no[g] + q0 + q];
+            soff = r;
+            gtimes = gs.times + tb.ev_off[g];
+        }
     } else {
         soff = o < nA ? A * bs + o : B * bs + (o - nA);
         srow = o;
@@ -286,19 +314,30 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
     // prefetch, after the split, so no wait before a barrier covers it) and
     // publish them a step later.  Indices are clamped into
     // the chunk (the sample is then replaced by zero), so loads need no branch.
-    __shared__ int colbuf[kKC];
+    // (When gathering, colbuf holds each column's first sample time instead.)
+    __shared__ int64_t colbuf[kKC];
     // per-wave max |x| of a step's samples, by step parity
     __shared__ float wmax[2][kGramThreads / 64];
-    const auto colidx = [&](int c) {
+    const auto colidx = [&](int c) -> int64_t {
         c = c < cend ? c : cend - 1;
-        return sub_cols ? idx[c] : c;
+        const int col = sub_cols ? idx[c] : c;
+        if (kGather) return gtimes[col] - gs.tmid;
+        return col;
     };
     const auto fetch = [&](int c0) {
         const int cb = c0 + 8 * kg0;
 #pragma unroll
         for (int j = 0; j < kKC; ++j) {
             if (j >= 8 * ng) break;
-            const float x = base[static_cast<int64_t>(colbuf[8 * kg0 + j]) * Sfull + soff];
+            float x;
+            if (kGather) {
+                const int64_t tt = colbuf[8 * kg0 + j] + soff;
+                const bool in = tt >= 0 && tt < gs.N;
+                x = gs.X[(in ? tt : 0) * gs.M + gch];
+                x = in ? x : 0.0f;
+            } else {
+                x = base[colbuf[8 * kg0 + j] * Sfull + soff];
+            }
             nxt[j] = cb + j < cend ? x : 0.0f;
         }
     };
@@ -315,7 +354,7 @@ __global__ void __launch_bounds__(kGramThreads) pca_gram_kernel(const float* __r
         for (int o2 = 32; o2 > 0; o2 >>= 1) m = fmaxf(m, __shfl_xor(m, o2, 64));
         if (lane == 0) wmax[par][wid] = m;
     };
-    int cnext = 0;
+    int64_t cnext = 0;
     if (cbeg < cend) {
         if (threadIdx.x < kKC) {
             colbuf[threadIdx.x] = colidx(cbeg + threadIdx.x);
@@ -541,12 +580,14 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
 // nf1 <= 12 (the default 10 features), else 16.
 inline int cpad_for(int nf1) { return nf1 <= 12 ? 12 : 16; }
 
-template <int kCPad>
+template <int kCPad, bool kGather = false>
 __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restrict__ clips,
                                                            PcaPlan p, PcaTables tb,
-                                                           float* __restrict__ reduced) {
+                                                           float* __restrict__ reduced,
+                                                           GatherSrc gs) {
     extern __shared__ float sp[];
     const int g = tb.j1_items[3 * blockIdx.x];
+    if (g < 0) return;  // padding item (xcd_order)
     const int e0 = tb.j1_items[3 * blockIdx.x + 1];
     const int q0 = tb.j1_items[3 * blockIdx.x + 2];
     const int L = tb.L[g];
@@ -563,7 +604,7 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
         Cs[e] = a < nf ? cg[wt * nf + a] : 0.0f;
     }
     __syncthreads();
-    const float* xg = clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T + q0;
+    const float* xg = kGather ? nullptr : clips + tb.clip_off[g] + static_cast<int64_t>(e0) * W * T + q0;
     // thread (event pair, channel): the two events share every component read
     const int np = (ne + 1) / 2;
     for (int o = threadIdx.x; o < np * Wg; o += blockDim.x) {
@@ -573,8 +614,34 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
         float acc0[kCPad], acc1[kCPad];
 #pragma unroll
         for (int a = 0; a < kCPad; ++a) acc0[a] = acc1[a] = 0.0f;
-        const float* x0 = xg + static_cast<int64_t>(ev) * W * T + wc;
-        const float* x1 = two ? x0 + W * T : x0;
+        // sample t of the two clips: x0[t * xs], x1[t * xs]; when gathering
+        // from the timeseries, samples outside [0, N) read as zero (clips near
+        // the recording's ends take the bounds-checked loads)
+        const float* x0;
+        const float* x1;
+        int64_t xs = W;
+        int64_t ta = 0, tb2 = 0;
+        bool edge = false;
+        if (kGather) {
+            const int64_t* te = gs.times + tb.ev_off[g] + e0;
+            const int64_t ch = gs.nc[gs.no[g] + q0 + wc];
+            ta = te[ev] - gs.tmid;
+            tb2 = two ? te[ev + 1] - gs.tmid : ta;
+            edge = ta < 0 || tb2 < 0 || ta + T > gs.N || tb2 + T > gs.N;
+            xs = gs.M;
+            x0 = gs.X + ch + (edge ? 0 : ta * gs.M);
+            x1 = gs.X + ch + (edge ? 0 : tb2 * gs.M);
+        } else {
+            x0 = xg + static_cast<int64_t>(ev) * W * T + wc;
+            x1 = two ? x0 + W * T : x0;
+        }
+        const auto ld = [&](const float* xp, int64_t t0c, int t) -> float {
+            if (!kGather || !edge) return xp[t * xs];
+            const int64_t tt = t0c + t;
+            const bool in = tt >= 0 && tt < gs.N;
+            const float v = xp[(in ? tt : 0) * xs];
+            return in ? v : 0.0f;
+        };
         const float* c = Cs + wc * T * kCPad;
         constexpr int U = 10;
         int t = 0;
@@ -582,8 +649,8 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
             float xa[U], xb[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                xa[u] = x0[(t + u) * W];
-                xb[u] = x1[(t + u) * W];
+                xa[u] = ld(x0, ta, t + u);
+                xb[u] = ld(x1, tb2, t + u);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -603,8 +670,8 @@ __global__ void __launch_bounds__(256) pca_project1_kernel(const float* __restri
             }
         }
         for (; t < T; ++t) {
-            const float xa = x0[t * W];
-            const float xb = x1[t * W];
+            const float xa = ld(x0, ta, t);
+            const float xb = ld(x1, tb2, t);
 #pragma unroll
             for (int a = 0; a < kCPad; ++a) {
                 acc0[a] = __builtin_fmaf(c[t * kCPad + a], xa, acc0[a]);
@@ -656,7 +723,7 @@ __global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restri
 struct HostPlan {
     std::vector<int64_t> clip_off, red_off, feat_off, g1_off, g2_off, c1_off, c2_off;
     std::vector<int32_t> L, W, p1, j1, j2;
-    std::vector<int64_t> w2_off, gp1_off, gp2_off;
+    std::vector<int64_t> w2_off, gp1_off, gp2_off, ev_off;
     std::vector<int32_t> gram1_items, gram2_items, gram_first;
     int64_t g1_total = 0, g2_total = 0, c1_total = 0, c2_total = 0, w2_total = 0;
     int64_t gp1_total = 0, gp2_total = 0;
@@ -679,6 +746,64 @@ inline int32_t group_width(int32_t T, int32_t maxW) {
     return static_cast<int32_t>(gw < 1 ? 1 : gw);
 }
 
+// Reorders stage-1 work items (3 ints each, neighborhood first) for the
+// last-level caches of the eight XCDs, to which workgroups are dispatched
+// round-robin: the neighborhoods are split into eight contiguous ranges of
+// about equal item counts, range k's items go to the block indices that
+// land on XCD k, and each range's items run in the order of pos (the
+// relative position of the item's events in its neighborhood's time-sorted
+// list, in 1024 buckets, stable).  The workgroups that run together on an
+// XCD then read clips of nearby channels at nearby times.  Ranges are padded
+// to one length with items whose neighborhood is -1 (the kernels return at
+// once).  Items write disjoint outputs (the Gram chunk partials are summed
+// in chunk order), so the order changes no result.  Linear time: the plan
+// is built on the host for every batch.
+void xcd_order(std::vector<int32_t>& items, const std::vector<float>& pos, int32_t G) {
+    constexpr int kXcd = 8;
+    constexpr int kBuckets = 1024;
+    const size_t n = items.size() / 3;
+    if (n < 4 * kXcd) return;
+    std::vector<int64_t> cnt(G, 0);
+    for (size_t i = 0; i < n; ++i) ++cnt[items[3 * i]];
+    std::vector<int32_t> xcd_of(G);
+    int64_t acc = 0;
+    for (int32_t g = 0; g < G; ++g) {
+        const int64_t k = (2 * acc + cnt[g]) * kXcd / (2 * static_cast<int64_t>(n));
+        xcd_of[g] = static_cast<int32_t>(k < kXcd - 1 ? k : kXcd - 1);
+        acc += cnt[g];
+    }
+    // counting sort by (XCD, bucket of pos), stable
+    std::vector<int64_t> start(kXcd * kBuckets + 1, 0);
+    std::vector<int32_t> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        int b = static_cast<int>(pos[i] * kBuckets);
+        b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+        key[i] = xcd_of[items[3 * i]] * kBuckets + b;
+        ++start[key[i] + 1];
+    }
+    for (int k = 0; k < kXcd * kBuckets; ++k) start[k + 1] += start[k];
+    size_t len = 0;
+    for (int k = 0; k < kXcd; ++k) {
+        const size_t lk = static_cast<size_t>(start[(k + 1) * kBuckets] - start[k * kBuckets]);
+        len = lk > len ? lk : len;
+    }
+    std::vector<int32_t> out(3 * kXcd * len);
+    for (size_t j = 0; j < kXcd * len; ++j) {
+        out[3 * j] = -1;
+        out[3 * j + 1] = out[3 * j + 2] = 0;
+    }
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (size_t i = 0; i < n; ++i) {
+        const int k = key[i] / kBuckets;
+        const int64_t r = fill[key[i]]++ - start[k * kBuckets];  // rank within XCD k
+        int32_t* d = &out[3 * (kXcd * static_cast<size_t>(r) + k)];
+        d[0] = items[3 * i];
+        d[1] = items[3 * i + 1];
+        d[2] = items[3 * i + 2];
+    }
+    items.swap(out);
+}
+
 HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, int32_t T,
                    int32_t nf1, int32_t nf2, int64_t max_s, const Tiles& tl, int32_t gw,
                    int32_t pw) {
@@ -686,14 +811,25 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
     int32_t maxW = 0;
     for (int32_t g = 0; g < G; ++g) maxW = widths[g] > maxW ? widths[g] : maxW;
     const bool w2_global = static_cast<int64_t>(nf1) * maxW > kPowMaxN;
-    int64_t co = 0, ro = 0, fo = 0;
+    int64_t co = 0, ro = 0, fo = 0, eo = 0;
     int32_t nchunks = 0;
+    {
+        int64_t n1 = 0, n2 = 0;
+        for (int32_t g = 0; g < G; ++g) {
+            n1 += ceil_div(num_events[g], static_cast<int64_t>(tl.pe1)) * ceil_div(widths[g], pw);
+            n2 += ceil_div(num_events[g], static_cast<int64_t>(tl.pe2));
+        }
+        h.j1.reserve(3 * n1);
+        h.j2.reserve(2 * n2);
+    }
     for (int32_t g = 0; g < G; ++g) {
         const int64_t L = num_events[g];
         const int32_t W = widths[g];
         h.L.push_back(static_cast<int32_t>(L));
         h.W.push_back(W);
         h.maxW = W > h.maxW ? W : h.maxW;
+        h.ev_off.push_back(eo);
+        eo += L;
         h.clip_off.push_back(co);
         h.red_off.push_back(ro);
         h.feat_off.push_back(fo);
@@ -751,6 +887,22 @@ HostPlan host_plan(int32_t G, const int64_t* num_events, const int32_t* widths, 
         }
     }
     h.gram_first.push_back(nchunks);
+    h.ev_off.push_back(eo);
+    {
+        std::vector<float> pos(h.gram1_items.size() / 3);
+        for (size_t i = 0; i < pos.size(); ++i) {
+            const int32_t g = h.gram1_items[3 * i];
+            const int32_t nch = h.gram_first[g + 1] - h.gram_first[g];
+            pos[i] = static_cast<float>(h.gram1_items[3 * i + 1]) / static_cast<float>(nch);
+        }
+        xcd_order(h.gram1_items, pos, G);
+        pos.assign(h.j1.size() / 3, 0.0f);
+        for (size_t i = 0; i < pos.size(); ++i) {
+            const int32_t g = h.j1[3 * i];
+            pos[i] = static_cast<float>(h.j1[3 * i + 1]) / static_cast<float>(h.L[g]);
+        }
+        xcd_order(h.j1, pos, G);
+    }
     return h;
 }
 
@@ -789,7 +941,9 @@ struct PcaLayout {
     int32_t* d_gram_first = nullptr;
     int64_t* d_gp1_off = nullptr;
     int64_t* d_gp2_off = nullptr;
+    int64_t* d_ev_off = nullptr;
     double* part = nullptr;
+    int64_t table_bytes = 0;
 };
 
 // Room for the subsample list of one neighborhood (at most max_s + 1
@@ -813,12 +967,14 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     l.d_j2 = take_copy(c, h.j2);
     l.d_sin = c.take<double>(kSinTab);
     l.d_w2_off = take_copy(c, h.w2_off);
-    double* work2 = c.take<double>(h.w2_total > 0 ? h.w2_total : 1);
     l.d_gram1_items = take_copy(c, h.gram1_items);
     l.d_gram2_items = take_copy(c, h.gram2_items);
     l.d_gram_first = take_copy(c, h.gram_first);
     l.d_gp1_off = take_copy(c, h.gp1_off);
     l.d_gp2_off = take_copy(c, h.gp2_off);
+    l.d_ev_off = take_copy(c, h.ev_off);
+    l.table_bytes = c.bytes();  // the host tables above are uploaded as one image
+    double* work2 = c.take<double>(h.w2_total > 0 ? h.w2_total : 1);
     l.part = c.take<double>(std::max<int64_t>(std::max(h.gp1_total, h.gp2_total), 1));
     int32_t* ns = c.take<int32_t>(G);
     int32_t* idx = c.take<int32_t>(static_cast<int64_t>(G) * idx_stride);
@@ -830,7 +986,7 @@ PcaLayout pca_layout(void* ws, const HostPlan& h, int32_t G, int64_t idx_stride)
     l.tb = PcaTables{l.d_clip_off, l.d_red_off, l.d_feat_off, l.d_L, l.d_W, l.d_g1_off,
                      l.d_g2_off, l.d_c1_off, l.d_c2_off, l.d_p1, l.d_j1, l.d_j2, ns, idx,
                      g1, g2, c1, c2, l.d_sin, work2, l.d_w2_off, l.d_gram1_items,
-                     l.d_gram2_items, l.d_gram_first, l.d_gp1_off, l.d_gp2_off};
+                     l.d_gram2_items, l.d_gram_first, l.d_gp1_off, l.d_gp2_off, l.d_ev_off};
     return l;
 }
 
@@ -911,6 +1067,8 @@ struct msamd_pca_plan {
     float* g_features = nullptr;
     // the stage-2 PCA alone, on vectors held where the reduced clips would be
     bool vectors = false;
+    // tables uploaded into a workspace (false after a deferred create)
+    bool attached = false;
 };
 
 namespace {
@@ -961,37 +1119,47 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
         for (int i = 0; i < kSinTab; ++i) v[i] = std::sin(static_cast<double>(i + 1));
         return v;
     }();
-    auto up = [&](void* dst, const void* src, size_t n) {
-        return n ? hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st) : hipSuccess;
+    // one host image of every table at its workspace offset, one copy
+    std::vector<char> img(static_cast<size_t>(l.table_bytes));
+    char* const wsb = static_cast<char*>(workspace);
+    auto put = [&](const void* dst, const void* src, size_t n) {
+        if (n) std::memcpy(img.data() + (static_cast<const char*>(dst) - wsb), src, n);
     };
-    MSAMD_HIP(up(l.d_clip_off, h.clip_off.data(), h.clip_off.size() * 8));
-    MSAMD_HIP(up(l.d_red_off, h.red_off.data(), h.red_off.size() * 8));
-    MSAMD_HIP(up(l.d_feat_off, h.feat_off.data(), h.feat_off.size() * 8));
-    MSAMD_HIP(up(l.d_L, h.L.data(), h.L.size() * 4));
-    MSAMD_HIP(up(l.d_W, h.W.data(), h.W.size() * 4));
-    MSAMD_HIP(up(l.d_g1_off, h.g1_off.data(), h.g1_off.size() * 8));
-    MSAMD_HIP(up(l.d_g2_off, h.g2_off.data(), h.g2_off.size() * 8));
-    MSAMD_HIP(up(l.d_c1_off, h.c1_off.data(), h.c1_off.size() * 8));
-    MSAMD_HIP(up(l.d_c2_off, h.c2_off.data(), h.c2_off.size() * 8));
-    MSAMD_HIP(up(l.d_p1, h.p1.data(), h.p1.size() * 4));
-    MSAMD_HIP(up(l.d_j1, h.j1.data(), h.j1.size() * 4));
-    MSAMD_HIP(up(l.d_j2, h.j2.data(), h.j2.size() * 4));
-    MSAMD_HIP(up(l.d_sin, sins.data(), sins.size() * sizeof(double)));
-    MSAMD_HIP(up(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8));
-    MSAMD_HIP(up(l.d_gram1_items, h.gram1_items.data(), h.gram1_items.size() * 4));
-    MSAMD_HIP(up(l.d_gram2_items, h.gram2_items.data(), h.gram2_items.size() * 4));
-    MSAMD_HIP(up(l.d_gram_first, h.gram_first.data(), h.gram_first.size() * 4));
-    MSAMD_HIP(up(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8));
-    MSAMD_HIP(up(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8));
-    MSAMD_HIP(hipStreamSynchronize(st));  // pageable sources: copies complete before return
+    put(l.d_clip_off, h.clip_off.data(), h.clip_off.size() * 8);
+    put(l.d_red_off, h.red_off.data(), h.red_off.size() * 8);
+    put(l.d_feat_off, h.feat_off.data(), h.feat_off.size() * 8);
+    put(l.d_L, h.L.data(), h.L.size() * 4);
+    put(l.d_W, h.W.data(), h.W.size() * 4);
+    put(l.d_g1_off, h.g1_off.data(), h.g1_off.size() * 8);
+    put(l.d_g2_off, h.g2_off.data(), h.g2_off.size() * 8);
+    put(l.d_c1_off, h.c1_off.data(), h.c1_off.size() * 8);
+    put(l.d_c2_off, h.c2_off.data(), h.c2_off.size() * 8);
+    put(l.d_p1, h.p1.data(), h.p1.size() * 4);
+    put(l.d_j1, h.j1.data(), h.j1.size() * 4);
+    put(l.d_j2, h.j2.data(), h.j2.size() * 4);
+    put(l.d_sin, sins.data(), sins.size() * sizeof(double));
+    put(l.d_w2_off, h.w2_off.data(), h.w2_off.size() * 8);
+    put(l.d_gram1_items, h.gram1_items.data(), h.gram1_items.size() * 4);
+    put(l.d_gram2_items, h.gram2_items.data(), h.gram2_items.size() * 4);
+    put(l.d_gram_first, h.gram_first.data(), h.gram_first.size() * 4);
+    put(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8);
+    put(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8);
+    put(l.d_ev_off, h.ev_off.data(), h.ev_off.size() * 8);
+    MSAMD_HIP(hipMemcpyAsync(workspace, img.data(), img.size(), hipMemcpyHostToDevice, st));
+    MSAMD_HIP(hipStreamSynchronize(st));  // pageable source: the copy completes before return
     auto lds_attr = [](const void* f, size_t bytes) {
         return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(bytes));
     };
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<1>), P->lds_gram1));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<1, true>), P->lds_gram1));
     MSAMD_HIP(lds_attr(cpad_for(P->p.nf1) == 12
                            ? reinterpret_cast<const void*>(&pca_project1_kernel<12>)
                            : reinterpret_cast<const void*>(&pca_project1_kernel<16>),
+                       P->lds_proj1));
+    MSAMD_HIP(lds_attr(cpad_for(P->p.nf1) == 12
+                           ? reinterpret_cast<const void*>(&pca_project1_kernel<12, true>)
+                           : reinterpret_cast<const void*>(&pca_project1_kernel<16, true>),
                        P->lds_proj1));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<2>), P->lds_gram2));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_power_kernel<2>), P->lds_power2));
@@ -1007,7 +1175,8 @@ int enqueue_stage2(const msamd_pca_plan* P, float* reduced, float* features, hip
     const int32_t G = p.G, nf1 = p.nf1;
     // stage 2 Gram
     const unsigned ng2 = static_cast<unsigned>(h.gram2_items.size() / 3);
-    pca_gram_kernel<2><<<ng2, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part);
+    pca_gram_kernel<2><<<ng2, kGramThreads, P->lds_gram2, st>>>(reduced, p, l.tb, l.part,
+                                                               GatherSrc{});
     MSAMD_LAUNCH_CHECK();
     {
         const int R = nf1 * h.maxW;
@@ -1029,8 +1198,10 @@ int enqueue_stage2(const msamd_pca_plan* P, float* reduced, float* features, hip
 
 
 // The launch sequence: nine kernels, each one launch over every neighborhood.
+// With gs, the stage-1 kernels read the clips' samples from the timeseries
+// (clips is then unused).
 int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, float* features,
-                 hipStream_t st) {
+                 hipStream_t st, const GatherSrc* gs = nullptr) {
     const HostPlan& h = P->h;
     const PcaPlan& p = P->p;
     const PcaLayout& l = P->l;
@@ -1040,7 +1211,12 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     if (P->vectors) return enqueue_stage2(P, reduced, features, st);
     // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
     const unsigned ng1 = static_cast<unsigned>(h.gram1_items.size() / 3);
-    pca_gram_kernel<1><<<ng1, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part);
+    if (gs)
+        pca_gram_kernel<1, true><<<ng1, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part,
+                                                                          *gs);
+    else
+        pca_gram_kernel<1><<<ng1, kGramThreads, P->lds_gram1, st>>>(clips, p, l.tb, l.part,
+                                                                    GatherSrc{});
     MSAMD_LAUNCH_CHECK();
     {
         const dim3 rg(static_cast<unsigned>(ceil_div(static_cast<int64_t>(h.maxW) * T * T, 256)),
@@ -1053,10 +1229,17 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     MSAMD_LAUNCH_CHECK();
     if (!h.j1.empty()) {  // stage 1 projections
         const unsigned g1 = static_cast<unsigned>(h.j1.size() / 3);
-        if (cpad_for(nf1) == 12)
-            pca_project1_kernel<12><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
+        const bool c12 = cpad_for(nf1) == 12;
+        if (gs && c12)
+            pca_project1_kernel<12, true><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced, *gs);
+        else if (gs)
+            pca_project1_kernel<16, true><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced, *gs);
+        else if (c12)
+            pca_project1_kernel<12><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced,
+                                                                    GatherSrc{});
         else
-            pca_project1_kernel<16><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced);
+            pca_project1_kernel<16><<<g1, 256, P->lds_proj1, st>>>(clips, p, l.tb, reduced,
+                                                                    GatherSrc{});
         MSAMD_LAUNCH_CHECK();
     }
     return enqueue_stage2(P, reduced, features, st);
@@ -1120,12 +1303,31 @@ extern "C" int msamd_pca_plan_create(int32_t G, const int64_t* num_events_host,
     msamd_pca_plan* P = new msamd_pca_plan();
     int rc = plan_build(P, G, num_events_host, widths_host, T, num_features_per_channel,
                         num_features, max_pca_samples, tiles_from(tiles));
-    if (rc == MSAMD_OK) rc = plan_upload(P, workspace, workspace_bytes, as_stream(stream));
+    const bool deferred = !workspace && workspace_bytes == 0;
+    if (rc == MSAMD_OK && !deferred) rc = plan_upload(P, workspace, workspace_bytes, as_stream(stream));
     if (rc != MSAMD_OK) {
         delete P;
         return rc;
     }
+    P->attached = !deferred;
     *plan = P;
+    return MSAMD_OK;
+}
+
+extern "C" int64_t msamd_pca_plan_workspace_bytes(const msamd_pca_plan* plan) {
+    return plan ? plan->l.bytes : -1;
+}
+
+extern "C" int msamd_pca_plan_attach(msamd_pca_plan* plan, void* workspace,
+                                     int64_t workspace_bytes, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(plan, "pca_plan_attach: null plan");
+    msamd_pca_plan* P = plan;
+    plan_free_graph(P);  // a graph captured on the old workspace is stale
+    P->attached = false;
+    const int rc = plan_upload(P, workspace, workspace_bytes, as_stream(stream));
+    if (rc != MSAMD_OK) return rc;
+    P->attached = true;
     return MSAMD_OK;
 }
 
@@ -1133,13 +1335,46 @@ extern "C" int msamd_pca_plan_run(msamd_pca_plan* plan, const float* clips, floa
                                   float* features, void* stream) {
     clear_error();
     MSAMD_REQUIRE(plan && clips && reduced && features, "pca_plan_run: null argument");
+    MSAMD_REQUIRE(plan->attached, "pca_plan_run: plan has no workspace (msamd_pca_plan_attach)");
     return plan_enqueue(plan, clips, reduced, features, as_stream(stream));
+}
+
+namespace {
+
+int gather_source(const msamd_pca_plan* P, const float* X, int64_t M, int64_t N,
+                  const int64_t* times, const int32_t* nbhd_channels,
+                  const int64_t* nbhd_offsets, GatherSrc* gs) {
+    MSAMD_REQUIRE(X && times && nbhd_channels && nbhd_offsets,
+                  "pca timeseries: null argument");
+    MSAMD_REQUIRE(M >= 1 && N >= 0, "pca timeseries: bad shape");
+    MSAMD_REQUIRE(!P || !P->vectors, "pca timeseries: plan holds the stage-2 PCA alone");
+    *gs = GatherSrc{X, M, N, times, nbhd_channels, nbhd_offsets,
+                    P ? (P->p.T + 1) / 2 - 1 : 0};
+    return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_pca_plan_run_timeseries(msamd_pca_plan* plan, const float* X, int64_t M,
+                                             int64_t N, const int64_t* times,
+                                             const int32_t* nbhd_channels,
+                                             const int64_t* nbhd_offsets, float* reduced,
+                                             float* features, void* stream) {
+    clear_error();
+    MSAMD_REQUIRE(plan && reduced && features, "pca_plan_run_timeseries: null argument");
+    MSAMD_REQUIRE(plan->attached,
+                  "pca_plan_run_timeseries: plan has no workspace (msamd_pca_plan_attach)");
+    GatherSrc gs;
+    const int rc = gather_source(plan, X, M, N, times, nbhd_channels, nbhd_offsets, &gs);
+    if (rc != MSAMD_OK) return rc;
+    return plan_enqueue(plan, nullptr, reduced, features, as_stream(stream), &gs);
 }
 
 extern "C" int msamd_pca_plan_launch_graph(msamd_pca_plan* plan, const float* clips,
                                            float* reduced, float* features, void* stream) {
     clear_error();
     MSAMD_REQUIRE(plan && clips && reduced && features, "pca_plan_launch_graph: null argument");
+    MSAMD_REQUIRE(plan->attached, "pca_plan_launch_graph: plan has no workspace (msamd_pca_plan_attach)");
     msamd_pca_plan* P = plan;
     if (!P->exec || P->g_clips != clips || P->g_reduced != reduced || P->g_features != features) {
         plan_free_graph(P);
@@ -1213,13 +1448,13 @@ extern "C" int64_t msamd_pca_autotune_workspace_size(int32_t G, const int64_t* n
     return best;
 }
 
-extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_host,
-                                  const int32_t* widths_host, int32_t T,
-                                  int32_t num_features_per_channel, int32_t num_features,
-                                  int64_t max_pca_samples, float* reduced, float* features,
-                                  void* workspace, int64_t workspace_bytes, void* stream,
-                                  int32_t* tiles_out, float* best_ms_out) {
-    clear_error();
+namespace {
+
+int autotune(const float* clips, const GatherSrc* gs, int32_t G, const int64_t* num_events_host,
+             const int32_t* widths_host, int32_t T, int32_t num_features_per_channel,
+             int32_t num_features, int64_t max_pca_samples, float* reduced, float* features,
+             void* workspace, int64_t workspace_bytes, void* stream, int32_t* tiles_out,
+             float* best_ms_out) {
     MSAMD_REQUIRE(tiles_out, "pca_autotune: null tiles_out");
     hipStream_t st = as_stream(stream);
     hipEvent_t e0, e1;
@@ -1246,7 +1481,7 @@ extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* 
             float ms = 0.f;
             for (int rep = 0; rep < 2 && rc == MSAMD_OK; ++rep) {  // second run timed
                 if (hipEventRecord(e0, st) != hipSuccess) { rc = MSAMD_ERR_HIP; break; }
-                rc = plan_enqueue(&P, clips, reduced, features, st);
+                rc = plan_enqueue(&P, clips, reduced, features, st, gs);
                 if (rc != MSAMD_OK) break;
                 if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -1270,6 +1505,39 @@ extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* 
     tiles_out[2] = best.pe2;
     if (best_ms_out) *best_ms_out = best_ms;
     return MSAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int msamd_pca_autotune(const float* clips, int32_t G, const int64_t* num_events_host,
+                                  const int32_t* widths_host, int32_t T,
+                                  int32_t num_features_per_channel, int32_t num_features,
+                                  int64_t max_pca_samples, float* reduced, float* features,
+                                  void* workspace, int64_t workspace_bytes, void* stream,
+                                  int32_t* tiles_out, float* best_ms_out) {
+    clear_error();
+    MSAMD_REQUIRE(clips, "pca_autotune: null clips");
+    return autotune(clips, nullptr, G, num_events_host, widths_host, T, num_features_per_channel,
+                    num_features, max_pca_samples, reduced, features, workspace,
+                    workspace_bytes, stream, tiles_out, best_ms_out);
+}
+
+extern "C" int msamd_pca_autotune_timeseries(
+    const float* X, int64_t M, int64_t N, const int64_t* times, const int32_t* nbhd_channels,
+    const int64_t* nbhd_offsets, int32_t G, const int64_t* num_events_host,
+    const int32_t* widths_host, int32_t T, int32_t num_features_per_channel,
+    int32_t num_features, int64_t max_pca_samples, float* reduced, float* features,
+    void* workspace, int64_t workspace_bytes, void* stream, int32_t* tiles_out,
+    float* best_ms_out) {
+    clear_error();
+    GatherSrc gs;
+    const int rc = gather_source(nullptr, X, M, N, times, nbhd_channels, nbhd_offsets, &gs);
+    if (rc != MSAMD_OK) return rc;
+    MSAMD_REQUIRE(T >= 1, "pca_autotune: bad clip_size");
+    gs.tmid = (T + 1) / 2 - 1;
+    return autotune(nullptr, &gs, G, num_events_host, widths_host, T, num_features_per_channel,
+                    num_features, max_pca_samples, reduced, features, workspace,
+                    workspace_bytes, stream, tiles_out, best_ms_out);
 }
 
 // --------------------------------------------------- PCA of index subsets

@@ -266,11 +266,11 @@ class PcaPlan:
         pt = self._tiles_arg.ctypes.data_as(ctypes.c_void_p) if self._tiles_arg is not None else None
         pL, pW = L.ctypes.data_as(ctypes.c_void_p), W.ctypes.data_as(ctypes.c_void_p)
         lib = _capi.lib()
-        nb = lib.msamd_pca_plan_workspace_size(self.G, pL, pW, self.T, self.nf1, self.nf2,
-                                               self.max_s, pt)
-        if nb < 0:
-            raise _capi.MsamdError("msamd_pca_plan_workspace_size", 1,
-                                   "unsupported neighborhood shapes or tiles")
+        # host tables built once (deferred create), then uploaded into the workspace
+        self._plan = ctypes.c_void_p()
+        _capi.call("msamd_pca_plan_create", self.G, pL, pW, self.T, self.nf1, self.nf2,
+                   self.max_s, pt, None, 0, _stream(self.device), ctypes.addressof(self._plan))
+        nb = lib.msamd_pca_plan_workspace_bytes(self._plan)
         self._ws = _workspace(nb, self.device)
         self.clip_elems = int((W.astype(np.int64) * self.T * L).sum())
         self.red_off = np.concatenate([[0], np.cumsum(W.astype(np.int64) * self.nf1 * L)])
@@ -279,10 +279,8 @@ class PcaPlan:
                                    device=self.device)
         self.features = torch.empty(max(int(self.feat_off[-1]), 1), dtype=torch.float32,
                                     device=self.device)
-        self._plan = ctypes.c_void_p()
-        _capi.call("msamd_pca_plan_create", self.G, pL, pW, self.T, self.nf1, self.nf2,
-                   self.max_s, pt, _ptr(self._ws), self._ws.numel(), _stream(self.device),
-                   ctypes.addressof(self._plan))
+        _capi.call("msamd_pca_plan_attach", self._plan, _ptr(self._ws), self._ws.numel(),
+                   _stream(self.device))
 
     @property
     def tiles(self):
@@ -301,6 +299,31 @@ class PcaPlan:
                              "on the plan's device")
         fn = "msamd_pca_plan_launch_graph" if graph else "msamd_pca_plan_run"
         _capi.call(fn, self._plan, _ptr(clips_flat), _ptr(self.reduced), _ptr(self.features),
+                   _stream(self.device))
+        return self.reduced, self.red_off, self.features, self.feat_off
+
+    def run_timeseries(self, X, times, nbhd_channels, nbhd_offsets):
+        """The PCA with the clip gather fused into the stage-1 kernels
+        (msamd_pca_plan_run_timeseries): clips are read from the timeseries
+        X (N, M) at times - Tmid, no clip array is made.  times: the event
+        centres packed neighborhood after neighborhood (the plan's
+        num_events); nbhd_channels/nbhd_offsets as for extract_clips_batched.
+        Returns (reduced, red_off, features, feat_off), bit-identical to
+        run() on the gathered clips."""
+        if self._plan is None or not self._plan.value:
+            raise RuntimeError("PcaPlan is closed")
+        _check_ts(X)
+        if X.device != self.device:
+            raise ValueError("X must be on the plan's device")
+        N, M = X.shape
+        times = times.to(self.device, torch.int64).contiguous()
+        no = torch.as_tensor(nbhd_offsets, dtype=torch.int64).to(self.device).contiguous()
+        nc = _channel_list(nbhd_channels, M, self.device)
+        if no.numel() != self.G + 1:
+            raise ValueError("nbhd_offsets must hold G + 1 offsets")
+        self._keep = (times, no, nc)  # alive until the next run (kernels are enqueued)
+        _capi.call("msamd_pca_plan_run_timeseries", self._plan, _ptr(X), M, N, _ptr(times),
+                   _ptr(nc), _ptr(no), _ptr(self.reduced), _ptr(self.features),
                    _stream(self.device))
         return self.reduced, self.red_off, self.features, self.feat_off
 
@@ -339,6 +362,61 @@ def pca_autotune(clips_flat, num_events, widths, clip_size, num_features_per_cha
     _capi.call("msamd_pca_autotune", _ptr(clips_flat), G, pL, pW, T, nf1, nf2,
                int(max_pca_samples), _ptr(red), _ptr(feat), _ptr(ws), ws.numel(),
                _stream(dev), tiles.ctypes.data_as(ctypes.c_void_p), ctypes.addressof(ms))
+    return tuple(int(x) for x in tiles), float(ms.value)
+
+
+def pca_features_from_timeseries(X, times, event_offsets, nbhd_channels, nbhd_offsets,
+                                 clip_size, num_features_per_channel=10, num_features=10,
+                                 max_pca_samples=10000, tiles=None):
+    """pca_features with the clip gather fused into the stage-1 kernels: the
+    same (reduced, red_off, features, feat_off), bit for bit, as
+    pca_features(extract_clips_batched(X, times, ...)), without the clip
+    array.  times: event centres, neighborhood g's at
+    times[event_offsets[g]:event_offsets[g+1]]."""
+    import numpy as np
+    eo = torch.as_tensor(event_offsets).cpu().numpy().astype(np.int64)
+    no = torch.as_tensor(nbhd_offsets).cpu().numpy().astype(np.int64)
+    L = np.ascontiguousarray(eo[1:] - eo[:-1], dtype=np.int64)
+    W = np.ascontiguousarray(no[1:] - no[:-1], dtype=np.int32)
+    plan = PcaPlan(L, W, clip_size, num_features_per_channel, num_features, max_pca_samples,
+                   tiles=tiles, device=X.device)
+    out = plan.run_timeseries(X, times[int(eo[0]):int(eo[-1])], nbhd_channels, nbhd_offsets)
+    plan.close()  # the outputs stay alive; the kernels are enqueued on the stream
+    return out
+
+
+def pca_autotune_timeseries(X, times, event_offsets, nbhd_channels, nbhd_offsets, clip_size,
+                            num_features_per_channel=10, num_features=10,
+                            max_pca_samples=10000):
+    """pca_autotune for the fused form (msamd_pca_autotune_timeseries).
+    Returns (tiles, milliseconds)."""
+    import numpy as np
+    _check_ts(X)
+    N, M = X.shape
+    dev = X.device
+    eo = torch.as_tensor(event_offsets).cpu().numpy().astype(np.int64)
+    no_h = torch.as_tensor(nbhd_offsets).cpu().numpy().astype(np.int64)
+    L, W = _pca_shapes(eo[1:] - eo[:-1], no_h[1:] - no_h[:-1])
+    G, T = int(L.size), int(clip_size)
+    nf1, nf2 = int(num_features_per_channel), int(num_features)
+    pL, pW = L.ctypes.data_as(ctypes.c_void_p), W.ctypes.data_as(ctypes.c_void_p)
+    lib = _capi.lib()
+    nb = lib.msamd_pca_autotune_workspace_size(G, pL, pW, T, nf1, nf2, int(max_pca_samples))
+    if nb < 0:
+        raise _capi.MsamdError("msamd_pca_autotune_workspace_size", 1, "unsupported shapes")
+    ws = _workspace(nb, dev)
+    red = torch.empty(max(int((W.astype(np.int64) * nf1 * L).sum()), 1), dtype=torch.float32,
+                      device=dev)
+    feat = torch.empty(max(int((nf2 * L).sum()), 1), dtype=torch.float32, device=dev)
+    t = times[int(eo[0]):int(eo[-1])].to(dev, torch.int64).contiguous()
+    no = torch.as_tensor(nbhd_offsets, dtype=torch.int64).to(dev).contiguous()
+    nc = _channel_list(nbhd_channels, M, dev)
+    tiles = np.zeros(3, dtype=np.int32)
+    ms = ctypes.c_float()
+    _capi.call("msamd_pca_autotune_timeseries", _ptr(X), M, N, _ptr(t), _ptr(nc), _ptr(no), G,
+               pL, pW, T, nf1, nf2, int(max_pca_samples), _ptr(red), _ptr(feat), _ptr(ws),
+               ws.numel(), _stream(dev), tiles.ctypes.data_as(ctypes.c_void_p),
+               ctypes.addressof(ms))
     return tuple(int(x) for x in tiles), float(ms.value)
 
 

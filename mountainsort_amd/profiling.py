@@ -260,9 +260,106 @@ def main_counters(argv):
     sys.stdout.write("\n".join(lines) + "\n")
 
 
+def csv_counters(path):
+    """{kernel: {counter: [value per dispatch]}} and {kernel: [durations us]} from a
+    rocprofv3 counter_collection.csv (one row per dispatch and counter)."""
+    import collections
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    dur = collections.defaultdict(dict)
+    for r in csv.DictReader(open(path)):
+        k = clean(r["Kernel_Name"])
+        agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        dur[k][r["Dispatch_Id"]] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
+    return agg, {k: list(v.values()) for k, v in dur.items()}
+
+
+def main_all_csv(argv):
+    """python -m mountainsort_amd.profiling pmc-all-csv FETCH.csv WRITE.csv OUT.md TITLE
+    ["fetch command" "write command"] -> as pmc-all, from counter_collection.csv files."""
+    fetch_csv, write_csv, out, title = argv[:4]
+    cmds = argv[4:6]
+    fa, fdur = csv_counters(fetch_csv)
+    wa, _ = csv_counters(write_csv)
+    lines = [f"# {title}", ""]
+    if cmds:
+        lines += ["Commands:", "", "```", *cmds, "```", ""]
+    calib = {}
+    for agg, counter in ((fa, "FETCH_SIZE"), (wa, "WRITE_SIZE")):
+        vals = [v for k, c in agg.items() if "neg" in k.lower() and "msamd" not in k
+                for v in c.get(counter, [])]
+        calib[counter] = sum(vals) / len(vals) if vals else float("nan")
+    fs, ws = 1048576 / calib["FETCH_SIZE"], 1048576 / calib["WRITE_SIZE"]
+    lines += [f"Calibration: a 1 GiB elementwise negation (reads and writes 1 GiB) reads "
+              f"FETCH_SIZE {calib['FETCH_SIZE']:.6g} KiB and WRITE_SIZE "
+              f"{calib['WRITE_SIZE']:.6g} KiB, so read bytes = FETCH_SIZE x {fs:.4g} KiB "
+              f"and write bytes = WRITE_SIZE x {ws:.4g} KiB.", ""]
+    lines += ["| kernel | dispatches | avg us | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB | "
+              "written GB |", "|---|---:|---:|---:|---:|---:|---:|"]
+    names = sorted((k for k in fdur if "msamd" in k), key=lambda k: -sum(fdur[k]))
+    for k in names:
+        f = fa.get(k, {}).get("FETCH_SIZE", [])
+        w = wa.get(k, {}).get("WRITE_SIZE", [])
+        d = fdur[k]
+        fm = sum(f) / len(f) if f else float("nan")
+        wm = sum(w) / len(w) if w else float("nan")
+        lines.append(f"| `{k}` | {len(d)} | {sum(d) / len(d):.2f} | {fm:.6g} | {wm:.6g} | "
+                     f"{fm * fs * 1024 / 1e9:.3f} | {wm * ws * 1024 / 1e9:.3f} |")
+    open(out, "w").write("\n".join(lines) + "\n")
+    sys.stdout.write("\n".join(lines) + "\n")
+
+
+def main_counters_csv(argv):
+    """python -m mountainsort_amd.profiling counters-csv REGEX OUT.md TITLE "command"
+    CSV [CSV ...] -> for each msamd kernel matching REGEX in rocprofv3
+    counter_collection.csv files (one row per dispatch and counter), each
+    counter averaged over the kernel's dispatches, the dispatch time, and the
+    derived MFMA busy fraction SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x
+    32 CUs per XCD) and texture-unit fractions per CU (x / (GRBM_GUI_ACTIVE /
+    8 XCDs x 256 CUs)) when those counters are present."""
+    import collections
+    regex, out, title, command = argv[:4]
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    durs = collections.defaultdict(dict)
+    for path in argv[4:]:
+        for r in csv.DictReader(open(path)):
+            k = clean(r["Kernel_Name"])
+            if not k.startswith("msamd::") or not re.search(regex, k):
+                continue
+            vals[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            durs[k][(path, r["Dispatch_Id"])] = (int(r["End_Timestamp"]) -
+                                                 int(r["Start_Timestamp"])) / 1000.0
+    lines = [f"# {title}", "", "Command(s):", "", "```", command, "```", ""]
+    for k in sorted(vals):
+        d = list(durs[k].values())
+        lines += [f"## `{k}`", "", f"Mean dispatch time {sum(d) / len(d):.1f} us over "
+                  f"{len(d)} dispatches (counter runs).", "",
+                  "| counter | mean per dispatch |", "|---|---:|"]
+        mean = {c: sum(v) / len(v) for c, v in vals[k].items()}
+        for c in sorted(mean):
+            lines.append(f"| {c} | {mean[c]:.4g} |")
+        g = mean.get("GRBM_GUI_ACTIVE")
+        derived = []
+        if g and "SQ_VALU_MFMA_BUSY_CYCLES" in mean:
+            derived.append(("MFMA busy (per CU)", mean["SQ_VALU_MFMA_BUSY_CYCLES"] / (g * 32)))
+        for c in ("TD_TD_BUSY", "TD_TC_STALL", "TA_TA_BUSY", "TA_ADDR_STALLED_BY_TC_CYCLES",
+                  "TCP_PENDING_STALL_CYCLES"):
+            if c in mean and "GRBM_GUI_ACTIVE" in mean:
+                derived.append((f"{c} / cycles (per CU)", mean[c] / (g / 8 * 256)))
+        if derived:
+            lines += ["", "| derived | fraction |", "|---|---:|"]
+            lines += [f"| {n} | {v:.3f} |" for n, v in derived]
+        lines.append("")
+    open(out, "w").write("\n".join(lines) + "\n")
+    sys.stdout.write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "counters":
         main_counters(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "counters-csv":
+        main_counters_csv(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "pmc-all-csv":
+        main_all_csv(sys.argv[2:])
     elif len(sys.argv) > 1 and sys.argv[1] == "traffic":
         main_traffic(sys.argv[2:])
     elif len(sys.argv) > 1 and sys.argv[1] == "pmc-all":

@@ -91,3 +91,33 @@ def test_pca_workspace_query():
     assert L.msamd_pca_workspace_size(5, args[0], w_bad.ctypes.data_as(ctypes.c_void_p),
                                       50, 10, 10, 10000) < 0
     assert L.msamd_pca_workspace_size(5, *args, 65, 10, 10, 10000) < 0
+
+
+def test_pca_plan_deferred_create_needs_attach():
+    """A plan created without a workspace holds its host tables only: it
+    reports the workspace it needs (the same as the size query), and runs
+    are refused until a workspace is attached.  Host-only calls."""
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libmsamd.so not built")
+    import numpy as np
+    L = _capi.lib()
+    ne = np.array([12000, 300, 0, 45000], dtype=np.int64)
+    w = np.array([5, 17, 3, 12], dtype=np.int32)
+    tiles = np.array([2560, 32, 16], dtype=np.int32)
+    args = (4, ne.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p), 50, 10, 10,
+            10000, tiles.ctypes.data_as(ctypes.c_void_p))
+    nb = L.msamd_pca_plan_workspace_size(*args)
+    assert nb > 0
+    plan = ctypes.c_void_p()
+    assert L.msamd_pca_plan_create(*args, None, 0, None, ctypes.addressof(plan)) == 0
+    assert plan.value
+    assert L.msamd_pca_plan_workspace_bytes(plan) == nb
+    assert L.msamd_pca_plan_run(plan, ctypes.c_void_p(16), ctypes.c_void_p(16),
+                                ctypes.c_void_p(16), None) != 0
+    assert b"attach" in L.msamd_last_error()
+    assert L.msamd_pca_plan_run_timeseries(plan, ctypes.c_void_p(16), 4, 100, ctypes.c_void_p(16),
+                                           ctypes.c_void_p(16), ctypes.c_void_p(16),
+                                           ctypes.c_void_p(16), ctypes.c_void_p(16), None) != 0
+    # attaching a workspace that is too small is refused before any copy
+    assert L.msamd_pca_plan_attach(plan, ctypes.c_void_p(256), 1024, None) != 0
+    L.msamd_pca_plan_destroy(plan)

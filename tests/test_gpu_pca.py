@@ -221,3 +221,83 @@ def test_sort_clips_branch_method_matches_oracle(gpu, W, sizes):
     assert np.array_equal(lab_g, lab_o)
     pairs = {(int(a), int(b)) for a, b in zip(lab_g, truth)}
     assert len(pairs) == len(set(lab_g.tolist())) == len(sizes)
+
+
+def _gather_case(rng, M=64, N=120_000, T=50):
+    """Neighborhoods of several widths (central channel first) with sorted
+    event times that include clips running past both ends of the recording,
+    and one neighborhood above max_pca_samples (subsampled Gram columns)."""
+    shapes = [(300, 1), (12345, 5), (900, 12), (40, 13), (2500, 17), (0, 3), (700, 40)]
+    nb, times = [], []
+    edge = np.array([0, 1, 5, 23, 24, 25, N - 26, N - 25, N - 2, N - 1])
+    for L, W in shapes:
+        c = int(rng.integers(0, M))
+        others = rng.choice(np.setdiff1d(np.arange(M), [c]), W - 1, replace=False)
+        nb.append(np.concatenate([[c], np.sort(others)]).astype(np.int32))
+        t = rng.integers(0, N, max(L - len(edge), 0))
+        t = np.sort(np.concatenate([t, edge[:min(L, len(edge))]]))
+        times.append(t.astype(np.int64))
+    return shapes, nb, times
+
+
+@pytest.mark.parametrize("tiles", [None, (1280, 32, 16), (10240, 256, 128)])
+def test_pca_fused_gather_matches_clip_path(gpu, tiles):
+    """The fused form (stage-1 kernels reading clips from the timeseries)
+    gives features bit-identical to the clip path with the same tiles, edge
+    clips (zero-filled outside the recording) and subsampled columns
+    included."""
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(17)
+    M, N = 64, 120_000
+    X = torch.from_numpy(rng.normal(size=(N, M)).astype(np.float32) * 5).to(gpu)
+    shapes, nb, times = _gather_case(rng, M, N)
+    say(f"pca fused gather: (L, W) = {shapes}, tiles {tiles}")
+    eo = np.concatenate([[0], np.cumsum([len(t) for t in times])])
+    no = np.concatenate([[0], np.cumsum([len(c) for c in nb])])
+    t_d = torch.from_numpy(np.concatenate(times)).to(gpu)
+    nc = np.concatenate(nb)
+    clips, co = fe.extract_clips_batched(X, t_d, eo, nc, no, 50)
+    L = eo[1:] - eo[:-1]
+    W = no[1:] - no[:-1]
+    plan = fe.PcaPlan(L, W, 50, tiles=tiles, device=gpu)
+    red1, ro, feat1, fo = plan.run(clips)
+    red1, feat1 = red1.clone(), feat1.clone()
+    plan.reduced.fill_(float("nan"))
+    plan.features.fill_(float("nan"))
+    red2, _, feat2, _ = plan.run_timeseries(X, t_d, nc, no)
+    assert torch.equal(red1, red2)
+    assert torch.equal(feat1, feat2)
+    plan.close()
+    # the one-call form and the oracle on the neighborhood with edge clips
+    red3, _, feat3, _ = fe.pca_features_from_timeseries(X, t_d, eo, nc, no, 50, tiles=tiles)
+    assert torch.equal(red1, red3) and torch.equal(feat1, feat3)
+    g = 2
+    c = clips[co[g]:co[g + 1]].cpu().numpy().reshape(L[g], 50, W[g])
+    r_o = oc.dimension_reduce_clips(c, 10, 10000)
+    f_o = oc.stage2_features(r_o, 10, 10000)
+    assert_pca_parity(red1[ro[g]:ro[g + 1]].cpu().numpy().reshape(L[g], 10, W[g]), r_o,
+                      feat1[fo[g]:fo[g + 1]].cpu().numpy().reshape(L[g], 10), f_o,
+                      f"fused g={g}", clips=c)
+
+
+def test_pca_autotune_timeseries(gpu):
+    """The fused form's autotuner returns tiles from the candidate sets, and
+    the fused features at those tiles equal the clip path's."""
+    from mountainsort_amd import frontend as fe
+    rng = np.random.default_rng(23)
+    M, N = 64, 120_000
+    X = torch.from_numpy(rng.normal(size=(N, M)).astype(np.float32)).to(gpu)
+    shapes, nb, times = _gather_case(rng, M, N)
+    eo = np.concatenate([[0], np.cumsum([len(t) for t in times])])
+    no = np.concatenate([[0], np.cumsum([len(c) for c in nb])])
+    t_d = torch.from_numpy(np.concatenate(times)).to(gpu)
+    nc = np.concatenate(nb)
+    say("pca autotune (fused gather)")
+    tiles, ms = fe.pca_autotune_timeseries(X, t_d, eo, nc, no, 50)
+    say(f"  tiles {tiles}, {ms:.3f} ms")
+    assert tiles[0] in (1280, 2560, 5120, 10240)
+    assert tiles[1] in (32, 64, 128, 256) and tiles[2] in (16, 32, 64, 128)
+    clips, co = fe.extract_clips_batched(X, t_d, eo, nc, no, 50)
+    r1, _, f1, _ = fe.pca_features(clips, co, eo, no, 50, tiles=tiles)
+    r2, _, f2, _ = fe.pca_features_from_timeseries(X, t_d, eo, nc, no, 50, tiles=tiles)
+    assert torch.equal(r1, r2) and torch.equal(f1, f2)
