@@ -209,11 +209,17 @@ __device__ __forceinline__ int stencil_float(const float* xp, int64_t t, float x
     return -1;
 }
 
-template <int kSign>
+// kGated: the learned detector's gated scan (channels 0..M-1): a crossing is
+// kept only when the network proposed its 32-sample window (class 1 or 2 in
+// cls[w * M + channel]); the stencil itself still reads every neighbour, so
+// every kept sample is an event of plain detection and the slot capacity
+// holds.  Window classes are read only for batches with a crossing.
+template <int kSign, bool kGated = false>
 __global__ void __launch_bounds__(256) detect_scan_kernel(
     const float* __restrict__ X, DetectPlan p, const int32_t* __restrict__ channels,
     const double* __restrict__ mean_in, const double* __restrict__ thr_in,
-    int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
+    int32_t* __restrict__ counts, uint16_t* __restrict__ slots,
+    const uint8_t* __restrict__ cls = nullptr) {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= p.nseg * p.Mc) return;
     const int j = static_cast<int>(g % p.Mc);
@@ -273,6 +279,24 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
                 }
                 over &= ~rej;
             }
+            if (kGated && over) {
+                constexpr int64_t W = tcn::kWin;
+                const int64_t nW = (p.N_file + W - 1) / W;
+                const int64_t wa_ = tb / W;
+                int64_t wb_ = (tb + kB - 1) / W;
+                wb_ = wb_ < nW ? wb_ : nW - 1;
+                const bool pa = cls[wa_ * p.M + j] != 0;
+                const bool pb = cls[wb_ * p.M + j] != 0;
+                const uint32_t all = (1u << kB) - 1;
+                uint32_t gate;
+                if (wb_ == wa_) {
+                    gate = pa ? all : 0u;
+                } else {
+                    const uint32_t lowm = (1u << static_cast<int>(wb_ * W - tb)) - 1;  // batch indices in wa_
+                    gate = (pa ? lowm : 0u) | (pb ? (all & ~lowm) : 0u);
+                }
+                over &= gate;
+            }
             // in time order; kept events are > D apart, so cnt < cap
             while (over) {
                 const int i = __builtin_ctz(over);
@@ -288,53 +312,6 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
             prev = cur[kB - 1];
 #pragma unroll
             for (int i = 0; i < kB; ++i) cur[i] = nxt[i];
-        }
-    }
-    counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
-}
-
-// Gated scan of the learned detector: one thread per (segment, channel), the
-// channel list being 0..M-1.  Windows of class 0 (no spike) are skipped; in
-// windows proposed by the network (class 1, uncertain, and class 2, present)
-// the plain rule above decides every sample, so every kept sample is an event
-// of plain detection and the slot capacity of the plain scan holds.
-__global__ void __launch_bounds__(256) detect_gated_scan_kernel(
-    const float* __restrict__ X, DetectPlan p, int sign, const double* __restrict__ mean_in,
-    const double* __restrict__ thr_in, const uint8_t* __restrict__ cls,
-    int32_t* __restrict__ counts, uint16_t* __restrict__ slots) {
-    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= p.nseg * p.Mc) return;
-    const int j = static_cast<int>(g % p.Mc);
-    const int64_t seg = g / p.Mc;
-    const int64_t ck = seg / p.spc;
-    const int64_t s = seg % p.spc;
-    int64_t size = p.n_range - ck * p.cs;
-    if (size > p.cs) size = p.cs;
-    const int64_t r0 = s * kSeg;
-    int64_t r1 = r0 + kSeg;
-    if (r1 > size) r1 = size;
-    int32_t cnt = 0;
-    if (r0 < r1) {
-        const double mean = mean_in[ck * p.Mc + j];
-        const double thr = thr_in[ck * p.Mc + j];
-        const int64_t core0 = p.t_start + ck * p.cs;
-        const int64_t wa = core0 - p.pad;
-        const int64_t wb = core0 + size + p.pad;
-        const float* xp = X + j;
-        uint16_t* my_slots = slots + (seg * p.Mc + j) * p.cap;
-        const int64_t T0 = core0 + r0, T1 = core0 + r1;
-        constexpr int64_t W = tcn::kWin;
-        for (int64_t w = T0 / W; w * W < T1; ++w) {
-            const int c = cls[w * p.M + j];
-            if (c == 0) continue;
-            const int64_t lo = w * W > T0 ? w * W : T0;
-            const int64_t hi = w * W + W < T1 ? w * W + W : T1;
-            for (int64_t t = lo; t < hi; ++t) {
-                const double v = detect_value(load_x(xp, t, p.M, p.N_file), mean, sign);
-                if (!(v > thr)) continue;
-                if (stencil_ok(xp, t, v, mean, sign, p.D, wa, wb, p.M, p.N_file) && cnt < p.cap)
-                    my_slots[cnt++] = static_cast<uint16_t>(t - T0);
-            }
         }
     }
     counts[static_cast<int64_t>(j) * p.nseg + seg] = cnt;
@@ -649,9 +626,16 @@ extern "C" int msamd_tcn_detect(const float* X, int64_t M, int64_t N_file, int64
     rc = tcn::classify(X, M, N_file, lb.frag, lo, hi, nullptr, lb.classes, st);
     if (rc != MSAMD_OK) return rc;
     {
-        const int64_t n = p.nseg * p.Mc;
-        detect_gated_scan_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
-            X, p, detect_sign, b.mean, b.thr, lb.classes, b.counts, b.slots);
+        const dim3 grid(static_cast<unsigned>(ceil_div(p.nseg * p.Mc, 256)));
+        if (detect_sign < 0)
+            detect_scan_kernel<-1, true><<<grid, 256, 0, st>>>(X, p, lb.channels, b.mean, b.thr,
+                                                                b.counts, b.slots, lb.classes);
+        else if (detect_sign == 0)
+            detect_scan_kernel<0, true><<<grid, 256, 0, st>>>(X, p, lb.channels, b.mean, b.thr,
+                                                               b.counts, b.slots, lb.classes);
+        else
+            detect_scan_kernel<1, true><<<grid, 256, 0, st>>>(X, p, lb.channels, b.mean, b.thr,
+                                                               b.counts, b.slots, lb.classes);
         MSAMD_LAUNCH_CHECK();
     }
     detect_seg_scan_kernel<<<dim3(p.Mc), 1024, 0, st>>>(b.counts, p.nseg, b.seg_off,

@@ -16,8 +16,9 @@
 //
 // Kernels:
 //   tcn_prep_kernel      bf16 weight fragments in MFMA lane order
-//   tcn_classify_kernel  inference: layers 1-2 on v_mfma_f32_16x16x32_bf16
-//                        from LDS-staged bf16 tiles, layer 3 on the VALU
+//   tcn_classify_kernel  inference streamed along time: layers 1-2 on
+//                        v_mfma_f32_16x16x32_bf16 from LDS-staged bf16 input
+//                        rows, layer 3 folded into per-position sums
 //   tcn_grad_kernel      training forward + backward of one segment per
 //                        workgroup (fp32 VALU), per-segment gradients
 //   tcn_reduce_kernel    ordered sum of the segment gradients
@@ -53,35 +54,39 @@ __device__ __forceinline__ double unit_open(uint64_t r) {
 // k = 8*(l/16)..+7; B (32 x 16): lane l holds column l%16, the same k range;
 // D (16 x 16): lane l, register r holds (4*(l/16) + r, l%16).
 //
-// Layer 1 as a GEMM over K = 16 groups of 8: group g = (ci = g/2, part = g%2);
-// part 0 pairs taps 0..7 with x[ci][8j+4 .. 8j+11], part 1 pairs tap 8 (first
-// slot, the rest zero) with x[ci][8j+12 ..]; groups 14, 15 are zero.  Every
-// B group is 8 consecutive samples of one channel row (one 16-byte LDS read).
-// Layer 2: K-step s = tap s, group b = hidden channels 8b..8b+7.
+// Layer 1 as a GEMM over K = 64 = 8 groups of 8: group g < 7 pairs taps 0..7
+// of input channel g with x[g][8j+4 .. 8j+11]; group 7 pairs tap 8 of input
+// channels 0..6 (and a zero) with x[0..6][8j+12].  K-step s holds groups
+// 4s .. 4s+3.  Layer 2: K-step s = tap s, group b = hidden channels 8b..8b+7.
+// Layer 3 (per-position sums u_k = sum_o W3[o][k] h2[o]): one K-step of 32
+// hidden channels ordered as the layer-2 accumulators hold them -- lane
+// group g's k = 8g + e is channel 4g + e (e < 4) or 16 + 4g + e - 4 -- and
+// rows k = tap (0..8; rows 9..15 zero), so the accumulators are the B
+// operand as they stand.
+__host__ __device__ inline int l3_channel(int g, int e) { return e < 4 ? 4 * g + e : 12 + 4 * g + e; }
+
 struct Frag {
-    bf16x8 a1[2][4][64];
+    bf16x8 a1[2][2][64];
     bf16x8 a2[2][9][64];
+    bf16x8 a3[64];
     float b1[kH];
     float b2[kH];
-    float w3[kH * kK];
+    float w3[kH * kK];  // bf16-rounded, [o][k] (training layout)
     float b3;
 };
 
 __global__ void tcn_prep_kernel(const float* __restrict__ P, Frag* __restrict__ F) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < 2 * 4 * 64) {
-        const int lane = q & 63, s = (q >> 6) & 3, mt = q >> 8;
+    if (q < 2 * 2 * 64) {
+        const int lane = q & 63, s = (q >> 6) & 1, mt = q >> 7;
         const int o = 16 * mt + (lane & 15);
-        const int gk = 4 * s + (lane >> 4);
-        const int ci = gk >> 1, part = gk & 1;
+        const int g = 4 * s + (lane >> 4);
         bf16x8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             float w = 0.0f;
-            if (ci < kCI) {
-                if (part == 0) w = P[kOffW1 + (o * kCI + ci) * kK + e];
-                else if (e == 0) w = P[kOffW1 + (o * kCI + ci) * kK + 8];
-            }
+            if (g < kCI) w = P[kOffW1 + (o * kCI + g) * kK + e];
+            else if (e < kCI) w = P[kOffW1 + (o * kCI + e) * kK + 8];
             v[e] = static_cast<__bf16>(w);
         }
         F->a1[mt][s][lane] = v;
@@ -101,160 +106,242 @@ __global__ void tcn_prep_kernel(const float* __restrict__ P, Frag* __restrict__ 
         F->b1[q] = P[kOffB1 + q];
         F->b2[q] = P[kOffB2 + q];
     }
+    if (q < 64) {
+        const int k = q & 15, g = q >> 4;
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            v[e] = static_cast<__bf16>(k < kK ? P[kOffW3 + l3_channel(g, e) * kK + k] : 0.0f);
+        F->a3[q] = v;
+    }
     if (q < kH * kK) F->w3[q] = bfr(P[kOffW3 + q]);
     if (q == 0) F->b3 = P[kOffB3];
 }
 
 // ------------------------------------------------------------ inference
-// One workgroup (4 waves) = kCB central channels x kWt windows.  The x tile
-// (the kCB+6 channel rows the centrals need, kXT samples, bf16) is staged
-// once; the centrals are then processed one after another: layer 1 (8
-// N-tiles of 16 positions, two per wave, both M-tiles per B fragment),
-// layer 2 (2 N-tiles x 2 M-tiles, one per wave), layer 3 (8 threads per
-// window).  The tile is sized so that three workgroups share a CU.
-constexpr int kCB = 8;
-constexpr int kWt = 22;                     // windows per tile
-constexpr int kNP2 = 32;                    // h2 positions (>= kWt + 8)
-constexpr int kNP1 = 128;                   // h1 positions (>= 4 * (kWt + 7) + 9)
-constexpr int kXT = 8 * (kNP1 - 1) + 16;    // samples per x row (1032)
-constexpr int kRows = kCB + kCI - 1;        // 14 channel rows
-constexpr int kH1S = 40;                    // h1 row stride (bf16): 80 bytes
-constexpr int kH2S = 33;                    // h2 row stride (float)
-constexpr int kCThreads = 256;
-constexpr size_t kXBytes = sizeof(__bf16) * kRows * kXT;
-constexpr size_t kH1Bytes = sizeof(__bf16) * kNP1 * kH1S;
-constexpr size_t kH2Bytes = sizeof(float) * kNP2 * kH2S;
-constexpr size_t kClassifyLds = kXBytes + kH1Bytes + kH2Bytes;
-static_assert(kNP2 >= kWt + 8 && kNP1 >= 4 * (kWt + 7) + 9, "tile too small");
-static_assert(kXT % 8 == 0 && (kXBytes % 16) == 0 && (kH1Bytes % 16) == 0, "alignment");
+// A workgroup (8 waves) scores kCB central channels over a range of kRB
+// blocks of 16 windows, streaming along time: block b (first window w0)
+// computes layer 1 at the 64 new positions j in [4w0+18, 4w0+82) from the
+// input samples [32w0+148, 32w0+661), layer 2 at the 16 positions
+// p in [w0+4, w0+20), and the logits of windows [w0, w0+16), whose
+// receptive fields end inside that block.  The layer-1 outputs (bf16) of the
+// last 5 positions and the layer-3 partial sums of the last 8 layer-2
+// positions are carried to the next block, so no position is computed
+// twice; one extra block before the range fills the carries (its logits
+// are dropped).  The input rows of a block (kCB + 6 channels, bf16) are
+// staged in LDS once for all waves and double-buffered: the next block's
+// samples are loaded into registers before this block's arithmetic.  Each
+// wave owns two central channels and their carried state, so the only
+// barrier is one per block.
+//
+// Layer 3 is folded through the per-position sums u_k[p] = sum_o W3[o][k]
+// h2[o][p], one MFMA on the layer-2 outputs as they sit in the accumulators
+// (split into two bf16 pieces, so h2 keeps ~16 bits), and
+// z[w] = b3 + sum_k u_k[w - 4 + k].
+constexpr int kSB = 16;                       // windows per block
+constexpr int kCB = 16;                       // central channels per workgroup
+constexpr int kRB = 64;                       // blocks per workgroup range
+constexpr int kRows = kCB + kCI - 1;          // 22 input rows
+constexpr int kXN = 8 * 64 + 1;               // 513 input samples per block
+constexpr int kXStride = 520;                 // bf16 per input row (1040 bytes)
+constexpr int kH1Pos = 72;                    // 5 carried + 64 new (+ slack)
+constexpr int kH1S = 40;                      // bf16 per layer-1 row (80 bytes)
+constexpr int kUPos = 24;                     // 8 carried + 16 new layer-2 positions
+constexpr int kCThreads = 512;
+constexpr int kCWaves = kCThreads / 64;
+constexpr int kXStep = kCThreads / kRows;    // 23 staging threads per input row
+constexpr int kXPer = (kXN + kXStep - 1) / kXStep;  // staged samples per thread
+constexpr size_t kXBytes = sizeof(__bf16) * kRows * kXStride;
+constexpr size_t kH1Bytes = sizeof(__bf16) * kH1Pos * kH1S;
+constexpr size_t kUBytes = sizeof(float) * kK * kUPos;
+constexpr size_t kZBytes = sizeof(float) * kSB * kCB;
+constexpr size_t kClassifyLds =
+    2 * kXBytes + kCB * (kH1Bytes + kUBytes) + 2 * kZBytes;
+static_assert(kCB == 2 * kCWaves, "two central channels per wave");
+static_assert(kClassifyLds <= 160 * 1024, "LDS");
+static_assert((kXBytes % 16) == 0 && (kH1Bytes % 16) == 0 && (kXStride * 2) % 16 == 0,
+              "alignment");
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
-    const float* __restrict__ X, int64_t M, int64_t N, int64_t nW, int nCG,
+    const float* __restrict__ X, int64_t M, int64_t N, int64_t nW, int nCB,
     const Frag* __restrict__ F, float lo, float hi, float* __restrict__ logits,
     uint8_t* __restrict__ classes) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    __bf16* xl = reinterpret_cast<__bf16*>(lds_raw);
-    __bf16* h1l = reinterpret_cast<__bf16*>(lds_raw + kXBytes);
-    float* h2l = reinterpret_cast<float*>(lds_raw + kXBytes + kH1Bytes);
+    __bf16* xbuf = reinterpret_cast<__bf16*>(lds_raw);                      // [2][kRows][kXStride]
+    __bf16* h1all = reinterpret_cast<__bf16*>(lds_raw + 2 * kXBytes);       // [kCB][kH1Pos][kH1S]
+    float* uall = reinterpret_cast<float*>(lds_raw + 2 * kXBytes + kCB * kH1Bytes);  // [kCB][kK][kUPos]
+    float* zt = uall + kCB * kK * kUPos;                                      // [2][kSB][kCB]
 
-    const int cg = static_cast<int>(blockIdx.x % nCG);
-    const int64_t tile = blockIdx.x / nCG;
-    const int64_t c0 = static_cast<int64_t>(cg) * kCB;
-    const int64_t w0 = tile * kWt;
-    const int64_t plo = w0 - 4;
-    const int64_t jlo = 4 * plo - 3;
-    const int64_t tx0 = 8 * jlo + 4;
+    const int cb = static_cast<int>(blockIdx.x % nCB);
+    const int64_t range = blockIdx.x / nCB;
+    const int64_t c0 = static_cast<int64_t>(cb) * kCB;
+    const int64_t wbase = range * (kRB * kSB);
+    int nb = kRB;  // real blocks of this range
+    if (wbase + static_cast<int64_t>(nb) * kSB > nW) nb = static_cast<int>((nW - wbase + kSB - 1) / kSB);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
+    const int n = lane & 15;
+    const int grp = lane >> 4;
 
     // weights into registers
-    bf16x8 a1[2][4];
+    bf16x8 a1[2][2], a2[2][9];
+    const bf16x8 a3 = F->a3[lane];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) a1[mt][s] = F->a1[mt][s][lane];
+#pragma unroll
+        for (int s = 0; s < 9; ++s) a2[mt][s] = F->a2[mt][s][lane];
+    }
+    float b1r[2][4], b2r[2][4];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) a1[mt][s] = F->a1[mt][s][lane];
-    const int nt2 = wv & 1, mt2 = wv >> 1;
-    bf16x8 a2[9];
-#pragma unroll
-    for (int s = 0; s < 9; ++s) a2[s] = F->a2[mt2][s][lane];
-    float b1r[2][4], b2r[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        b1r[0][r] = F->b1[4 * (lane >> 4) + r];
-        b1r[1][r] = F->b1[16 + 4 * (lane >> 4) + r];
-        b2r[r] = F->b2[16 * mt2 + 4 * (lane >> 4) + r];
-    }
-    const int wl3 = tid >> 3;         // layer-3 window of this thread
-    const int ig = tid & 7;           // its hidden channels 4ig..4ig+3
-    float w3r[4][kK];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < kK; ++k) w3r[i][k] = F->w3[(4 * ig + i) * kK + k];
+        for (int r = 0; r < 4; ++r) {
+            b1r[mt][r] = F->b1[16 * mt + 4 * grp + r];
+            b2r[mt][r] = F->b2[16 * mt + 4 * grp + r];
+        }
     const float b3 = F->b3;
 
-    // stage the x tile: rows = channels c0-3 .. c0+kCB+2, bf16
-    for (int q = tid; q < kRows * kXT; q += kCThreads) {
-        const int r = q % kRows;
-        const int tau = q / kRows;
-        const int64_t t = tx0 + tau;
-        const int64_t m = c0 - 3 + r;
-        float v = 0.0f;
-        if (t >= 0 && t < N && m >= 0 && m < M) v = X[t * M + m];
-        xl[r * kXT + tau] = static_cast<__bf16>(v);
-    }
+    // input samples of block ib (first window wbase + 16 (ib - 1)): registers,
+    // then LDS.  Thread (row fr, first sample fs) stages samples fs + 23 i of
+    // row fr; consecutive threads take consecutive rows, so a load
+    // instruction reads a few whole 88-byte runs of channels.
+    float pre[kXPer];
+    const int fr = tid % kRows;
+    const int fs = tid / kRows;
+    const int64_t fm = c0 - 3 + fr;
+    const bool fok = fs < kXStep && fm >= 0 && fm < M;
+    const float* frow = X + (fok ? fm : 0);
+    const uint64_t M64 = static_cast<uint64_t>(M);
+    const auto fetch = [&](int ib) {
+        const int64_t t0 = 32 * (wbase + kSB * static_cast<int64_t>(ib - 1)) + 148 + fs;
+#pragma unroll
+        for (int i = 0; i < kXPer; ++i) {
+            const int64_t t = t0 + kXStep * i;
+            const bool in = fok && fs + kXStep * i < kXN && t >= 0 && t < N;
+            const uint32_t tt = in ? static_cast<uint32_t>(t) : 0u;
+            const float v = frow[tt * M64];
+            pre[i] = in ? v : 0.0f;
+        }
+    };
+    const auto stash = [&](int buf) {
+        __bf16* xb = xbuf + buf * (kRows * kXStride) + fr * kXStride + fs;
+#pragma unroll
+        for (int i = 0; i < kXPer; ++i)
+            if (fs < kXStep && fs + kXStep * i < kXN) xb[kXStep * i] = static_cast<__bf16>(pre[i]);
+    };
+    fetch(0);
+    stash(0);
     __syncthreads();
 
-    for (int cc = 0; cc < kCB; ++cc) {
-        const int64_t c = c0 + cc;
-        if (c >= M) break;
-        // ---- layer 1: N-tiles wv and wv + 4
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int nt = wv + 4 * h;
-            const int jl = 16 * nt + (lane & 15);
-            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int gk = 4 * s + (lane >> 4);
-                const int ci = gk >> 1, part = gk & 1;
-                bf16x8 b;
-                if (ci < kCI) {
-                    b = *reinterpret_cast<const bf16x8*>(xl + (cc + ci) * kXT + 8 * jl + 8 * part);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) b[e] = static_cast<__bf16>(0.0f);
-                }
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[0][s], b, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[1][s], b, acc1, 0, 0, 0);
-            }
-            bf16x4 o0, o1;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                o0[r] = static_cast<__bf16>(fmaxf(acc0[r] + b1r[0][r], 0.0f));
-                o1[r] = static_cast<__bf16>(fmaxf(acc1[r] + b1r[1][r], 0.0f));
-            }
-            __bf16* dst = h1l + jl * kH1S + 4 * (lane >> 4);
-            *reinterpret_cast<bf16x4*>(dst) = o0;
-            *reinterpret_cast<bf16x4*>(dst + 16) = o1;
-        }
-        __syncthreads();
-        // ---- layer 2: N-tile nt2, M-tile mt2
-        {
-            const int pl = 16 * nt2 + (lane & 15);
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s = 0; s < 9; ++s) {
-                // positions past kWt + 8 are computed but unused; keep their reads in the tile
-                const int row = min(4 * pl + s, kNP1 - 1);
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(h1l + row * kH1S + 8 * (lane >> 4));
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[s], b, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                h2l[pl * kH2S + 16 * mt2 + 4 * (lane >> 4) + r] = fmaxf(acc[r] + b2r[r], 0.0f);
-        }
-        __syncthreads();
-        // ---- layer 3
-        {
-            float z = 0.0f;
-            if (wl3 < kWt) {
-#pragma unroll
-                for (int k = 0; k < kK; ++k)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) z += w3r[i][k] * h2l[(wl3 + k) * kH2S + 4 * ig + i];
-            }
-            z += __shfl_xor(z, 4, 64);
-            z += __shfl_xor(z, 2, 64);
-            z += __shfl_xor(z, 1, 64);
-            const int64_t w = w0 + wl3;
-            if (ig == 0 && wl3 < kWt && w < nW) {
-                z += b3;
+    for (int ib = 0; ib <= nb + 1; ++ib) {
+        // logits of block ib - 1 (block 0 fills the carries and is not written)
+        if (ib >= 2 && tid < kSB * kCB) {
+            const int wl = tid / kCB, cc = tid % kCB;
+            const int64_t w = wbase + kSB * static_cast<int64_t>(ib - 2) + wl;
+            const int64_t c = c0 + cc;
+            if (w < nW && c < M) {
+                const float z = zt[((ib - 1) & 1) * kSB * kCB + wl * kCB + cc];
                 if (logits) logits[w * M + c] = z;
                 if (classes) classes[w * M + c] = z < lo ? 0 : (z > hi ? 2 : 1);
             }
         }
+        if (ib == nb + 1) break;
+        const bool more = ib < nb;
+        if (more) fetch(ib + 1);
+        const __bf16* xb = xbuf + (ib & 1) * (kRows * kXStride);
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int cc = 2 * wv + h;
+            if (c0 + cc >= M) break;
+            __bf16* h1 = h1all + cc * (kH1Pos * kH1S);
+            float* u = uall + cc * (kK * kUPos);
+            // carries: layer-1 positions 64..68 -> 0..4, layer-3 sums 16..23 -> 0..7
+            if (lane < 20) {
+                const int q = lane >> 2, part = lane & 3;
+                uint2* dst = reinterpret_cast<uint2*>(h1 + q * kH1S) + part * 2;
+                const uint2* src = reinterpret_cast<const uint2*>(h1 + (64 + q) * kH1S) + part * 2;
+                const uint2 zero = make_uint2(0u, 0u);
+                const uint2 v0 = ib > 0 ? src[0] : zero, v1 = ib > 0 ? src[1] : zero;
+                dst[0] = v0;
+                dst[1] = v1;
+            }
+            for (int q = lane; q < kK * 8; q += 64) {
+                const int k = q >> 3, p = q & 7;
+                u[k * kUPos + p] = ib > 0 ? u[k * kUPos + 16 + p] : 0.0f;
+            }
+            wave_sync();
+            // ---- layer 1: 4 tiles of 16 new positions
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int jj = 16 * t + n;
+                const bf16x8 bA = *reinterpret_cast<const bf16x8*>(xb + (cc + grp) * kXStride + 8 * jj);
+                bf16x8 bB;
+                if (grp < 3) {
+                    bB = *reinterpret_cast<const bf16x8*>(xb + (cc + 4 + grp) * kXStride + 8 * jj);
+                } else {
+#pragma unroll
+                    for (int ci = 0; ci < kCI; ++ci) bB[ci] = xb[(cc + ci) * kXStride + 8 * jj + 8];
+                    bB[7] = static_cast<__bf16>(0.0f);
+                }
+                f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[0][0], bA, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[1][0], bA, acc1, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[0][1], bB, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[1][1], bB, acc1, 0, 0, 0);
+                bf16x4 o0, o1;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    o0[r] = static_cast<__bf16>(fmaxf(acc0[r] + b1r[0][r], 0.0f));
+                    o1[r] = static_cast<__bf16>(fmaxf(acc1[r] + b1r[1][r], 0.0f));
+                }
+                __bf16* dst = h1 + (5 + jj) * kH1S + 4 * grp;
+                *reinterpret_cast<bf16x4*>(dst) = o0;
+                *reinterpret_cast<bf16x4*>(dst + 16) = o1;
+            }
+            wave_sync();
+            // ---- layer 2: positions w0 + 4 + n, h1 rows 4n + s
+            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kK; ++s) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(h1 + (4 * n + s) * kH1S + 8 * grp);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][s], b, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][s], b, acc1, 0, 0, 0);
+            }
+            // ---- layer 3: u_k at positions w0 + 4 + n (rows k = 4 grp + r < 9)
+            bf16x8 hh, hl;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float v0 = fmaxf(acc0[r] + b2r[0][r], 0.0f);
+                const float v1 = fmaxf(acc1[r] + b2r[1][r], 0.0f);
+                hh[r] = static_cast<__bf16>(v0);
+                hh[4 + r] = static_cast<__bf16>(v1);
+                hl[r] = static_cast<__bf16>(v0 - static_cast<float>(hh[r]));
+                hl[4 + r] = static_cast<__bf16>(v1 - static_cast<float>(hh[4 + r]));
+            }
+            f32x4 uacc = {0.f, 0.f, 0.f, 0.f};
+            uacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, hl, uacc, 0, 0, 0);
+            uacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, hh, uacc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (4 * grp + r < kK) u[(4 * grp + r) * kUPos + 8 + n] = uacc[r];
+            wave_sync();
+            if (lane < kSB) {
+                float z = b3;
+#pragma unroll
+                for (int k = 0; k < kK; ++k) z += u[k * kUPos + lane + k];
+                zt[(ib & 1) * kSB * kCB + lane * kCB + cc] = z;
+            }
+        }
+        if (more) stash((ib + 1) & 1);
         __syncthreads();
     }
 }
@@ -559,13 +646,14 @@ int classify(const float* X, int64_t M, int64_t N, const void* frag, float lo_lo
              float hi_logit, float* logits, uint8_t* classes, hipStream_t s) {
     const int64_t nW = num_windows(N);
     if (nW == 0 || M == 0) return MSAMD_OK;
-    const int nCG = static_cast<int>(ceil_div(M, kCB));
-    const int64_t nTiles = ceil_div(nW, kWt);
+    const int nCB = static_cast<int>(ceil_div(M, kCB));
+    const int64_t nR = ceil_div(nW, static_cast<int64_t>(kRB) * kSB);
+    MSAMD_REQUIRE(nR * nCB < (1LL << 31), "tcn classify: recording too long");
     MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tcn_classify_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(kClassifyLds)));
-    tcn_classify_kernel<<<static_cast<unsigned>(nCG * nTiles), kCThreads, kClassifyLds, s>>>(
-        X, M, N, nW, nCG, static_cast<const Frag*>(frag), lo_logit, hi_logit, logits, classes);
+    tcn_classify_kernel<<<static_cast<unsigned>(nCB * nR), kCThreads, kClassifyLds, s>>>(
+        X, M, N, nW, nCB, static_cast<const Frag*>(frag), lo_logit, hi_logit, logits, classes);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }
