@@ -465,78 +465,154 @@ __global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTabl
 }
 
 // ----------------------------------------------------------- power method
-// One workgroup per Gram matrix (n x n, n <= 128 held in LDS as FP64).
+// One workgroup per Gram matrix (n x n, FP64): held in registers for
+// n <= 128 (power_method_reg), else in LDS or the workspace (power_method).
 // Components are written as float [row][component].
 constexpr int kPowMaxN = 128;
 // Row stride of a Gram matrix held in LDS: n rounded up to odd.
 __host__ __device__ inline int pow_ld(int n) { return n | 1; }
 
-__device__ double pow_block_sum(double x, double* red) {
+// Sum over i < n of f(i), computed by every wavefront of the workgroup on
+// its own (lane l adds i = l, l + 64, ... in order, then a butterfly over the
+// lanes): each wave gets the same value in the same order, with no barrier.
+template <typename F>
+__device__ __forceinline__ double wave_sum(int n, F f) {
     const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) red[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s = s + red[w];
-        red[8] = s;
-    }
-    __syncthreads();
-    const double s = red[8];
-    __syncthreads();
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += f(i);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     return s;
 }
 
-// Gs is n x n with row stride ld (odd in LDS: thread r reads row r, and an
-// odd stride puts 32 consecutive rows' doubles in distinct bank pairs).
-__device__ void power_method(double* Gs, int n, int ld, int K, const double* sin_tab, float* comp,
-                             double* v, double* w, double* red) {
+// Gs is n x n with row stride ld (odd in LDS, so that the rows read
+// together start on different banks).  Matrix-vector products give each row
+// tpr adjacent threads (4 for n <= 64, 2 for n <= 128), each summing every
+// tpr-th column into two accumulators, combined across the threads by
+// shuffles; the norms and the deflation's v'Gv are summed by every wave on
+// its own (wave_sum), so an iteration has two barriers.
+__device__ __forceinline__ void power_method(double* Gs, int n, int ld, int K, const double* sin_tab, float* comp,
+                             double* v, double* w) {
+    const int nt = blockDim.x;
+    const int tpr = n <= 64 ? 4 : (n <= 128 ? 2 : 1);
+    const int sub = threadIdx.x % tpr;
+    const int rpp = nt / tpr;  // rows per pass
     for (int k = 0; k < K; ++k) {
         // v = normalize(sin(1..n))
-        double part = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            v[i] = sin_tab[i];
-            part += v[i] * v[i];
-        }
-        double nrm = sqrt(pow_block_sum(part, red));
-        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = nrm != 0.0 ? v[i] / nrm : v[i];
+        const double nrm = sqrt(wave_sum(n, [&](int i) { return sin_tab[i] * sin_tab[i]; }));
+        for (int i = threadIdx.x; i < n; i += nt) v[i] = nrm != 0.0 ? sin_tab[i] / nrm : sin_tab[i];
         __syncthreads();
         for (int it = 0; it < 11; ++it) {
-            for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                const double* row = Gs + r * ld;
-                double a = 0.0;
-                for (int c = 0; c < n; ++c) a += row[c] * v[c];
-                w[r] = a;
+            for (int r0 = 0; r0 < n; r0 += rpp) {
+                const int r = r0 + static_cast<int>(threadIdx.x) / tpr;
+                double a0 = 0.0, a1 = 0.0;
+                if (r < n) {
+                    const double* row = Gs + static_cast<int64_t>(r) * ld;
+                    int c = sub;
+                    for (; c + tpr < n; c += 2 * tpr) {
+                        a0 += row[c] * v[c];
+                        a1 += row[c + tpr] * v[c + tpr];
+                    }
+                    if (c < n) a0 += row[c] * v[c];
+                }
+                double a = a0 + a1;
+                for (int o = 1; o < tpr; o <<= 1) a += __shfl_xor(a, o, 64);
+                if (r < n && sub == 0) w[r] = a;
             }
             __syncthreads();
             if (it < 10) {
-                part = 0.0;
-                for (int i = threadIdx.x; i < n; i += blockDim.x) part += w[i] * w[i];
-                const double sigma = sqrt(pow_block_sum(part, red));
-                for (int i = threadIdx.x; i < n; i += blockDim.x)
-                    v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
+                const double sigma = sqrt(wave_sum(n, [&](int i) { return w[i] * w[i]; }));
+                for (int i = threadIdx.x; i < n; i += nt) v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
                 __syncthreads();
             } else {
                 // deflation with u = G v
-                part = 0.0;
-                for (int i = threadIdx.x; i < n; i += blockDim.x) part += v[i] * w[i];
-                const double alpha = pow_block_sum(part, red);
-                for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
+                const double alpha = wave_sum(n, [&](int i) { return v[i] * w[i]; });
+                for (int q = threadIdx.x; q < n * n; q += nt) {
                     const int i = q / n;
                     const int j = q - i * n;
-                    double& gq = Gs[i * ld + j];
+                    double& gq = Gs[static_cast<int64_t>(i) * ld + j];
                     gq = (gq - (v[i] * w[j] + w[i] * v[j])) + alpha * (v[i] * v[j]);
                 }
                 __syncthreads();
             }
         }
-        for (int i = threadIdx.x; i < n; i += blockDim.x) comp[i * K + k] = static_cast<float>(v[i]);
+        for (int i = threadIdx.x; i < n; i += nt) comp[i * K + k] = static_cast<float>(v[i]);
         __syncthreads();
     }
 }
 
-template <int kStage>
+// The same power method for n <= 64 * kHalves with G held in registers: 4
+// adjacent threads per row, each with a contiguous block of ceil(n/4) <=
+// kCols columns of up to kHalves rows (rows r, r + 64).  A matrix-vector
+// product is then register FMAs against v from LDS and two shuffles; the
+// deflation updates the registers in place.  Only v and w (2n doubles) live
+// in LDS, so many Gram matrices run on one CU at once.
+template <int kHalves, int kCols>
+__device__ __forceinline__ void power_method_reg(const double* __restrict__ Gsrc, int n, int K,
+                                 const double* sin_tab, float* comp, double* v, double* w) {
+    constexpr int kRegCols = kCols;
+    const int nt = blockDim.x;
+    const int sub = threadIdx.x & 3;
+    const int cpt = (n + 3) / 4;
+    const int c0 = sub * cpt;
+    const int nc = c0 + cpt <= n ? cpt : (n > c0 ? n - c0 : 0);
+    double g[kHalves][kRegCols];
+    int rr[kHalves];
+#pragma unroll
+    for (int h = 0; h < kHalves; ++h) {
+        rr[h] = h * 64 + static_cast<int>(threadIdx.x >> 2);
+#pragma unroll
+        for (int k = 0; k < kRegCols; ++k)
+            g[h][k] = (rr[h] < n && k < nc) ? Gsrc[static_cast<int64_t>(rr[h]) * n + c0 + k] : 0.0;
+    }
+    for (int k = 0; k < K; ++k) {
+        const double nrm = sqrt(wave_sum(n, [&](int i) { return sin_tab[i] * sin_tab[i]; }));
+        for (int i = threadIdx.x; i < n; i += nt) v[i] = nrm != 0.0 ? sin_tab[i] / nrm : sin_tab[i];
+        __syncthreads();
+        for (int it = 0; it < 11; ++it) {
+#pragma unroll
+            for (int h = 0; h < kHalves; ++h) {
+                double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                for (int q = 0; q < kRegCols; q += 2) {
+                    if (q < nc) a0 += g[h][q] * v[c0 + q];
+                    if (q + 1 < nc) a1 += g[h][q + 1] * v[c0 + q + 1];
+                }
+                double a = a0 + a1;
+                a += __shfl_xor(a, 1, 64);
+                a += __shfl_xor(a, 2, 64);
+                if (rr[h] < n && sub == 0) w[rr[h]] = a;
+            }
+            __syncthreads();
+            if (it < 10) {
+                const double sigma = sqrt(wave_sum(n, [&](int i) { return w[i] * w[i]; }));
+                for (int i = threadIdx.x; i < n; i += nt) v[i] = sigma != 0.0 ? w[i] / sigma : w[i];
+                __syncthreads();
+            } else {
+                // deflation with u = G v, on the registers
+                const double alpha = wave_sum(n, [&](int i) { return v[i] * w[i]; });
+#pragma unroll
+                for (int h = 0; h < kHalves; ++h) {
+                    const bool row_ok = rr[h] < n;
+                    const double vi = row_ok ? v[rr[h]] : 0.0, wi = row_ok ? w[rr[h]] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < kRegCols; ++q) {
+                        if (row_ok && q < nc) {
+                            const double vj = v[c0 + q], wj = w[c0 + q];
+                            g[h][q] = (g[h][q] - (vi * wj + wi * vj)) + alpha * (vi * vj);
+                        }
+                    }
+                }
+                __syncthreads();  // v and w are rewritten by the next component
+            }
+        }
+        for (int i = threadIdx.x; i < n; i += nt) comp[i * K + k] = static_cast<float>(v[i]);
+        __syncthreads();
+    }
+}
+
+// kReg: 1 -> G in registers for n <= 64, 2 -> for n <= 128, 0 -> G in LDS
+// or the workspace (any n).
+template <int kStage, int kReg>
 __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb) {
     extern __shared__ double psh[];
     int g, q;
@@ -554,19 +630,27 @@ __global__ void __launch_bounds__(256) pca_power_kernel(PcaPlan p, PcaTables tb)
                                      : tb.g2 + tb.g2_off[g];
     float* comp = kStage == 1 ? tb.c1 + tb.c1_off[g] + static_cast<int64_t>(q) * n * K
                               : tb.c2 + tb.c2_off[g];
-    // the FP64 copy of G lives in LDS when it fits, otherwise in the workspace
+    if (kReg == 1) {  // G in registers, v and w in LDS
+        power_method_reg<1, 16>(Gsrc, n, K, tb.sin_tab, comp, psh, psh + n);
+        return;
+    }
+    if (kReg == 2) {
+        power_method_reg<2, 32>(Gsrc, n, K, tb.sin_tab, comp, psh, psh + n);
+        return;
+    }
+    // larger Gram matrices: the FP64 copy of G lives in LDS when it fits,
+    // otherwise in the workspace
     const bool in_lds = kStage == 1 || p.nf1 * p.maxW <= kPowMaxN;  // decided per launch
     double* Gs = in_lds ? psh : tb.work2 + tb.w2_off[g];
     const int ld = in_lds ? pow_ld(n) : n;
     double* v = in_lds ? Gs + n * ld : psh;
     double* w = v + n;
-    double* red = w + n;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e / n;
         Gs[i * ld + (e - i * n)] = Gsrc[e];
     }
     __syncthreads();
-    power_method(Gs, n, ld, K, tb.sin_tab, comp, v, w, red);
+    power_method(Gs, n, ld, K, tb.sin_tab, comp, v, w);
 }
 
 // ------------------------------------------------------------ projections
@@ -1056,6 +1140,7 @@ struct msamd_pca_plan {
     PcaPlan p{};
     PcaLayout l;
     Tiles tiles;
+    int pow2_reg = 0;  // stage-2 power kernel form (pca_power_kernel's kReg)
     size_t lds_gram1 = 0, lds_power1 = 0, lds_proj1 = 0, lds_gram2 = 0, lds_power2 = 0,
            lds_proj2 = 0;
     // hipGraph of the launch sequence, keyed by the buffers it was captured with
@@ -1095,10 +1180,14 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     // stage-2 Gram items stage one band (up to 240 rows) or a pair of 128-row bands
     const int rows2 = n2 <= 240 ? n2 : 256;
     P->lds_gram1 = sizeof(_Float16) * (kPieces * gw * gram_rows_alloc(T) + 16) * kRowStride;
-    P->lds_power1 = sizeof(double) * (T * pow_ld(T) + 2 * T + 16);
+    // stage 1 (n = T <= 64): G in registers, v and w in LDS
+    P->lds_power1 = sizeof(double) * 2 * T;
     P->lds_proj1 = sizeof(float) * pw * T * cpad_for(nf1);
     P->lds_gram2 = sizeof(_Float16) * (kPieces * gram_rows_alloc(rows2) + 16) * kRowStride;
-    P->lds_power2 = sizeof(double) * ((n2 <= kPowMaxN ? n2 * pow_ld(n2) : 0) + 2 * n2 + 16);
+    // stage 2: G in registers for n2 <= 128 (v and w in LDS), else G in LDS or
+    // the workspace
+    P->pow2_reg = n2 <= 64 ? 1 : (n2 <= kPowMaxN ? 2 : 0);
+    P->lds_power2 = sizeof(double) * (P->pow2_reg ? 2 * n2 : 2 * n2 + 16);
     P->lds_proj2 = static_cast<size_t>(project2_lds(nf1, P->h.maxW, nf2, t.pe2));
     return MSAMD_OK;
 }
@@ -1162,7 +1251,7 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
                            : reinterpret_cast<const void*>(&pca_project1_kernel<16, true>),
                        P->lds_proj1));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_gram_kernel<2>), P->lds_gram2));
-    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_power_kernel<2>), P->lds_power2));
+    MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_power_kernel<2, 0>), P->lds_power2));
     MSAMD_HIP(lds_attr(reinterpret_cast<const void*>(&pca_project2_kernel), P->lds_proj2));
     return MSAMD_OK;
 }
@@ -1186,7 +1275,12 @@ int enqueue_stage2(const msamd_pca_plan* P, float* reduced, float* features, hip
         MSAMD_LAUNCH_CHECK();
     }
     // stage 2 power method
-    pca_power_kernel<2><<<G, 256, P->lds_power2, st>>>(p, l.tb);
+    if (P->pow2_reg == 1)
+        pca_power_kernel<2, 1><<<G, 256, P->lds_power2, st>>>(p, l.tb);
+    else if (P->pow2_reg == 2)
+        pca_power_kernel<2, 2><<<G, 256, P->lds_power2, st>>>(p, l.tb);
+    else
+        pca_power_kernel<2, 0><<<G, 256, P->lds_power2, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     if (!h.j2.empty()) {  // stage 2 projections
         pca_project2_kernel<<<static_cast<unsigned>(h.j2.size() / 2), 256, P->lds_proj2, st>>>(
@@ -1225,7 +1319,7 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
         MSAMD_LAUNCH_CHECK();
     }
     // stage 1 power method
-    pca_power_kernel<1><<<static_cast<unsigned>(h.p1.size() / 2), 256, P->lds_power1, st>>>(p, l.tb);
+    pca_power_kernel<1, 1><<<static_cast<unsigned>(h.p1.size() / 2), 256, P->lds_power1, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     if (!h.j1.empty()) {  // stage 1 projections
         const unsigned g1 = static_cast<unsigned>(h.j1.size() / 3);
