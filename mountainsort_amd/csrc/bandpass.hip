@@ -616,6 +616,111 @@ int bp_streams() {
 // Per-device side streams and fork/join events, created on first use.  The
 // issue of one filter call's launches holds the device's lock, so the event
 // pairs of calls from several host threads do not interleave.
+// Host tables of one filter shape (the gain H/n transposed to [k1][k2],
+// the Stockham and four-step twiddles), computed once per parameter set in
+// pinned host memory, so that every later call uploads them with
+// asynchronous copies on its stream instead of recomputing 240000 erf
+// values and waiting for pageable copies.  Entries live until the process
+// ends (at most kMaxTables parameter sets; beyond that the tables are built
+// per call).
+struct BpTables {
+    int64_t n;
+    int32_t N1, N2, nTwHi;
+    double fs, fmin, fmax, fwid;
+    char* host;       // pinned: Ht | twN1 | twN2 | twHi | twLo
+    size_t off[5], len[5];
+};
+constexpr int kMaxTables = 32;
+std::mutex g_tables_mu;
+std::vector<BpTables*> g_tables;
+
+void fill_tables(const BpPlan& p, double fs, double fmin, double fmax, double fwid, BpTables* t) {
+    std::vector<double> H(p.n);
+    msamd_bandpass_kernel(p.n, fs, fmin, fmax, fwid, H.data());
+    const double factor = 1.0 / p.n;
+    float* Ht = reinterpret_cast<float*>(t->host + t->off[0]);
+    for (int64_t k1 = 0; k1 < p.N1; ++k1)
+        for (int64_t k2 = 0; k2 < p.N2; ++k2)
+            Ht[k1 * p.N2 + k2] = static_cast<float>(H[k1 + p.N1 * k2] * factor);
+    auto table = [](cf* dst, int64_t len, int64_t period, int64_t step) {
+        for (int64_t m = 0; m < len; ++m) {
+            const long double ang = -2.0L * 3.14159265358979323846264338327950288L *
+                                    static_cast<long double>((m * step) % period) / period;
+            dst[m] = {static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang))};
+        }
+    };
+    table(reinterpret_cast<cf*>(t->host + t->off[1]), p.N1, p.N1, 1);
+    table(reinterpret_cast<cf*>(t->host + t->off[2]), p.N2, p.N2, 1);
+    table(reinterpret_cast<cf*>(t->host + t->off[3]), p.nTwHi, p.n, kTwLo);
+    table(reinterpret_cast<cf*>(t->host + t->off[4]), kTwLo, p.n, 1);
+}
+
+void layout_tables(const BpPlan& p, BpTables* t) {
+    const size_t len[5] = {sizeof(float) * p.n, sizeof(cf) * p.N1, sizeof(cf) * p.N2,
+                           sizeof(cf) * p.nTwHi, sizeof(cf) * kTwLo};
+    size_t o = 0;
+    for (int i = 0; i < 5; ++i) {
+        t->off[i] = o;
+        t->len[i] = len[i];
+        o += align_up(static_cast<int64_t>(len[i]), 256);
+    }
+}
+
+size_t tables_bytes(const BpTables& t) { return t.off[4] + t.len[4]; }
+
+// Enqueues the upload of the tables into b (asynchronous when they are cached).
+int upload_tables(const BpPlan& p, double fs, double fmin, double fmax, double fwid,
+                  const BpBuffers& b, hipStream_t st) {
+    BpTables* hit = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_tables_mu);
+        for (BpTables* t : g_tables)
+            if (t->n == p.n && t->N1 == p.N1 && t->N2 == p.N2 && t->nTwHi == p.nTwHi &&
+                t->fs == fs && t->fmin == fmin && t->fmax == fmax && t->fwid == fwid) {
+                hit = t;
+                break;
+            }
+        if (!hit && static_cast<int>(g_tables.size()) < kMaxTables) {
+            BpTables* t = new BpTables{};
+            t->n = p.n;
+            t->N1 = p.N1;
+            t->N2 = p.N2;
+            t->nTwHi = p.nTwHi;
+            t->fs = fs;
+            t->fmin = fmin;
+            t->fmax = fmax;
+            t->fwid = fwid;
+            layout_tables(p, t);
+            void* host = nullptr;
+            if (hipHostMalloc(&host, tables_bytes(*t), hipHostMallocDefault) == hipSuccess) {
+                t->host = static_cast<char*>(host);
+                fill_tables(p, fs, fmin, fmax, fwid, t);
+                g_tables.push_back(t);
+                hit = t;
+            } else {
+                delete t;
+                (void)hipGetLastError();  // no pinned memory: build per call below
+            }
+        }
+    }
+    void* const dst[5] = {b.Ht, b.twN1, b.twN2, b.twHi, b.twLo};
+    if (hit) {
+        for (int i = 0; i < 5; ++i)
+            MSAMD_HIP(hipMemcpyAsync(dst[i], hit->host + hit->off[i], hit->len[i],
+                                     hipMemcpyHostToDevice, st));
+        return MSAMD_OK;
+    }
+    BpTables t{};
+    layout_tables(p, &t);
+    std::vector<char> host(tables_bytes(t));
+    t.host = host.data();
+    fill_tables(p, fs, fmin, fmax, fwid, &t);
+    for (int i = 0; i < 5; ++i)
+        MSAMD_HIP(hipMemcpyAsync(dst[i], t.host + t.off[i], t.len[i], hipMemcpyHostToDevice, st));
+    MSAMD_HIP(hipStreamSynchronize(st));  // pageable source goes out of scope
+    return MSAMD_OK;
+}
+
 struct SideStream {
     hipStream_t stream[kMaxStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kMaxStreams - 1] = {};
@@ -791,32 +896,8 @@ int bandpass_run(const void* X, int dtype, int64_t M, int64_t N, int64_t xt0, in
     // host tables: gain H/n transposed to [k1][k2], Stockham twiddles,
     // four-step twiddle factors (all computed in double, rounded once)
     {
-        std::vector<double> H(p.n);
-        msamd_bandpass_kernel(p.n, samplerate, freq_min, freq_max, freq_wid, H.data());
-        const double factor = 1.0 / p.n;
-        std::vector<float> Ht(p.n);
-        for (int64_t k1 = 0; k1 < p.N1; ++k1)
-            for (int64_t k2 = 0; k2 < p.N2; ++k2)
-                Ht[k1 * p.N2 + k2] = static_cast<float>(H[k1 + p.N1 * k2] * factor);
-        auto table = [](int64_t len, int64_t period, int64_t step) {
-            std::vector<cf> t(len);
-            for (int64_t m = 0; m < len; ++m) {
-                const long double ang = -2.0L * 3.14159265358979323846264338327950288L *
-                                        static_cast<long double>((m * step) % period) / period;
-                t[m] = {static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang))};
-            }
-            return t;
-        };
-        const std::vector<cf> t1 = table(p.N1, p.N1, 1);
-        const std::vector<cf> t2 = table(p.N2, p.N2, 1);
-        const std::vector<cf> th = table(p.nTwHi, p.n, kTwLo);
-        const std::vector<cf> tl = table(kTwLo, p.n, 1);
-        MSAMD_HIP(hipMemcpyAsync(b.Ht, Ht.data(), sizeof(float) * p.n, hipMemcpyHostToDevice, st));
-        MSAMD_HIP(hipMemcpyAsync(b.twN1, t1.data(), sizeof(cf) * p.N1, hipMemcpyHostToDevice, st));
-        MSAMD_HIP(hipMemcpyAsync(b.twN2, t2.data(), sizeof(cf) * p.N2, hipMemcpyHostToDevice, st));
-        MSAMD_HIP(hipMemcpyAsync(b.twHi, th.data(), sizeof(cf) * p.nTwHi, hipMemcpyHostToDevice, st));
-        MSAMD_HIP(hipMemcpyAsync(b.twLo, tl.data(), sizeof(cf) * kTwLo, hipMemcpyHostToDevice, st));
-        MSAMD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
+        const int rc = upload_tables(p, samplerate, freq_min, freq_max, freq_wid, b, st);
+        if (rc != MSAMD_OK) return rc;
     }
     // whole-line staging of the int16 input (stage A) when a unit holds 32
     // pairs and rows are 16-byte aligned

@@ -24,6 +24,7 @@
 // array or, fused with the clip gather, the timeseries itself (GatherSrc).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -1192,6 +1193,19 @@ int plan_build(msamd_pca_plan* P, int32_t G, const int64_t* L, const int32_t* W,
     return MSAMD_OK;
 }
 
+// Pinned staging for the plan's table image, one grow-only buffer per
+// device: the upload is then an asynchronous copy on the caller's stream, and
+// the next upload waits for that copy (an event) before refilling the
+// buffer.
+struct PlanStaging {
+    std::mutex mu;
+    char* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+constexpr int kMaxDevices = 16;
+PlanStaging g_staging[kMaxDevices];
+
 // Copies the host tables into the carved workspace (stream-ordered; the
 // tables live as long as the plan).
 int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hipStream_t st) {
@@ -1208,11 +1222,45 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
         for (int i = 0; i < kSinTab; ++i) v[i] = std::sin(static_cast<double>(i + 1));
         return v;
     }();
-    // one host image of every table at its workspace offset, one copy
-    std::vector<char> img(static_cast<size_t>(l.table_bytes));
+    // one host image of every table at its workspace offset, one copy: from
+    // the device's pinned staging buffer (asynchronous), else from pageable
+    // memory (waited for)
+    const size_t need = static_cast<size_t>(l.table_bytes);
+    int dev = 0;
+    MSAMD_HIP(hipGetDevice(&dev));
+    PlanStaging* stg = dev >= 0 && dev < kMaxDevices ? &g_staging[dev] : nullptr;
+    std::unique_lock<std::mutex> stg_lock;
+    std::vector<char> pageable;
+    char* img = nullptr;
+    if (stg) {
+        stg_lock = std::unique_lock<std::mutex>(stg->mu);
+        if (stg->done) MSAMD_HIP(hipEventSynchronize(stg->done));  // the last copy has read it
+        if (stg->cap < need) {
+            if (stg->buf) (void)hipHostFree(stg->buf);
+            stg->buf = nullptr;
+            stg->cap = 0;
+            void* b = nullptr;
+            if (hipHostMalloc(&b, need, hipHostMallocDefault) == hipSuccess) {
+                stg->buf = static_cast<char*>(b);
+                stg->cap = need;
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+        if (stg->buf && !stg->done &&
+            hipEventCreateWithFlags(&stg->done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            stg->done = nullptr;
+        }
+        if (stg->buf && stg->done) img = stg->buf;
+    }
+    if (!img) {
+        pageable.resize(need);
+        img = pageable.data();
+    }
     char* const wsb = static_cast<char*>(workspace);
     auto put = [&](const void* dst, const void* src, size_t n) {
-        if (n) std::memcpy(img.data() + (static_cast<const char*>(dst) - wsb), src, n);
+        if (n) std::memcpy(img + (static_cast<const char*>(dst) - wsb), src, n);
     };
     put(l.d_clip_off, h.clip_off.data(), h.clip_off.size() * 8);
     put(l.d_red_off, h.red_off.data(), h.red_off.size() * 8);
@@ -1234,8 +1282,11 @@ int plan_upload(msamd_pca_plan* P, void* workspace, int64_t workspace_bytes, hip
     put(l.d_gp1_off, h.gp1_off.data(), h.gp1_off.size() * 8);
     put(l.d_gp2_off, h.gp2_off.data(), h.gp2_off.size() * 8);
     put(l.d_ev_off, h.ev_off.data(), h.ev_off.size() * 8);
-    MSAMD_HIP(hipMemcpyAsync(workspace, img.data(), img.size(), hipMemcpyHostToDevice, st));
-    MSAMD_HIP(hipStreamSynchronize(st));  // pageable source: the copy completes before return
+    MSAMD_HIP(hipMemcpyAsync(workspace, img, need, hipMemcpyHostToDevice, st));
+    if (img == pageable.data())
+        MSAMD_HIP(hipStreamSynchronize(st));  // pageable source: the copy completes before return
+    else
+        MSAMD_HIP(hipEventRecord(stg->done, st));
     auto lds_attr = [](const void* f, size_t bytes) {
         return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(bytes));
