@@ -121,43 +121,66 @@ struct GatherSrc {
 };
 
 // ----------------------------------------------------- subsample indices
-// pca_subsampled, pca.cpp:106-115: one thread per neighborhood, the same
-// sequential double accumulation as the reference.
-__global__ void pca_subsample_kernel(PcaPlan p, PcaTables tb) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= p.G) return;
+// pca_subsampled, pca.cpp:106-115: the indices (int64)i for i = 0, inc, 2inc,
+// ... < L, accumulated sequentially in double.  One workgroup per
+// neighborhood: thread 0 runs the sequential accumulation and keeps its
+// value at every kSubSeg-th index (checkpoints in LDS); then every thread
+// replays kSubSeg additions from one checkpoint and writes those indices.
+// The replay repeats the same additions in the same order, so the indices
+// are those of the one sequential loop.  Neighborhoods with more checkpoints
+// than the LDS holds run the whole loop on thread 0.
+constexpr int kSubSeg = 64;
+constexpr int kSubCk = 4096;
+
+__global__ void __launch_bounds__(256) pca_subsample_kernel(PcaPlan p, PcaTables tb) {
+    __shared__ double ck[kSubCk];
+    __shared__ int64_t nseg_s;
+    const int g = blockIdx.x;
     const int64_t L = tb.L[g];
     if (L <= p.max_s) {
-        tb.ns[g] = static_cast<int32_t>(L);
+        if (threadIdx.x == 0) tb.ns[g] = static_cast<int32_t>(L);
         return;
     }
     const double increment = L * 1.0 / p.max_s;
-    int32_t* out = tb.idx + g * p.idx_stride;  // 16-byte aligned: idx_stride % 4 == 0
-    int64_t c = 0;
-    double i = 0;
-    // four indices per store: one lane per neighborhood writes its own list,
-    // so single-index stores would touch 64 cache lines per instruction
-    for (;;) {
-        int32_t v[4];
-        int n = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (n == q && i < L) {
-                v[q] = static_cast<int32_t>(static_cast<int64_t>(i));
-                i += increment;
-                n = q + 1;
-            }
+    int32_t* out = tb.idx + g * p.idx_stride;
+    const int64_t cap = p.idx_stride;
+    const int64_t maxseg = (cap + kSubSeg - 1) / kSubSeg;
+    if (maxseg > kSubCk) {  // one thread, the whole loop
+        if (threadIdx.x == 0) {
+            int64_t c = 0;
+            for (double i = 0; i < L && c < cap; i += increment) out[c++] = static_cast<int32_t>(static_cast<int64_t>(i));
+            tb.ns[g] = static_cast<int32_t>(c);
         }
-        if (n == 4 && c + 4 <= p.idx_stride) {
-            *reinterpret_cast<int4*>(out + c) = make_int4(v[0], v[1], v[2], v[3]);
-        } else {
-            for (int q = 0; q < n; ++q)
-                if (c + q < p.idx_stride) out[c + q] = v[q];
-        }
-        c += n;
-        if (n < 4) break;
+        return;
     }
-    tb.ns[g] = static_cast<int32_t>(c < p.idx_stride ? c : p.idx_stride);
+    // checkpoints: the accumulated value at indices 0, kSubSeg, 2 kSubSeg, ...
+    // (the values increase, so a segment whose end value is below L lies
+    // wholly below it; the segment that reaches L holds the last index)
+    if (threadIdx.x == 0) {
+        double i = 0;
+        int64_t seg = 0;
+        for (;;) {
+            ck[seg++] = i;
+            double j = i;
+#pragma unroll
+            for (int k = 0; k < kSubSeg; ++k) j += increment;
+            if (!(j < L) || seg >= maxseg) break;
+            i = j;
+        }
+        nseg_s = seg;
+    }
+    __syncthreads();
+    const int64_t nseg = nseg_s;
+    for (int64_t seg = threadIdx.x; seg < nseg; seg += blockDim.x) {
+        double i = ck[seg];
+        int64_t c = seg * kSubSeg;
+        const int64_t c1 = c + kSubSeg < cap ? c + kSubSeg : cap;
+        for (; c < c1 && i < L; ++c) {
+            out[c] = static_cast<int32_t>(static_cast<int64_t>(i));
+            i += increment;
+        }
+        if (seg == nseg - 1) tb.ns[g] = static_cast<int32_t>(c);
+    }
 }
 
 // ------------------------------------------------------------------ Gram
@@ -1351,7 +1374,7 @@ int plan_enqueue(const msamd_pca_plan* P, const float* clips, float* reduced, fl
     const PcaPlan& p = P->p;
     const PcaLayout& l = P->l;
     const int32_t G = p.G, T = p.T, nf1 = p.nf1;
-    pca_subsample_kernel<<<static_cast<unsigned>(ceil_div(G, 64)), 64, 0, st>>>(p, l.tb);
+    pca_subsample_kernel<<<static_cast<unsigned>(G), 256, 0, st>>>(p, l.tb);
     MSAMD_LAUNCH_CHECK();
     if (P->vectors) return enqueue_stage2(P, reduced, features, st);
     // stage 1 Gram: chunk partials, then per-neighborhood FP64 sums
