@@ -128,7 +128,7 @@ __global__ void tcn_prep_kernel(const float* __restrict__ P, Frag* __restrict__ 
 // last 5 positions and the layer-3 partial sums of the last 8 layer-2
 // positions are carried to the next block, so no position is computed
 // twice; one extra block before the range fills the carries (its logits
-// are dropped).  The input rows of a block (kCB + 6 channels, bf16) are
+// are dropped).  The input rows of a block (kCB + 8 channels, bf16) are
 // staged in LDS once for all waves and double-buffered: the next block's
 // samples are loaded into registers before this block's arithmetic.  Each
 // wave owns two central channels and their carried state, so the only
@@ -141,7 +141,9 @@ __global__ void tcn_prep_kernel(const float* __restrict__ P, Frag* __restrict__ 
 constexpr int kSB = 16;                       // windows per block
 constexpr int kCB = 16;                       // central channels per workgroup
 constexpr int kRB = 64;                       // blocks per workgroup range
-constexpr int kRows = kCB + kCI - 1;          // 22 input rows
+// input rows: channels c0-4 .. c0+kCB+3 (central cc uses rows cc+1 .. cc+7),
+// starting on a 16-byte boundary so that rows load as float4 when M % 4 == 0
+constexpr int kRows = kCB + 8;                // 24 input rows
 constexpr int kXN = 8 * 64 + 1;               // 513 input samples per block
 constexpr int kXStride = 520;                 // bf16 per input row (1040 bytes)
 constexpr int kH1Pos = 72;                    // 5 carried + 64 new (+ slack)
@@ -149,8 +151,10 @@ constexpr int kH1S = 40;                      // bf16 per layer-1 row (80 bytes)
 constexpr int kUPos = 24;                     // 8 carried + 16 new layer-2 positions
 constexpr int kCThreads = 512;
 constexpr int kCWaves = kCThreads / 64;
-constexpr int kXStep = kCThreads / kRows;    // 23 staging threads per input row
-constexpr int kXPer = (kXN + kXStep - 1) / kXStep;  // staged samples per thread
+constexpr int kXStep = kCThreads / kRows;    // 21 staging threads per input row (scalar form)
+constexpr int kXPer = (kXN + kXStep - 1) / kXStep;  // staged samples per thread (scalar form)
+constexpr int kQ4 = kRows / 4;                // float4 pieces per sample (vector form)
+constexpr int kXPer4 = (kQ4 * kXN + kCThreads - 1) / kCThreads;  // pieces per thread
 constexpr size_t kXBytes = sizeof(__bf16) * kRows * kXStride;
 constexpr size_t kH1Bytes = sizeof(__bf16) * kH1Pos * kH1S;
 constexpr size_t kUBytes = sizeof(float) * kK * kUPos;
@@ -162,12 +166,13 @@ static_assert(kClassifyLds <= 160 * 1024, "LDS");
 static_assert((kXBytes % 16) == 0 && (kH1Bytes % 16) == 0 && (kXStride * 2) % 16 == 0,
               "alignment");
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// Orders a wavefront's LDS accesses across lanes: the LDS performs one
+// wave's operations in issue order, so only the compiler must not move
+// loads and stores across this point.  (A wavefront-scope fence would also
+// wait for the block's outstanding global prefetch.)
+__device__ __forceinline__ void wave_sync() { asm volatile("" ::: "memory"); }
 
+template <bool kVec>
 __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
     const float* __restrict__ X, int64_t M, int64_t N, int64_t nW, int nCB,
     const Frag* __restrict__ F, float lo, float hi, float* __restrict__ logits,
@@ -211,32 +216,68 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
     const float b3 = F->b3;
 
     // input samples of block ib (first window wbase + 16 (ib - 1)): registers,
-    // then LDS.  Thread (row fr, first sample fs) stages samples fs + 23 i of
-    // row fr; consecutive threads take consecutive rows, so a load
-    // instruction reads a few whole 88-byte runs of channels.
-    float pre[kXPer];
+    // then LDS.  Vector form (M % 4 == 0): thread (piece q, sample s) loads
+    // channels c0-4+4q .. +3 of one sample as a float4, consecutive threads
+    // taking consecutive pieces, so a load instruction reads whole 96-byte
+    // runs of channels.  Scalar form: thread (row fr, first sample fs) stages
+    // samples fs + 21 i of row fr.
+    constexpr int kPer = kVec ? 4 * kXPer4 : kXPer;
+    float pre[kPer];
     const int fr = tid % kRows;
     const int fs = tid / kRows;
-    const int64_t fm = c0 - 3 + fr;
+    const int64_t fm = c0 - 4 + fr;
     const bool fok = fs < kXStep && fm >= 0 && fm < M;
     const float* frow = X + (fok ? fm : 0);
     const uint64_t M64 = static_cast<uint64_t>(M);
     const auto fetch = [&](int ib) {
-        const int64_t t0 = 32 * (wbase + kSB * static_cast<int64_t>(ib - 1)) + 148 + fs;
+        const int64_t tb = 32 * (wbase + kSB * static_cast<int64_t>(ib - 1)) + 148;
+        if (kVec) {
 #pragma unroll
-        for (int i = 0; i < kXPer; ++i) {
-            const int64_t t = t0 + kXStep * i;
-            const bool in = fok && fs + kXStep * i < kXN && t >= 0 && t < N;
-            const uint32_t tt = in ? static_cast<uint32_t>(t) : 0u;
-            const float v = frow[tt * M64];
-            pre[i] = in ? v : 0.0f;
+            for (int i = 0; i < kXPer4; ++i) {
+                const int e = tid + kCThreads * i;
+                const int q = e % kQ4;
+                const int s = e / kQ4;
+                const int64_t t = tb + s;
+                const int64_t m = c0 - 4 + 4 * q;  // M % 4 == 0: the piece is wholly in or out
+                const bool in = s < kXN && t >= 0 && t < N && m >= 0 && m < M;
+                const uint32_t tt = in ? static_cast<uint32_t>(t) : 0u;
+                const float4 v = *reinterpret_cast<const float4*>(X + tt * M64 + (in ? m : 0));
+                pre[4 * i + 0] = in ? v.x : 0.0f;
+                pre[4 * i + 1] = in ? v.y : 0.0f;
+                pre[4 * i + 2] = in ? v.z : 0.0f;
+                pre[4 * i + 3] = in ? v.w : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kXPer; ++i) {
+                const int64_t t = tb + fs + kXStep * i;
+                const bool in = fok && fs + kXStep * i < kXN && t >= 0 && t < N;
+                const uint32_t tt = in ? static_cast<uint32_t>(t) : 0u;
+                const float v = frow[tt * M64];
+                pre[i] = in ? v : 0.0f;
+            }
         }
     };
     const auto stash = [&](int buf) {
-        __bf16* xb = xbuf + buf * (kRows * kXStride) + fr * kXStride + fs;
+        __bf16* xb = xbuf + buf * (kRows * kXStride);
+        if (kVec) {
 #pragma unroll
-        for (int i = 0; i < kXPer; ++i)
-            if (fs < kXStep && fs + kXStep * i < kXN) xb[kXStep * i] = static_cast<__bf16>(pre[i]);
+            for (int i = 0; i < kXPer4; ++i) {
+                const int e = tid + kCThreads * i;
+                const int q = e % kQ4;
+                const int s = e / kQ4;
+                if (s < kXN) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        xb[(4 * q + k) * kXStride + s] = static_cast<__bf16>(pre[4 * i + k]);
+                }
+            }
+        } else {
+            xb += fr * kXStride + fs;
+#pragma unroll
+            for (int i = 0; i < kXPer; ++i)
+                if (fs < kXStep && fs + kXStep * i < kXN) xb[kXStep * i] = static_cast<__bf16>(pre[i]);
+        }
     };
     fetch(0);
     stash(0);
@@ -258,38 +299,45 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
         const bool more = ib < nb;
         if (more) fetch(ib + 1);
         const __bf16* xb = xbuf + (ib & 1) * (kRows * kXStride);
-#pragma unroll 1
-        for (int h = 0; h < 2; ++h) {
-            const int cc = 2 * wv + h;
-            if (c0 + cc >= M) break;
-            __bf16* h1 = h1all + cc * (kH1Pos * kH1S);
-            float* u = uall + cc * (kK * kUPos);
-            // carries: layer-1 positions 64..68 -> 0..4, layer-3 sums 16..23 -> 0..7
-            if (lane < 20) {
-                const int q = lane >> 2, part = lane & 3;
-                uint2* dst = reinterpret_cast<uint2*>(h1 + q * kH1S) + part * 2;
-                const uint2* src = reinterpret_cast<const uint2*>(h1 + (64 + q) * kH1S) + part * 2;
-                const uint2 zero = make_uint2(0u, 0u);
-                const uint2 v0 = ib > 0 ? src[0] : zero, v1 = ib > 0 ? src[1] : zero;
-                dst[0] = v0;
-                dst[1] = v1;
-            }
-            for (int q = lane; q < kK * 8; q += 64) {
-                const int k = q >> 3, p = q & 7;
-                u[k * kUPos + p] = ib > 0 ? u[k * kUPos + 16 + p] : 0.0f;
-            }
-            wave_sync();
-            // ---- layer 1: 4 tiles of 16 new positions
+        // the wave's two central channels, phase by phase (independent MFMA
+        // chains of both in flight together)
+        __bf16* h1c[2];
+        float* uc[2];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+        for (int h = 0; h < 2; ++h) {
+            h1c[h] = h1all + (2 * wv + h) * (kH1Pos * kH1S);
+            uc[h] = uall + (2 * wv + h) * (kK * kUPos);
+        }
+        // carries: layer-1 positions 64..68 -> 0..4, layer-3 sums 16..23 -> 0..7
+        if (lane < 40) {
+            const int h = lane / 20, q = (lane % 20) >> 2, part = lane & 3;
+            uint2* dst = reinterpret_cast<uint2*>(h1c[h] + q * kH1S) + part * 2;
+            const uint2* src = reinterpret_cast<const uint2*>(h1c[h] + (64 + q) * kH1S) + part * 2;
+            const uint2 zero = make_uint2(0u, 0u);
+            const uint2 v0 = ib > 0 ? src[0] : zero, v1 = ib > 0 ? src[1] : zero;
+            dst[0] = v0;
+            dst[1] = v1;
+        }
+        for (int q = lane; q < 2 * kK * 8; q += 64) {
+            const int h = q / (kK * 8), k = (q % (kK * 8)) >> 3, pp = q & 7;
+            float* u = uc[h];
+            u[k * kUPos + pp] = ib > 0 ? u[k * kUPos + 16 + pp] : 0.0f;
+        }
+        wave_sync();
+        // ---- layer 1: 4 tiles of 16 new positions for each central
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int cc = 2 * wv + h;
                 const int jj = 16 * t + n;
-                const bf16x8 bA = *reinterpret_cast<const bf16x8*>(xb + (cc + grp) * kXStride + 8 * jj);
+                const bf16x8 bA = *reinterpret_cast<const bf16x8*>(xb + (cc + 1 + grp) * kXStride + 8 * jj);
                 bf16x8 bB;
                 if (grp < 3) {
-                    bB = *reinterpret_cast<const bf16x8*>(xb + (cc + 4 + grp) * kXStride + 8 * jj);
+                    bB = *reinterpret_cast<const bf16x8*>(xb + (cc + 5 + grp) * kXStride + 8 * jj);
                 } else {
 #pragma unroll
-                    for (int ci = 0; ci < kCI; ++ci) bB[ci] = xb[(cc + ci) * kXStride + 8 * jj + 8];
+                    for (int ci = 0; ci < kCI; ++ci) bB[ci] = xb[(cc + 1 + ci) * kXStride + 8 * jj + 8];
                     bB[7] = static_cast<__bf16>(0.0f);
                 }
                 f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
@@ -303,25 +351,33 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
                     o0[r] = static_cast<__bf16>(fmaxf(acc0[r] + b1r[0][r], 0.0f));
                     o1[r] = static_cast<__bf16>(fmaxf(acc1[r] + b1r[1][r], 0.0f));
                 }
-                __bf16* dst = h1 + (5 + jj) * kH1S + 4 * grp;
+                __bf16* dst = h1c[h] + (5 + jj) * kH1S + 4 * grp;
                 *reinterpret_cast<bf16x4*>(dst) = o0;
                 *reinterpret_cast<bf16x4*>(dst + 16) = o1;
             }
-            wave_sync();
-            // ---- layer 2: positions w0 + 4 + n, h1 rows 4n + s
-            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+        }
+        wave_sync();
+        // ---- layer 2: positions w0 + 4 + n, layer-1 rows 4n + s
+        f32x4 acc[2][2];
 #pragma unroll
-            for (int s = 0; s < kK; ++s) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(h1 + (4 * n + s) * kH1S + 8 * grp);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][s], b, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][s], b, acc1, 0, 0, 0);
+        for (int h = 0; h < 2; ++h) acc[h][0] = acc[h][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < kK; ++s) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(h1c[h] + (4 * n + s) * kH1S + 8 * grp);
+                acc[h][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][s], b, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][s], b, acc[h][1], 0, 0, 0);
             }
-            // ---- layer 3: u_k at positions w0 + 4 + n (rows k = 4 grp + r < 9)
+        }
+        // ---- layer 3: u_k at positions w0 + 4 + n (rows k = 4 grp + r < 9)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
             bf16x8 hh, hl;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float v0 = fmaxf(acc0[r] + b2r[0][r], 0.0f);
-                const float v1 = fmaxf(acc1[r] + b2r[1][r], 0.0f);
+                const float v0 = fmaxf(acc[h][0][r] + b2r[0][r], 0.0f);
+                const float v1 = fmaxf(acc[h][1][r] + b2r[1][r], 0.0f);
                 hh[r] = static_cast<__bf16>(v0);
                 hh[4 + r] = static_cast<__bf16>(v1);
                 hl[r] = static_cast<__bf16>(v0 - static_cast<float>(hh[r]));
@@ -332,14 +388,17 @@ __global__ void __launch_bounds__(kCThreads) tcn_classify_kernel(
             uacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, hh, uacc, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                if (4 * grp + r < kK) u[(4 * grp + r) * kUPos + 8 + n] = uacc[r];
-            wave_sync();
-            if (lane < kSB) {
-                float z = b3;
+                if (4 * grp + r < kK) uc[h][(4 * grp + r) * kUPos + 8 + n] = uacc[r];
+        }
+        wave_sync();
+        // logits: lanes 0..15 the first central, 16..31 the second
+        if (lane < 2 * kSB) {
+            const int h = lane >> 4, wl = lane & 15;
+            const float* u = uc[h];
+            float z = b3;
 #pragma unroll
-                for (int k = 0; k < kK; ++k) z += u[k * kUPos + lane + k];
-                zt[(ib & 1) * kSB * kCB + lane * kCB + cc] = z;
-            }
+            for (int k = 0; k < kK; ++k) z += u[k * kUPos + wl + k];
+            zt[(ib & 1) * kSB * kCB + wl * kCB + 2 * wv + h] = z;
         }
         if (more) stash((ib + 1) & 1);
         __syncthreads();
@@ -649,11 +708,18 @@ int classify(const float* X, int64_t M, int64_t N, const void* frag, float lo_lo
     const int nCB = static_cast<int>(ceil_div(M, kCB));
     const int64_t nR = ceil_div(nW, static_cast<int64_t>(kRB) * kSB);
     MSAMD_REQUIRE(nR * nCB < (1LL << 31), "tcn classify: recording too long");
-    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&tcn_classify_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+    const bool vec = M % 4 == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0;
+    const void* fn = vec ? reinterpret_cast<const void*>(&tcn_classify_kernel<true>)
+                         : reinterpret_cast<const void*>(&tcn_classify_kernel<false>);
+    MSAMD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(kClassifyLds)));
-    tcn_classify_kernel<<<static_cast<unsigned>(nCB * nR), kCThreads, kClassifyLds, s>>>(
-        X, M, N, nW, nCB, static_cast<const Frag*>(frag), lo_logit, hi_logit, logits, classes);
+    const unsigned grid = static_cast<unsigned>(nCB * nR);
+    if (vec)
+        tcn_classify_kernel<true><<<grid, kCThreads, kClassifyLds, s>>>(
+            X, M, N, nW, nCB, static_cast<const Frag*>(frag), lo_logit, hi_logit, logits, classes);
+    else
+        tcn_classify_kernel<false><<<grid, kCThreads, kClassifyLds, s>>>(
+            X, M, N, nW, nCB, static_cast<const Frag*>(frag), lo_logit, hi_logit, logits, classes);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }

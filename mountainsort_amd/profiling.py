@@ -314,7 +314,7 @@ def main_counters_csv(argv):
     counter_collection.csv files (one row per dispatch and counter), each
     counter averaged over the kernel's dispatches, the dispatch time, and the
     derived MFMA busy fraction SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x
-    32 CUs per XCD) and texture-unit fractions per CU (x / (GRBM_GUI_ACTIVE /
+    128 SIMDs per XCD) and texture-unit fractions per CU (x / (GRBM_GUI_ACTIVE /
     8 XCDs x 256 CUs)) when those counters are present."""
     import collections
     regex, out, title, command = argv[:4]
@@ -340,7 +340,10 @@ def main_counters_csv(argv):
         g = mean.get("GRBM_GUI_ACTIVE")
         derived = []
         if g and "SQ_VALU_MFMA_BUSY_CYCLES" in mean:
-            derived.append(("MFMA busy (per CU)", mean["SQ_VALU_MFMA_BUSY_CYCLES"] / (g * 32)))
+            # GRBM_GUI_ACTIVE summed over the 8 XCDs; the MFMA busy cycles
+            # summed over every SIMD (1024 = 8 x 128)
+            derived.append(("MFMA busy (per SIMD)",
+                            mean["SQ_VALU_MFMA_BUSY_CYCLES"] / (g * 128)))
         for c in ("TD_TD_BUSY", "TD_TC_STALL", "TA_TA_BUSY", "TA_ADDR_STALLED_BY_TC_CYCLES",
                   "TCP_PENDING_STALL_CYCLES"):
             if c in mean and "GRBM_GUI_ACTIVE" in mean:
