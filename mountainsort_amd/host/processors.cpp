@@ -733,7 +733,7 @@ bool features_of(Ctx& c, const float* X, int64_t M, int64_t N, const std::vector
         pos[i] = slot[chan[i]]++;
         times[pos[i]] = center[i];
     }
-    std::vector<int64_t> num(G), no(G + 1, 0), co(G + 1, 0);
+    std::vector<int64_t> num(G), no(G + 1, 0);
     std::vector<int32_t> width(G), nc;
     for (int32_t g = 0; g < G; ++g) {
         num[g] = eo[g + 1] - eo[g];
@@ -744,37 +744,34 @@ bool features_of(Ctx& c, const float* X, int64_t M, int64_t N, const std::vector
                           " channels; the PCA kernels take at most 128 (reduce adjacency_radius)");
         nc.insert(nc.end(), nb[g].begin(), nb[g].end());
         no[g + 1] = no[g] + width[g];
-        co[g + 1] = co[g] + num[g] * width[g] * T;
     }
-    const int64_t wsb = msamd_pca_workspace_size(G, num.data(), width.data(), static_cast<int32_t>(T),
-                                                 static_cast<int32_t>(nf1), static_cast<int32_t>(nf2),
-                                                 maxs);
-    if (wsb < 0) return lib_ok(c, MSAMD_ERR_ARG, "pca_workspace_size");
+    // the clip gather fused into the stage-1 PCA: no clip array on the device
+    struct Plan {
+        msamd_pca_plan* p = nullptr;
+        ~Plan() { msamd_pca_plan_destroy(p); }
+    } plan;
+    if (!lib_ok(c, msamd_pca_plan_create(G, num.data(), width.data(), static_cast<int32_t>(T),
+                                         static_cast<int32_t>(nf1), static_cast<int32_t>(nf2),
+                                         maxs, nullptr, nullptr, 0, s, &plan.p),
+                "pca_plan_create"))
+        return false;
+    const int64_t wsb = msamd_pca_plan_workspace_bytes(plan.p);
     int64_t red = 0;
     for (int32_t g = 0; g < G; ++g) red += width[g] * nf1 * num[g];
-    DeviceBuffer dtimes, deo, dnc, dno, dco, clips, reduced, features, ws;
-    if (!dtimes.alloc(L * 8, c.err) || !deo.alloc((G + 1) * 8, c.err) ||
-        !dnc.alloc(static_cast<int64_t>(nc.size()) * 4, c.err) || !dno.alloc((G + 1) * 8, c.err) ||
-        !dco.alloc((G + 1) * 8, c.err) || !clips.alloc(co[G] * 4, c.err) ||
-        !reduced.alloc(red * 4, c.err) || !features.alloc(nf2 * L * 4, c.err) ||
-        !ws.alloc(wsb, c.err))
+    DeviceBuffer dtimes, dnc, dno, reduced, features, ws;
+    if (!dtimes.alloc(L * 8, c.err) || !dnc.alloc(static_cast<int64_t>(nc.size()) * 4, c.err) ||
+        !dno.alloc((G + 1) * 8, c.err) || !reduced.alloc(red * 4, c.err) ||
+        !features.alloc(nf2 * L * 4, c.err) || !ws.alloc(wsb, c.err))
         return false;
     if (!to_device(dtimes.get(), times.data(), L * 8, s, c.err) ||
-        !to_device(deo.get(), eo.data(), (G + 1) * 8, s, c.err) ||
         !to_device(dnc.get(), nc.data(), static_cast<int64_t>(nc.size()) * 4, s, c.err) ||
-        !to_device(dno.get(), no.data(), (G + 1) * 8, s, c.err) ||
-        !to_device(dco.get(), co.data(), (G + 1) * 8, s, c.err))
+        !to_device(dno.get(), no.data(), (G + 1) * 8, s, c.err))
         return false;
-    if (!lib_ok(c, msamd_extract_clips_batched(X, M, N, dtimes.as<int64_t>(), deo.as<int64_t>(), L,
-                                               dnc.as<int32_t>(), dno.as<int64_t>(), G,
-                                               static_cast<int32_t>(T), dco.as<int64_t>(),
-                                               clips.as<float>(), s),
-                "extract_clips_batched") ||
-        !lib_ok(c, msamd_pca_features(clips.as<float>(), G, num.data(), width.data(),
-                                      static_cast<int32_t>(T), static_cast<int32_t>(nf1),
-                                      static_cast<int32_t>(nf2), maxs, reduced.as<float>(),
-                                      features.as<float>(), ws.get(), ws.bytes(), s),
-                "pca_features"))
+    if (!lib_ok(c, msamd_pca_plan_attach(plan.p, ws.get(), ws.bytes(), s), "pca_plan_attach") ||
+        !lib_ok(c, msamd_pca_plan_run_timeseries(plan.p, X, M, N, dtimes.as<int64_t>(),
+                                                 dnc.as<int32_t>(), dno.as<int64_t>(),
+                                                 reduced.as<float>(), features.as<float>(), s),
+                "pca_plan_run_timeseries"))
         return false;
     std::vector<float> grouped(nf2 * L);
     if (!to_host(grouped.data(), features.get(), nf2 * L * 4, s, c.err)) return false;
