@@ -283,14 +283,22 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
     // 16-byte chunk c of row n1 holds pairs 4c..4c+3; with M % 8 == 0 a chunk
     // is wholly inside or wholly past the last channel
     constexpr int kCh = kWideP / 4;  // chunks per row
-    for (int q = tid; q < N1 * kCh; q += blockDim.x) {
-        const int n1 = q / kCh;
-        const int c = q % kCh;
-        const int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
-        uint4 v = {0u, 0u, 0u, 0u};
-        if (t >= 0 && t < p.N && ch0 + 8 * c < p.M)
-            v = *reinterpret_cast<const uint4*>(X + (t - p.xt0) * p.M + ch0 + 8 * c);
-        *reinterpret_cast<uint4*>(stg + n1 * kWideP + 4 * c) = v;
+    {
+        // thread q takes rows n1 = q / kCh + k * (blockDim / kCh) of chunk q % kCh:
+        // the row time and the source address advance by fixed steps
+        const int c = tid % kCh;
+        const int dn1 = static_cast<int>(blockDim.x) / kCh;
+        int n1 = tid / kCh;
+        int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
+        const int64_t dt = static_cast<int64_t>(dn1) * p.N2;
+        const bool ch_ok = ch0 + 8 * c < p.M;
+        const int16_t* src = X + (t - p.xt0) * p.M + ch0 + 8 * c;
+        const int64_t dsrc = dt * p.M;
+        for (; n1 < N1; n1 += dn1, t += dt, src += dsrc) {
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (ch_ok && t >= 0 && t < p.N) v = *reinterpret_cast<const uint4*>(src);
+            *reinterpret_cast<uint4*>(stg + n1 * kWideP + 4 * c) = v;
+        }
     }
     __syncthreads();
     const int pl = tid % kWideP;
@@ -319,14 +327,17 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
             y[r] = (r == 0 || j == 0) ? x : cmul(x, tw1[r * j]);
         }
         fft::Dft<R2, -1>::run(y);
-        cf* Su = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG;
+        // spectra element (k1 = j + r R1, n2) of this pair; k1 advances by R1
+        cf* dst = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG +
+                  (static_cast<int64_t>(j) * p.N2 + n2) * p.CG + pb * kWideP + pl;
+        const int64_t dstep = static_cast<int64_t>(R1) * p.N2 * p.CG;
         cf wj = {1.f, 0.f};
         if (kTwLds) {
             const int e = n2 * j;
             wj = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
         }
 #pragma unroll
-        for (int r = 0; r < R2; ++r) {
+        for (int r = 0; r < R2; ++r, dst += dstep) {
             const int k1 = j + r * R1;
             cf w4;
             if (kTwLds) {
@@ -335,7 +346,7 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
                 const int e = n2 * k1;
                 w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
             }
-            Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kWideP + pl] = cmul(y[r], w4);
+            *dst = cmul(y[r], w4);
         }
     }
 }
@@ -365,11 +376,15 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
     u += unit0;
     cf* Srow = S + (u - unit0) * static_cast<int64_t>(p.n) * p.CG +
                static_cast<int64_t>(k1) * N2 * p.CG + pb * kPB + pl;
+    // element n2 = j + r*Q2 of the row: Srow + j*CG, then steps of Q2*CG
+    cf* const Sj = Srow + static_cast<int64_t>(j) * p.CG;
+    const int64_t rstep = static_cast<int64_t>(Q2) * p.CG;
     // forward pass 1: radix Q1 over n2 = j + r*Q2 (Q2 tasks)
     if (j < Q2) {
         cf v[Q1];
+        const cf* src = Sj;
 #pragma unroll
-        for (int r = 0; r < Q1; ++r) v[r] = Srow[static_cast<int64_t>(j + r * Q2) * p.CG];
+        for (int r = 0; r < Q1; ++r, src += rstep) v[r] = *src;
         fft::Dft<Q1, -1>::run(v);
 #pragma unroll
         for (int r = 0; r < Q1; ++r) buf[(j * QA + r) * kPB + pl] = v[r];
@@ -409,8 +424,9 @@ __global__ void __launch_bounds__(kPB * (Q1 > Q2 ? Q1 : Q2)) bp_stage_b(
             v[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw2[r * j]);
         }
         fft::Dft<Q1, 1>::run(v);
+        cf* dst = Sj;
 #pragma unroll
-        for (int r = 0; r < Q1; ++r) Srow[static_cast<int64_t>(j + r * Q2) * p.CG] = v[r];
+        for (int r = 0; r < Q1; ++r, dst += rstep) *dst = v[r];
     }
 }
 
@@ -456,8 +472,10 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
             const int e = n2 * j;
             wj = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
         }
+        const cf* src = Su + (static_cast<int64_t>(j) * p.N2 + n2) * p.CG + pb * kPB + pl;
+        const int64_t sstep = static_cast<int64_t>(R1) * p.N2 * p.CG;  // k1 advances by R1
 #pragma unroll
-        for (int r = 0; r < R2; ++r) {
+        for (int r = 0; r < R2; ++r, src += sstep) {
             const int k1 = j + r * R1;
             cf w4;
             if (kTwLds) {
@@ -466,7 +484,7 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
                 const int e = n2 * k1;
                 w4 = cmul(twHi[e / kTwLo], twLo[e % kTwLo]);
             }
-            v[r] = cconjmul(Su[(static_cast<int64_t>(k1) * p.N2 + n2) * p.CG + pb * kPB + pl], w4);
+            v[r] = cconjmul(*src, w4);
         }
         fft::Dft<R2, 1>::run(v);
 #pragma unroll
@@ -481,7 +499,38 @@ __global__ void __launch_bounds__(kPB * (R1 > R2 ? R1 : R2)) bp_stage_c(
             y[r] = (r == 0 || j == 0) ? x : cconjmul(x, tw1[r * j]);
         }
         fft::Dft<R1, 1>::run(y);
-        if (pair < p.P) {
+        if (kSub1 && pair < p.P && (p.M % 2) == 0) {
+            // output row to = w*cs + tl, tl = (j + r R2) N2 + n2 - ov (|tl| < n <
+            // 2^31): the row is kept when 0 <= tl < min(cs, N_out - w*cs), one
+            // unsigned 32-bit compare per r, and the store address advances by a
+            // fixed step
+            const int step = R2 * p.N2;
+            const int tl0 = j * p.N2 + n2 - static_cast<int>(p.ov);
+            const int64_t rest = p.N_out - w * p.cs;
+            const uint32_t lim = static_cast<uint32_t>(
+                rest <= 0 ? 0 : (rest < p.cs ? rest : p.cs));
+            const int64_t off0 = (w * p.cs + tl0 - p.yt0) * p.M + ch0;
+            const int64_t ostep = static_cast<int64_t>(step) * p.M;
+            if (kOutI16) {
+                uint32_t* dst = reinterpret_cast<uint32_t*>(static_cast<int16_t*>(Yv) + off0);
+#pragma unroll
+                for (int r = 0; r < R1; ++r, dst += ostep / 2) {
+                    if (static_cast<uint32_t>(tl0 + r * step) >= lim) continue;
+                    const float s0 = static_cast<float>(static_cast<double>(y[r].x) * p.qscale);
+                    const float s1 = static_cast<float>(static_cast<double>(y[r].y) * p.qscale);
+                    const uint32_t lo = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s0)));
+                    const uint32_t hi = static_cast<uint16_t>(static_cast<int16_t>(static_cast<int>(s1)));
+                    *dst = lo | (hi << 16);
+                }
+            } else {
+                float2* dst = reinterpret_cast<float2*>(static_cast<float*>(Yv) + off0);
+#pragma unroll
+                for (int r = 0; r < R1; ++r, dst += ostep / 2) {
+                    if (static_cast<uint32_t>(tl0 + r * step) >= lim) continue;
+                    *dst = make_float2(y[r].x, y[r].y);
+                }
+            }
+        } else if (pair < p.P) {
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 const int64_t n1 = j + r * R2;
