@@ -452,14 +452,20 @@ __device__ __forceinline__ u64 w_tagged(double x, unsigned tag) {
     return ((b + 8ull) & ~15ull) | tag;
 }
 
+// kLocal: every workgroup of the exchange runs on one XCD, so a word need
+// only reach that XCD's L2 (a workgroup-scope store writes through the CU's
+// L1 into it); the polls are device-scope loads, which miss the L1 and find
+// the line in that L2.  Otherwise the store is device scope too.
+template <bool kLocal>
 __device__ __forceinline__ void w_publish(u64* slot, double x, unsigned tag) {
-    __hip_atomic_store(slot, w_tagged(x, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot, w_tagged(x, tag), __ATOMIC_RELAXED,
+                       kLocal ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Lane q's columns q + L*j of a published vector (pub points at column q of
 // the slot); words beyond M are never written and read as 0.  Returns false
 // after a timeout or when another wavefront has raised the error flag.
-template <int L, int J>
+template <int L, int J, bool kLocal>
 __device__ __forceinline__ bool w_fetch(const u64* pub, int q, int M, unsigned tag, double* out,
                                         int* err, unsigned limit) {
     unsigned spins = 0;
@@ -487,20 +493,27 @@ __device__ __forceinline__ bool w_fetch(const u64* pub, int q, int M, unsigned t
 struct WBufs {
     u64* wbuf;  // [2][Mp] published rows of G v (tagged words)
     u64* ybuf;  // [2][Mp] published rows of G v0 (deflation exchange)
-    int* err;
+    int* err;     // 1: a wait timed out; 2: the XCD-local form found its workgroups on several XCDs
+    u64* place;   // XCD-local form: working workgroups counted per XCD, 8 bits each
     unsigned spin_limit;
+    int stride;   // workgroup b works as row block b / stride when b % stride == 0
 };
 
-template <int L, int J>
+template <int L, int J, bool kLocal>
 __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
     const double* __restrict__ XXt, WPlan p, const double* __restrict__ v0g,
     double* __restrict__ comp, double* __restrict__ sig, WBufs bufs) {
     extern __shared__ double sh[];
+    // XCD-local form: workgroups are dispatched round-robin over the XCDs, so
+    // the workgroups b = 0, stride, 2 stride, ... all run on the first one;
+    // the others end here
+    if (kLocal && blockIdx.x % bufs.stride != 0) return;
+    const int blk = kLocal ? static_cast<int>(blockIdx.x) / bufs.stride : static_cast<int>(blockIdx.x);
     const int M = p.M, S = p.S, Mp = p.Mp;
     const int tid = threadIdx.x;
     const int rg = tid / L;   // row group = local row
     const int q = tid % L;
-    const int r0 = blockIdx.x * p.RW;
+    const int r0 = blk * p.RW;
     const int nrows = (M - r0) < p.RW ? (M - r0) : p.RW;
     const int row = r0 + rg;
     const bool has_row = rg < nrows;
@@ -514,6 +527,44 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
         Gl[e] = (r < nrows && c < M) ? XXt[static_cast<int64_t>(r0 + r) * M + c] : 0.0;
     }
     for (int i = tid; i < Mp; i += blockDim.x) v0l[i] = i < M ? v0g[i] : 0.0;
+    if (kLocal) {
+        // The exchange below relies on every working workgroup running on one
+        // XCD.  Each counts itself in its XCD's 8-bit field of one word and
+        // waits (bounded) until all NP have counted; if they are not all on
+        // one XCD every workgroup sees the same word and stops with error 2
+        // (the host then runs the device-scope form).
+        __shared__ int placed;
+        if (tid == 0) {
+            const unsigned xcc = static_cast<unsigned>(__builtin_amdgcn_s_getreg((15 << 11) | 20)) & 7u;
+            __hip_atomic_fetch_add(bufs.place, 1ull << (8 * xcc), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int ok = 0;
+            unsigned spins = 0;
+            for (;;) {
+                const u64 w = __hip_atomic_load(bufs.place, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned n = 0;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) n += static_cast<unsigned>((w >> (8 * x)) & 0xffull);
+                if (n >= static_cast<unsigned>(p.NP)) {
+                    ok = w == (static_cast<u64>(p.NP) << (8 * xcc)) ? 1 : 2;
+                    break;
+                }
+                ++spins;
+                if (((spins & 63u) == 0 || spins > bufs.spin_limit) &&
+                    (spins > bufs.spin_limit ||
+                     __hip_atomic_load(bufs.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                    ok = 3;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (ok == 2) __hip_atomic_store(bufs.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ok == 3) __hip_atomic_store(bufs.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            placed = ok;
+        }
+        __syncthreads();
+        if (placed != 1) return;
+    }
     __syncthreads();
     double* Grow = Gl + static_cast<int64_t>(has_row ? rg : 0) * S + q;
     u64* wpub = bufs.wbuf + q;
@@ -539,8 +590,8 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
 #pragma unroll
                 for (int j = 0; j < J; ++j) acc = fma(g[j], v[j], acc);
                 acc = group_sum<L>(acc);
-                if (q == 0) w_publish(bufs.wbuf + par * Mp + row, acc, tag);
-                w_fetch<L, J>(wpub + par * Mp, q, M, tag, v, bufs.err, bufs.spin_limit);
+                if (q == 0) w_publish<kLocal>(bufs.wbuf + par * Mp + row, acc, tag);
+                w_fetch<L, J, kLocal>(wpub + par * Mp, q, M, tag, v, bufs.err, bufs.spin_limit);
             }
             double ss = 0.0;
 #pragma unroll
@@ -561,7 +612,7 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
                 if (n >= r0 && n < r0 + nrows) comp[static_cast<int64_t>(n) * M + k] = v[j];
             }
         }
-        if (blockIdx.x == 0 && tid == 0) sig[k] = sigma;
+        if (blk == 0 && tid == 0) sig[k] = sigma;
         if (k == M - 1) break;
 
         // deflation exchange: u = G v and y = G v0
@@ -582,11 +633,11 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
             au = group_sum<L>(au);
             ay = group_sum<L>(ay);
             if (q == 0) {
-                w_publish(bufs.wbuf + par * Mp + row, au, tag);
-                w_publish(bufs.ybuf + par * Mp + row, ay, tag);
+                w_publish<kLocal>(bufs.wbuf + par * Mp + row, au, tag);
+                w_publish<kLocal>(bufs.ybuf + par * Mp + row, ay, tag);
             }
-            w_fetch<L, J>(wpub + par * Mp, q, M, tag, u, bufs.err, bufs.spin_limit);
-            w_fetch<L, J>(ypub + par * Mp, q, M, tag, y, bufs.err, bufs.spin_limit);
+            w_fetch<L, J, kLocal>(wpub + par * Mp, q, M, tag, u, bufs.err, bufs.spin_limit);
+            w_fetch<L, J, kLocal>(ypub + par * Mp, q, M, tag, y, bufs.err, bufs.spin_limit);
         }
         double d_uv0 = 0.0, d_vv0 = 0.0, alpha = 0.0;
 #pragma unroll
@@ -638,11 +689,32 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
 
 template <int L, int J>
 int launch_power(dim3 grid, dim3 block, size_t lds, hipStream_t st, const double* XXt,
-                 const WPlan& wp, const double* v0, double* comp, double* sig, const WBufs& wb) {
-    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&whitening_power_kernel<L, J>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)));
-    whitening_power_kernel<L, J><<<grid, block, lds, st>>>(XXt, wp, v0, comp, sig, wb);
+                 const WPlan& wp, const double* v0, double* comp, double* sig, WBufs wb,
+                 bool want_local) {
+    const auto kloc = &whitening_power_kernel<L, J, true>;
+    const auto kdev = &whitening_power_kernel<L, J, false>;
+    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kloc),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    MSAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kdev),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    // the XCD-local form when every workgroup fits on one XCD at once
+    wb.stride = 1;
+    if (want_local) {
+        int dev = 0, nx = 1, cus = 0, per_cu = 0;  // (8 bits per XCD in the placement word)
+        MSAMD_HIP(hipGetDevice(&dev));
+        MSAMD_HIP(hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, dev));
+        MSAMD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        MSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(kloc), static_cast<int>(block.x), lds));
+        if (nx > 1 && nx <= 8 && grid.x < 256 &&
+            static_cast<int64_t>(per_cu) * (cus / nx) >= static_cast<int64_t>(grid.x))
+            wb.stride = nx;
+    }
+    if (wb.stride > 1)
+        kloc<<<dim3(grid.x * static_cast<unsigned>(wb.stride)), block, lds, st>>>(XXt, wp, v0, comp,
+                                                                                 sig, wb);
+    else
+        kdev<<<grid, block, lds, st>>>(XXt, wp, v0, comp, sig, wb);
     return MSAMD_OK;
 }
 
@@ -1103,6 +1175,7 @@ extern "C" int64_t msamd_whitening_matrix_workspace_size(int64_t M) {
     c.take<double>(M);           // sigma
     c.take<double>(M);           // v0
     c.take<u64>(4 * wp.Mp);      // published rows of G v and of G v0, two slots each
+    c.take<u64>(1);              // placement word
     c.take<int>(1);              // error flag
     return c.bytes();
 }
@@ -1118,6 +1191,7 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     double* v0 = c.take<double>(M);
     u64* wbuf = c.take<u64>(4 * wp.Mp);
     u64* ybuf = wbuf + 2 * wp.Mp;
+    u64* place = c.take<u64>(1);
     int* err = c.take<int>(1);
     if (!workspace || workspace_bytes < c.bytes()) {
         set_error("whitening_matrix: workspace too small (%lld < %lld)",
@@ -1153,28 +1227,41 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     wb.ybuf = ybuf;
     wb.err = err;
     wb.spin_limit = w_spin_limit();
-    // published words start at 0.  A word is first written at step 1 or 2
-    // (G v slots) or 11 (G v0 slot), whose tags are not 0; later a waiting
-    // lane can only find the tag of an earlier use (step e-2, or e-10 for the
-    // G v0 slot), never equal to e mod 16.
-    MSAMD_HIP(hipMemsetAsync(wbuf, 0, 4 * sizeof(u64) * wp.Mp, st));
-    MSAMD_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    wb.place = place;
+    // The exchange runs XCD-local (all workgroups on one XCD, words meeting in
+    // its L2) when they fit there; MSAMD_W_XCD_LOCAL=0 keeps the device-scope
+    // form.  If the local form finds its workgroups spread over several XCDs
+    // it stops before any step and the device-scope form runs instead.
+    const char* xl = std::getenv("MSAMD_W_XCD_LOCAL");
+    const bool want_local = !(xl && atoi(xl) == 0);
     const size_t lds = w_lds_bytes(wp);
     const dim3 grid(static_cast<unsigned>(wp.NP)), block(static_cast<unsigned>(wp.T));
-    int rc = MSAMD_OK;
-    switch (wp.L * 100 + wp.J) {
-    case 1604: rc = launch_power<16, 4>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    case 1608: rc = launch_power<16, 8>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    case 1616: rc = launch_power<16, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    case 1624: rc = launch_power<16, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    case 3224: rc = launch_power<32, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    default: rc = launch_power<64, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb); break;
-    }
-    if (rc != MSAMD_OK) return rc;
-    MSAMD_LAUNCH_CHECK();
     int herr = 0;
-    MSAMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
-    MSAMD_HIP(hipStreamSynchronize(st));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const bool local = want_local && attempt == 0;
+        // published words start at 0.  A word is first written at step 1 or 2
+        // (G v slots) or 11 (G v0 slot), whose tags are not 0; later a waiting
+        // lane can only find the tag of an earlier use (step e-2, or e-10 for
+        // the G v0 slot), never equal to e mod 16.
+        MSAMD_HIP(hipMemsetAsync(wbuf, 0, 4 * sizeof(u64) * wp.Mp, st));
+        MSAMD_HIP(hipMemsetAsync(place, 0, sizeof(u64), st));
+        MSAMD_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+        int rc = MSAMD_OK;
+        switch (wp.L * 100 + wp.J) {
+        case 1604: rc = launch_power<16, 4>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        case 1608: rc = launch_power<16, 8>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        case 1616: rc = launch_power<16, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        case 1624: rc = launch_power<16, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        case 3224: rc = launch_power<32, 24>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        default: rc = launch_power<64, 16>(grid, block, lds, st, XXt, wp, v0, comp, sig, wb, local); break;
+        }
+        if (rc != MSAMD_OK) return rc;
+        MSAMD_LAUNCH_CHECK();
+        herr = 0;
+        MSAMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+        MSAMD_HIP(hipStreamSynchronize(st));
+        if (herr != 2 || !local) break;
+    }
     if (herr) {
         set_error("whitening_matrix: the exchange between the %d workgroups of the power method "
                   "timed out (other work on the device kept some of them from running); "

@@ -186,6 +186,24 @@ def test_whitening_matrix_spiked_spectrum(gpu, M):
     assert rel < 1e-7
 
 
+@pytest.mark.parametrize("M", [16, 64, 128, 384, 500])
+def test_whitening_matrix_xcd_local_exchange_is_bit_identical(gpu, M, monkeypatch):
+    """The power method's exchange confined to one XCD (the default where its
+    workgroups fit there; M = 500 does not and runs the device-scope form
+    either way) gives the same W, bit for bit, as the device-scope exchange
+    (MSAMD_W_XCD_LOCAL=0): only where the published words meet changes."""
+    from mountainsort_amd import frontend as fe
+    say(f"whitening matrix XCD-local vs device-scope exchange M={M}")
+    rng = np.random.default_rng(700 + M)
+    A = rng.normal(size=(M, 3 * M))
+    C = torch.from_numpy(A @ A.T / (3 * M) + 0.5 * np.eye(M)).to(gpu)
+    W_local = [fe.whitening_matrix(C).cpu().numpy() for _ in range(2)]
+    monkeypatch.setenv("MSAMD_W_XCD_LOCAL", "0")
+    W_dev = fe.whitening_matrix(C).cpu().numpy()
+    assert np.array_equal(W_local[0], W_local[1])
+    assert np.array_equal(W_local[0], W_dev)
+
+
 @pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000),
                                  (128, 3000), (192, 100_000), (200, 4001), (256, 2500),
                                  (320, 3333), (130, 2000)])
