@@ -860,13 +860,35 @@ __global__ void __launch_bounds__(256) apply_scale_kernel(const double* __restri
 __global__ void __launch_bounds__(256) apply_xmax_kernel(const float* __restrict__ X, int64_t M,
                                                          int64_t N, const uint32_t* __restrict__ rk,
                                                          uint32_t* __restrict__ sc) {
+    // a workgroup takes groups of 32 rows (one keep word): 32*M contiguous
+    // floats, read as float4 when M % 4 == 0; a row's samples count only when
+    // its keep bit is set (all set: no per-element test)
     float m = 0.0f;
-    const int64_t total = M * N;
-    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < total;
-         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t t = q / M;
-        if (rk && !((rk[t >> 5] >> (t & 31)) & 1u)) continue;
-        m = fmaxf(m, fabsf(X[q]));
+    const int64_t ngroups = (N + 31) / 32;
+    const bool v4 = (M % 4) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+    for (int64_t gq = blockIdx.x; gq < ngroups; gq += gridDim.x) {
+        const int64_t t0 = 32 * gq;
+        const int rows = N - t0 < 32 ? static_cast<int>(N - t0) : 32;
+        const uint32_t kw = rk ? rk[gq] : ~0u;
+        if (rows == 32 && kw == 0u) continue;
+        const bool all = rows == 32 && kw == ~0u;
+        const float* base = X + t0 * M;
+        const int n = static_cast<int>(rows * M);  // <= 32 * 1024
+        const int Mi = static_cast<int>(M);
+        if (v4) {
+            const float4* b4 = reinterpret_cast<const float4*>(base);
+            const int M4 = Mi / 4;
+            for (int f = threadIdx.x; f < n / 4; f += blockDim.x) {
+                if (!all && !((kw >> (f / M4)) & 1u)) continue;
+                const float4 v = b4[f];
+                m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+            }
+        } else {
+            for (int f = threadIdx.x; f < n; f += blockDim.x) {
+                if (!all && !((kw >> (f / Mi)) & 1u)) continue;
+                m = fmaxf(m, fabsf(base[f]));
+            }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(sc + 1, __float_as_uint(m));
@@ -1309,8 +1331,8 @@ int apply_impl(const float* X, int64_t M, int64_t N, const double* W, const doub
     apply_scale_kernel<<<1, 256, 0, st>>>(W, M, C, N, sc);
     MSAMD_LAUNCH_CHECK();
     if (!C && N > 0) {
-        int64_t blocks = ceil_div(M * N, 256 * 16);
-        if (blocks > 8192) blocks = 8192;
+        int64_t blocks = ceil_div(N, 32);
+        if (blocks > 4096) blocks = 4096;
         apply_xmax_kernel<<<static_cast<unsigned>(blocks), 256, 0, st>>>(X, M, N, rowkeep, sc);
         MSAMD_LAUNCH_CHECK();
     }
