@@ -11,6 +11,7 @@
 // term is dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The
 // covariance sums FP32 block partials in FP64; the whitening matrix is FP64.
 #include <cstdint>
+#include <cstring>
 #include <mutex>
 
 #include "common.h"
@@ -497,6 +498,7 @@ struct WBufs {
     u64* place;   // XCD-local form: working workgroups counted per XCD, 8 bits each
     unsigned spin_limit;
     int stride;   // workgroup b works as row block b / stride when b % stride == 0
+    int spread;   // tests: the XCD-local form reports its workgroups as spread (error 2)
 };
 
 template <int L, int J, bool kLocal>
@@ -546,7 +548,7 @@ __global__ void __launch_bounds__(kWMaxThreads) whitening_power_kernel(
 #pragma unroll
                 for (int x = 0; x < 8; ++x) n += static_cast<unsigned>((w >> (8 * x)) & 0xffull);
                 if (n >= static_cast<unsigned>(p.NP)) {
-                    ok = w == (static_cast<u64>(p.NP) << (8 * xcc)) ? 1 : 2;
+                    ok = (w == (static_cast<u64>(p.NP) << (8 * xcc)) && !bufs.spread) ? 1 : 2;
                     break;
                 }
                 ++spins;
@@ -1256,6 +1258,9 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
     // it stops before any step and the device-scope form runs instead.
     const char* xl = std::getenv("MSAMD_W_XCD_LOCAL");
     const bool want_local = !(xl && atoi(xl) == 0);
+    // MSAMD_W_XCD_LOCAL=spread (tests): the local form finds its workgroups
+    // spread over XCDs, so the fallback to the device-scope form runs
+    wb.spread = (xl && std::strcmp(xl, "spread") == 0) ? 1 : 0;
     const size_t lds = w_lds_bytes(wp);
     const dim3 grid(static_cast<unsigned>(wp.NP)), block(static_cast<unsigned>(wp.T));
     int herr = 0;

@@ -191,7 +191,9 @@ def test_whitening_matrix_xcd_local_exchange_is_bit_identical(gpu, M, monkeypatc
     """The power method's exchange confined to one XCD (the default where its
     workgroups fit there; M = 500 does not and runs the device-scope form
     either way) gives the same W, bit for bit, as the device-scope exchange
-    (MSAMD_W_XCD_LOCAL=0): only where the published words meet changes."""
+    (MSAMD_W_XCD_LOCAL=0): only where the published words meet changes.  The
+    fallback after a failed placement check (forced from the host) gives it
+    too."""
     from mountainsort_amd import frontend as fe
     say(f"whitening matrix XCD-local vs device-scope exchange M={M}")
     rng = np.random.default_rng(700 + M)
@@ -200,8 +202,13 @@ def test_whitening_matrix_xcd_local_exchange_is_bit_identical(gpu, M, monkeypatc
     W_local = [fe.whitening_matrix(C).cpu().numpy() for _ in range(2)]
     monkeypatch.setenv("MSAMD_W_XCD_LOCAL", "0")
     W_dev = fe.whitening_matrix(C).cpu().numpy()
+    # the local form told that its workgroups are spread over XCDs stops
+    # before any step and the call reruns in the device-scope form
+    monkeypatch.setenv("MSAMD_W_XCD_LOCAL", "spread")
+    W_fallback = fe.whitening_matrix(C).cpu().numpy()
     assert np.array_equal(W_local[0], W_local[1])
     assert np.array_equal(W_local[0], W_dev)
+    assert np.array_equal(W_fallback, W_dev)
 
 
 @pytest.mark.parametrize("M,N", [(4, 60_000), (16, 30_000), (64, 20_000), (384, 6000), (7, 5000),
