@@ -96,6 +96,45 @@ def summarize(path, title, command=None, per=1, note=None):
 
 
 
+def call_spans(trace_csv, pattern, min_kernels=100):
+    """Spans of the library calls made of many kernels whose names contain
+    `pattern` (e.g. the filter's three stages over its window batches, on two
+    streams): from a `rocprofv3 --kernel-trace` CSV, each maximal run of such
+    kernels with no other kernel between them, first start to last end.
+    Returns [(span_ms, kernel_count, busy_ms)], busy = summed kernel time."""
+    rows = list(csv.DictReader(open(trace_csv)))
+    ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                for r in rows)
+    out, cur = [], None
+    for s, e, n in ks + [(0, 0, "")]:
+        if pattern and pattern in n:
+            if cur is None:
+                cur = [s, e, 0, 0]
+            cur[1] = max(cur[1], e)
+            cur[2] += 1
+            cur[3] += e - s
+        else:
+            if cur is not None and cur[2] >= min_kernels:
+                out.append(((cur[1] - cur[0]) / 1e6, cur[2], cur[3] / 1e6))
+            cur = None
+    return out
+
+
+def main_spans(argv):
+    """profiling spans <trace.csv> <pattern> <out.md> <title> <command>"""
+    trace, pattern, out, title, command = argv[:5]
+    spans = call_spans(trace, pattern)
+    lines = [f"# {title}", "", "Command:", "", "```", command, "```", "",
+             f"Calls made of kernels whose names contain `{pattern}`, from the kernel trace "
+             "(first start to last end of each run of them):", "",
+             "| call | kernels | span ms | summed kernel time ms |", "|---:|---:|---:|---:|"]
+    for i, (sp, n, busy) in enumerate(spans):
+        lines.append(f"| {i} | {n} | {sp:.3f} | {busy:.3f} |")
+    lines.append("")
+    open(out, "w").write("\n".join(lines) + "\n")
+    sys.stdout.write("\n".join(lines) + "\n")
+
+
 # --------------------------------------------------------------------------
 # PMC traffic: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (rocpd databases)
 # --------------------------------------------------------------------------
@@ -359,6 +398,8 @@ def main_counters_csv(argv):
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "counters":
         main_counters(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "spans":
+        main_spans(sys.argv[2:])
     elif len(sys.argv) > 1 and sys.argv[1] == "counters-csv":
         main_counters_csv(sys.argv[2:])
     elif len(sys.argv) > 1 and sys.argv[1] == "pmc-all-csv":
