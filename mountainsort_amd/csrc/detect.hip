@@ -209,6 +209,37 @@ __device__ __forceinline__ int stencil_float(const float* xp, int64_t t, float x
     return -1;
 }
 
+// The same test with the neighbours read from the wavefront's LDS window:
+// lane column `win` (stride 64 floats per row) holds samples
+// tb - kLdsD .. tb + kB + kLdsD - 1 of its channel (row kLdsD + i is sample
+// tb + i, zero outside the recording, as load_x), so D <= kLdsD neighbours of
+// batch index i need no load through the texture path.
+constexpr int kLdsD = 10;
+constexpr int kWinRows = kLdsD + kB + kLdsD;
+template <int kSign>
+__device__ __forceinline__ int stencil_win(const float* win, int i, int64_t t, float xt, float mabs,
+                                           int D, int64_t wa, int64_t wb) {
+    const int nbk = static_cast<int>(t - wa < D ? t - wa : D);
+    const int nfw = static_cast<int>(wb - 1 - t < D ? wb - 1 - t : D);
+    const float kt = kSign < 0 ? -xt : xt;
+    float bk = -INFINITY, fk = -INFINITY;
+#pragma unroll
+    for (int d = 1; d <= kLdsD; ++d) {
+        if (d <= nbk) {
+            const float x = win[(kLdsD + i - d) * 64];
+            bk = fmaxf(bk, kSign < 0 ? -x : x);
+        }
+        if (d <= nfw) {
+            const float x = win[(kLdsD + i + d) * 64];
+            fk = fmaxf(fk, kSign < 0 ? -x : x);
+        }
+    }
+    const float tau = 0x1p-40f * (fabsf(xt) + mabs);
+    if (bk >= kt || fk - kt > tau) return 0;
+    if (fk <= kt && kt - bk > tau) return 1;
+    return -1;
+}
+
 // kGated: the learned detector's gated scan (channels 0..M-1): a crossing is
 // kept only when the network proposed its 32-sample window (class 1 or 2 in
 // cls[w * M + channel]); the stencil itself still reads every neighbour, so
@@ -220,6 +251,10 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
     const double* __restrict__ mean_in, const double* __restrict__ thr_in,
     int32_t* __restrict__ counts, uint16_t* __restrict__ slots,
     const uint8_t* __restrict__ cls = nullptr) {
+    // per wavefront: the window of each lane's channel around the batch (a
+    // column of stride 64 floats), written only by lanes with a crossing
+    __shared__ float win_all[4][kWinRows * 64];
+    float* const win = win_all[threadIdx.x >> 6] + (threadIdx.x & 63);
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= p.nseg * p.Mc) return;
     const int j = static_cast<int>(g % p.Mc);
@@ -244,12 +279,17 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
         const int64_t T1 = core0 + r1;
         // one-sided sign and a window within kB: the float test first
         const bool fast = kSign != 0 && p.D <= kB;
+        // D <= kLdsD: a crossing's neighbours come from the wave's LDS window
+        const bool winpath = fast && p.D <= kLdsD;
         // |mean| rounded up to float
         const float mabs = static_cast<float>(fabs(mean)) * (1.0f + 0x1p-20f);
         float cur[kB], nxt[kB];
 #pragma unroll
         for (int i = 0; i < kB; ++i) cur[i] = load_x(xp, T0 + i, p.M, p.N_file);
-        float prev = load_x(xp, T0 - 1, p.M, p.N_file);  // the sample before the batch
+        float prv[kLdsD];  // the kLdsD samples before the batch
+#pragma unroll
+        for (int k = 0; k < kLdsD; ++k) prv[k] = load_x(xp, T0 - kLdsD + k, p.M, p.N_file);
+        float prev = prv[kLdsD - 1];  // the sample before the batch
         for (int64_t tb = T0; tb < T1; tb += kB) {
 #pragma unroll
             for (int i = 0; i < kB; ++i) nxt[i] = load_x(xp, tb + kB + i, p.M, p.N_file);
@@ -297,19 +337,32 @@ __global__ void __launch_bounds__(256) detect_scan_kernel(
                 }
                 over &= gate;
             }
+            if (winpath && over) {
+                // this lane's column of the window: rows tb - kLdsD .. tb + kB + kLdsD - 1
+#pragma unroll
+                for (int k = 0; k < kLdsD; ++k) win[k * 64] = prv[k];
+#pragma unroll
+                for (int k = 0; k < kB; ++k) win[(kLdsD + k) * 64] = cur[k];
+#pragma unroll
+                for (int k = 0; k < kLdsD; ++k) win[(kLdsD + kB + k) * 64] = nxt[k];
+            }
             // in time order; kept events are > D apart, so cnt < cap
             while (over) {
                 const int i = __builtin_ctz(over);
                 over &= over - 1;
                 const int64_t t = tb + i;
-                const float xt = load_x(xp, t, p.M, p.N_file);
-                int keep = fast ? stencil_float<kSign>(xp, t, xt, mabs, p.D, wa, wb, p.M, p.N_file) : -1;
+                const float xt = winpath ? win[(kLdsD + i) * 64] : load_x(xp, t, p.M, p.N_file);
+                int keep = winpath ? stencil_win<kSign>(win, i, t, xt, mabs, p.D, wa, wb)
+                           : fast  ? stencil_float<kSign>(xp, t, xt, mabs, p.D, wa, wb, p.M, p.N_file)
+                                   : -1;
                 if (keep < 0)
                     keep = stencil_ok(xp, t, detect_value(xt, mean, kSign), mean, kSign, p.D, wa, wb,
                                       p.M, p.N_file);
                 if (keep && cnt < p.cap) my_slots[cnt++] = static_cast<uint16_t>(t - T0);
             }
             prev = cur[kB - 1];
+#pragma unroll
+            for (int k = 0; k < kLdsD; ++k) prv[k] = cur[kB - kLdsD + k];
 #pragma unroll
             for (int i = 0; i < kB; ++i) cur[i] = nxt[i];
         }
