@@ -686,7 +686,7 @@ def main():
     ap.add_argument("--f2f-config", choices=["1", "2"], default="2")
     ap.add_argument("--unfused", action="store_true",
                     help="time clips and PCA as two stages (clip array written, then read)")
-    ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS) + ["pca", "detect"],
+    ap.add_argument("--pmc-pass", choices=sorted(ROOFLINE_KERNELS) + ["pca", "detect", "tcn"],
                     help="for rocprofv3 --pmc runs: one pipeline pass, then a 1 GiB "
                          "elementwise negation (counter calibration) and the named stage, "
                          "three times each")
@@ -730,6 +730,11 @@ def main():
                 fe.whiten_covariance(pipe.filt, row_keep=pipe.rows)
             elif args.pmc_pass == "detect":
                 fe.detect_events(pipe.white, 3.0, 10, -1, num_threads=8)
+            elif args.pmc_pass == "tcn":
+                # the network's window scores (untrained weights: the kernel's
+                # work does not depend on their values)
+                from mountainsort_amd import learned
+                learned.score(pipe.white, learned.init_params(0, device=dev))
             elif args.pmc_pass == "pca":
                 fe.pca_features_from_timeseries(pipe.white, pipe.last_times, pipe.last_offsets,
                                                 pipe.nb_channels, pipe.nb_offsets, 50,
