@@ -723,22 +723,37 @@ int launch_power(dim3 grid, dim3 block, size_t lds, hipStream_t st, const double
 // W[m, n] = sum_l (C[m,l] d_l) C[n,l], l ascending (mult_AB then
 // mult_ABtrans, pca.cpp:648-649), d_l = 1/sqrt(sigma_l) or 0.
 // comp is row-major C[i*M + k]; W is written column-major like the Mda.
+// A 16x16 tile of W per workgroup: the rows of comp for its 16 m and 16 n
+// are staged through LDS 32 components at a time, with d_l computed once per
+// component; every element keeps the sum over l in ascending order and the
+// same operations, ((C[m,l] d_l) C[n,l]) added to the running sum.
 __global__ void __launch_bounds__(256) whitening_assemble_kernel(
     const double* __restrict__ comp, const double* __restrict__ sig, int M,
     double* __restrict__ W) {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= static_cast<int64_t>(M) * M) return;
-    const int n = static_cast<int>(q / M);
-    const int m = static_cast<int>(q - static_cast<int64_t>(n) * M);
-    const double* cm = comp + static_cast<int64_t>(m) * M;
-    const double* cn = comp + static_cast<int64_t>(n) * M;
+    __shared__ double cmT[32][17], cnT[32][17], ds[32];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    const int m = m0 + tx, n = n0 + ty;
     double acc = 0.0;
-    for (int l = 0; l < M; ++l) {
-        const double s = sig[l];
-        const double d = s != 0.0 ? 1.0 / sqrt(s) : 0.0;
-        acc = acc + (cm[l] * d) * cn[l];
+    for (int l0 = 0; l0 < M; l0 += 32) {
+        // stage comp[m0 + r][l0 + c] and comp[n0 + r][l0 + c], r < 16, c < 32
+        for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+            const int r = e >> 5, c = e & 31;
+            const int l = l0 + c;
+            cmT[c][r] = (m0 + r < M && l < M) ? comp[static_cast<int64_t>(m0 + r) * M + l] : 0.0;
+            cnT[c][r] = (n0 + r < M && l < M) ? comp[static_cast<int64_t>(n0 + r) * M + l] : 0.0;
+        }
+        if (threadIdx.x < 32) {
+            const int l = l0 + threadIdx.x;
+            const double sv = l < M ? sig[l] : 0.0;
+            ds[threadIdx.x] = sv != 0.0 ? 1.0 / sqrt(sv) : 0.0;
+        }
+        __syncthreads();
+        const int lc = M - l0 < 32 ? M - l0 : 32;
+        for (int c = 0; c < lc; ++c) acc = acc + (cmT[c][tx] * ds[c]) * cnT[c][ty];
+        __syncthreads();
     }
-    W[m + static_cast<int64_t>(M) * n] = acc;
+    if (m < M && n < M) W[m + static_cast<int64_t>(M) * n] = acc;
 }
 
 // ------------------------------------------------------------------- apply
@@ -1296,8 +1311,8 @@ extern "C" int msamd_whitening_matrix(const double* XXt, int64_t M, double* W, v
                   wp.NP);
         return MSAMD_ERR_TIMEOUT;
     }
-    whitening_assemble_kernel<<<static_cast<unsigned>(ceil_div(M * M, 256)), 256, 0, st>>>(
-        comp, sig, static_cast<int>(M), W);
+    const dim3 ag(static_cast<unsigned>(ceil_div(M, 16)), static_cast<unsigned>(ceil_div(M, 16)));
+    whitening_assemble_kernel<<<ag, 256, 0, st>>>(comp, sig, static_cast<int>(M), W);
     MSAMD_LAUNCH_CHECK();
     return MSAMD_OK;
 }
