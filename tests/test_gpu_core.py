@@ -39,6 +39,8 @@ def _whitened(M, N, seed):
     (64, 100_000, 8, -1, 10, -1, -1, 20_000),
     (64, 100_000, 1, -1, 0, -1, -1, None),
     (3, 45_000, 8, -1, 25, -1, -1, 20_000),
+    (7, 150_000, 8, -1, 10, 777, 140_000, 20_000),  # LDS-window stencil at the range's edges
+    (6, 60_000, 8, -1, 11, -1, -1, 20_000),   # shortest window past the LDS window (loads)
     (5, 60_000, 8, -1, 16, -1, -1, 20_000),   # longest window of the register/float test
     (5, 60_000, 8, 1, 17, -1, -1, 20_000),    # shortest window of the FP64 neighbour walk
 ])
