@@ -225,21 +225,57 @@ __global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float*
     const int nitems = 4 * p.Mp;
     const int plane = p.nB * 64;
     float v[kCovAllItems][8];
+    // Each item's byte offset within a step (rows 8b..8b+7 of channel m);
+    // items past the channels get an offset beyond any step's range.
+    const uint32_t um4 = 4u * static_cast<uint32_t>(p.M);
+    uint32_t vo[kCovAllItems];
+    int vb[kCovAllItems];
+#pragma unroll
+    for (int it = 0; it < kCovAllItems; ++it) {
+        const int q = threadIdx.x + it * kCovAllWaves * 64;
+        const int m = q % p.Mp;
+        const int b = q / p.Mp;
+        const bool ok = q < nitems && m < p.M;
+        vo[it] = ok ? static_cast<uint32_t>(8 * b) * um4 + 4u * static_cast<uint32_t>(m)
+                    : 0x80000000u;
+        vb[it] = ok ? 8 * b : 0;
+    }
+    // The keep words (bit t%32 of word t/32 set when row t is kept) come in
+    // through a buffer load too, issued with the step's samples and applied
+    // when the step is split, after its loads have landed.  Without a bitmap
+    // the range is empty, the load returns 0 and nokeep sets every bit.
+    const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(p.rowkeep), 0,
+        p.rowkeep ? static_cast<int>(4 * ((p.N + 31) >> 5)) : 0, 0x00020000);
+    const uint32_t nokeep = p.rowkeep ? 0u : ~0u;
+    uint32_t kwv = 0;
     auto load = [&](int64_t t0) {
+        // One buffer range per step: the step's rows below te.  A raw buffer
+        // load past the range returns 0.0f, which covers the rows past te and
+        // the items past the channels.
+        const int64_t left = te - t0;
+        const uint32_t rows = left < 32 ? static_cast<uint32_t>(left) : 32u;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(X + t0 * p.M), 0, static_cast<int>(rows * um4), 0x00020000);
         // t0 is a multiple of 32: one keep word covers the step's rows
-        const uint32_t kw = p.rowkeep ? p.rowkeep[t0 >> 5] : ~0u;
+        kwv = __builtin_amdgcn_raw_buffer_load_b32(rk, static_cast<uint32_t>(t0 >> 5) * 4u, 0, 0);
 #pragma unroll
         for (int it = 0; it < kCovAllItems; ++it) {
-            const int q = threadIdx.x + it * kCovAllWaves * 64;
-            const int m = q % p.Mp;
-            const int b = q / p.Mp;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t t = t0 + 8 * b + e;
-                v[it][e] = (q < nitems && m < p.M && t < te && ((kw >> (8 * b + e)) & 1u))
-                               ? __builtin_nontemporal_load(X + t * p.M + m)
-                               : 0.0f;
-            }
+            for (int e = 0; e < 8; ++e)
+                v[it][e] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo[it] + e * um4, 0, 2));
+        }
+    };
+    // rows outside the kept intervals read as zero
+    auto keep = [&]() {
+        const uint32_t kw = kwv | nokeep;
+#pragma unroll
+        for (int it = 0; it < kCovAllItems; ++it) {
+            uint32_t kb = kw >> vb[it];
+            asm volatile("" : "+v"(kb));  // one shift per step, not 8 masks held live
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[it][e] = ((kb >> e) & 1u) ? v[it][e] : 0.0f;
         }
     };
     load(tb);
@@ -247,6 +283,7 @@ __global__ void __launch_bounds__(kCovAllWaves * 64) cov_all_kernel(const float*
     for (int64_t t0 = tb; t0 < te; t0 += 32, buf ^= 1) {
         bf16x8* fh = lds + buf * 2 * plane;
         bf16x8* fl = fh + plane;
+        keep();
 #pragma unroll
         for (int it = 0; it < kCovAllItems; ++it) {
             const int q = threadIdx.x + it * kCovAllWaves * 64;
