@@ -284,21 +284,31 @@ __global__ void __launch_bounds__(kWideP * (R1 > R2 ? R1 : R2)) bp_stage_a_wide(
     // is wholly inside or wholly past the last channel
     constexpr int kCh = kWideP / 4;  // chunks per row
     {
-        // thread q takes rows n1 = q / kCh + k * (blockDim / kCh) of chunk q % kCh:
-        // the row time and the source address advance by fixed steps
+        // thread q takes rows n1 = q / kCh + k * (threads / kCh) of chunk q % kCh:
+        // the row time and the source address advance by fixed steps.  All of
+        // a thread's loads are issued before its first LDS store, so their
+        // latencies overlap instead of adding up.
+        constexpr int kThreads = kWideP * (R1 > R2 ? R1 : R2);  // the launch's block size
+        constexpr int kDn1 = kThreads / kCh;
+        constexpr int kIt = (N1 + kDn1 - 1) / kDn1;
         const int c = tid % kCh;
-        const int dn1 = static_cast<int>(blockDim.x) / kCh;
-        int n1 = tid / kCh;
+        const int n1 = tid / kCh;
         int64_t t = tbase + static_cast<int64_t>(n1) * p.N2 + n2;
-        const int64_t dt = static_cast<int64_t>(dn1) * p.N2;
+        const int64_t dt = static_cast<int64_t>(kDn1) * p.N2;
         const bool ch_ok = ch0 + 8 * c < p.M;
         const int16_t* src = X + (t - p.xt0) * p.M + ch0 + 8 * c;
         const int64_t dsrc = dt * p.M;
-        for (; n1 < N1; n1 += dn1, t += dt, src += dsrc) {
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (ch_ok && t >= 0 && t < p.N) v = *reinterpret_cast<const uint4*>(src);
-            *reinterpret_cast<uint4*>(stg + n1 * kWideP + 4 * c) = v;
+        uint4 rv[kIt];
+#pragma unroll
+        for (int k = 0; k < kIt; ++k, t += dt, src += dsrc) {
+            rv[k] = uint4{0u, 0u, 0u, 0u};
+            if (n1 + k * kDn1 < N1 && ch_ok && t >= 0 && t < p.N)
+                rv[k] = *reinterpret_cast<const uint4*>(src);
         }
+#pragma unroll
+        for (int k = 0; k < kIt; ++k)
+            if (n1 + k * kDn1 < N1)
+                *reinterpret_cast<uint4*>(stg + (n1 + k * kDn1) * kWideP + 4 * c) = rv[k];
     }
     __syncthreads();
     const int pl = tid % kWideP;
