@@ -998,9 +998,28 @@ __global__ void __launch_bounds__(kApplyWaves * 64) apply_kernel(
     {
         const int64_t total = static_cast<int64_t>(p.nColTiles) * p.nK * 64;
         const int64_t base = static_cast<int64_t>(cb) * nfr;
-        for (int q = threadIdx.x; q < nfr; q += blockDim.x) {
-            fh[q] = gfr[base + q];
-            fl[q] = gfr[total + base + q];
+        // four fragments of each piece per thread in flight before their
+        // stores (the 9 rounds at 6 column tiles would otherwise each wait
+        // out a load)
+        constexpr int kB = 4;
+        for (int q0 = threadIdx.x; q0 < nfr; q0 += kB * blockDim.x) {
+            f16x8 h[kB], l[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const int q = q0 + k * static_cast<int>(blockDim.x);
+                if (q < nfr) {
+                    h[k] = gfr[base + q];
+                    l[k] = gfr[total + base + q];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const int q = q0 + k * static_cast<int>(blockDim.x);
+                if (q < nfr) {
+                    fh[q] = h[k];
+                    fl[q] = l[k];
+                }
+            }
         }
     }
     const float sx = __uint_as_float(sc[2]);
