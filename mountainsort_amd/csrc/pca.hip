@@ -812,10 +812,23 @@ __global__ void __launch_bounds__(256) pca_project2_kernel(const float* __restri
     const int ne = (L - e0) < p.pe2 ? (L - e0) : p.pe2;
     float* Cs = sq;            // [R][nf]
     float* Xs = sq + R * nf;   // [ne][R]
-    const float* cg = tb.c2 + tb.c2_off[g];
-    for (int e = threadIdx.x; e < R * nf; e += blockDim.x) Cs[e] = cg[e];
-    const float* xg = reduced + tb.red_off[g] + static_cast<int64_t>(e0) * R;
-    for (int e = threadIdx.x; e < ne * R; e += blockDim.x) Xs[e] = xg[e];
+    // the copies into LDS keep 8 loads per thread in flight before their
+    // stores (a plain load-store loop waits out every load)
+    const auto stage = [&](float* dst, const float* src, int n) {
+        constexpr int kB = 8;
+        const int bd = static_cast<int>(blockDim.x);
+        for (int e0i = threadIdx.x; e0i < n; e0i += kB * bd) {
+            float t8[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k)
+                if (e0i + k * bd < n) t8[k] = src[e0i + k * bd];
+#pragma unroll
+            for (int k = 0; k < kB; ++k)
+                if (e0i + k * bd < n) dst[e0i + k * bd] = t8[k];
+        }
+    };
+    stage(Cs, tb.c2 + tb.c2_off[g], R * nf);
+    stage(Xs, reduced + tb.red_off[g] + static_cast<int64_t>(e0) * R, ne * R);
     __syncthreads();
     for (int o = threadIdx.x; o < ne * nf; o += blockDim.x) {
         const int ev = o / nf;
