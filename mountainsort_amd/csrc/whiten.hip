@@ -348,9 +348,17 @@ __global__ void __launch_bounds__(256) cov_reduce_kernel(const float* __restrict
     const int64_t jj = J * 16 + (e & 15);
     if (ii >= p.M || jj >= p.M) return;
     if (I == J && ii > jj) return;  // diagonal tiles: upper half only, mirrored below
+    // partials summed in split order; 8 loads in flight ahead of their adds
     double s = 0.0;
-    for (int64_t sp = 0; sp < p.nsplit; ++sp)
-        s = s + static_cast<double>(part[(sp * p.nTiles + tile) * 256 + e]);
+    for (int64_t sp0 = 0; sp0 < p.nsplit; sp0 += 8) {
+        float t8[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (sp0 + k < p.nsplit) t8[k] = part[((sp0 + k) * p.nTiles + tile) * 256 + e];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (sp0 + k < p.nsplit) s = s + static_cast<double>(t8[k]);
+    }
     if (normalize && p.N > 1) s = s / static_cast<double>(p.N - 1);
     if (accumulate) s = C[ii * p.M + jj] + s;
     C[ii * p.M + jj] = s;
@@ -885,7 +893,23 @@ __global__ void __launch_bounds__(256) apply_scale_kernel(const double* __restri
                                                           int64_t N, uint32_t* __restrict__ sc) {
     __shared__ double red[2][4];
     double wm = 0.0, cm = 0.0;
-    for (int64_t e = threadIdx.x; e < M * M; e += blockDim.x) wm = fmax(wm, fabs(Wcm[e]));
+    {
+        // 16 loads per thread in flight (the max is exact in any order)
+        constexpr int kB = 16;
+        double wk[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) wk[k] = 0.0;
+        const int64_t bd = blockDim.x;
+        for (int64_t e0 = threadIdx.x; e0 < M * M; e0 += kB * bd) {
+            double v[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) v[k] = Wcm[e0 + k * bd < M * M ? e0 + k * bd : 0];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) wk[k] = fmax(wk[k], fabs(v[k]));  // element 0 again past the end
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) wm = fmax(wm, wk[k]);
+    }
     if (C)
         for (int64_t j = threadIdx.x; j < M; j += blockDim.x) cm = fmax(cm, C[j + M * j]);
     for (int o = 32; o > 0; o >>= 1) {
