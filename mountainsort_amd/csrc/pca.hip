@@ -482,8 +482,17 @@ __global__ void __launch_bounds__(256) pca_gram_reduce_kernel(PcaPlan p, PcaTabl
     const int i0 = tb.gram_first[g];
     const int nch = tb.gram_first[g + 1] - i0;
     const int64_t* offs = kStage == 1 ? tb.gp1_off : tb.gp2_off;
+    // chunk order; 8 chunks' loads in flight ahead of their adds
     double s = 0.0;
-    for (int c = 0; c < nch; ++c) s = s + part[offs[i0 + c] + e];
+    for (int c0 = 0; c0 < nch; c0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (c0 + k < nch) v[k] = part[offs[i0 + c0 + k] + e];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (c0 + k < nch) s = s + v[k];
+    }
     double* out = kStage == 1 ? tb.g1 + tb.g1_off[g] : tb.g2 + tb.g2_off[g];
     out[e] = s;
 }
